@@ -1,0 +1,23 @@
+"""skema_amd -- MI355X-native streaming SketchySVD (update + fixed-rank finalize).
+
+The compute lives in ``libskema_mi355x.so`` (hand-written gfx950 HIP kernels
+behind the C ABI in ``include/skema_mi355x.h``); this package is the
+reference-shaped host layer over it.
+"""
+from ._lib import SkemaError, load  # noqa: F401
+from .sketchy import (  # noqa: F401
+    AlgParams,
+    Context,
+    CrsMatrix,
+    GaussDimRedux,
+    SketchySVD,
+    SparseSignDimRedux,
+    colmajor,
+    default_context,
+    gauss_rbf_window,
+    make_maps,
+    map_shapes,
+    residual_fro,
+)
+
+__version__ = "1.0.0"

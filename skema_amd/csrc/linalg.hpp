@@ -1,0 +1,40 @@
+// FP64 dense linear algebra used by the fixed-rank finalize
+// (reference: linalg::qr / ls / svd in src/Skema_BlasLapack.hpp:71-243).
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace skema {
+
+// Householder QR with LAPACK dgeqr2 conventions (dlarfg: beta =
+// -sign(alpha)*||x||, tau = (beta-alpha)/beta, v(0) = 1).  On return A (m x n,
+// m >= n, leading dimension lda) holds R on and above the diagonal and the
+// reflectors below it; tau has n entries.
+void householder_qr(hipStream_t st, int64_t m, int64_t n, double* A, int64_t lda, double* tau);
+
+// Explicit thin Q (m x n) from the reflectors left in A by householder_qr
+// (dorg2r semantics).  Q must not alias A.
+void householder_q(hipStream_t st, int64_t m, int64_t n, const double* A, int64_t lda,
+                   const double* tau, double* Q, int64_t ldq);
+
+// R (n x n, upper, zero below) from the factored A.
+void extract_r(hipStream_t st, int64_t n, const double* A, int64_t lda, double* R, int64_t ldr);
+
+// B <- T^{-1} B for upper-triangular T (n x n), B (n x nrhs).  This is what
+// dgels('N') reduces to when its input is already upper triangular.
+void trsm_upper(hipStream_t st, int64_t n, int64_t nrhs, const double* T, int64_t ldt, double* B,
+                int64_t ldb);
+
+// B (cols x rows) <- A^T for A (rows x cols).
+void transpose(hipStream_t st, int64_t rows, int64_t cols, const double* A, int64_t lda, double* B,
+               int64_t ldb);
+
+// One-sided Jacobi SVD of a square n x n matrix A (overwritten).  Produces
+// U (n x n), S (n, descending) and V (n x n) with A = U diag(S) V^T.
+// Returns the number of sweeps used.
+int jacobi_svd(hipStream_t st, int64_t n, double* A, int64_t lda, double* U, int64_t ldu, double* S,
+               double* V, int64_t ldv, DevBuf* scratch);
+
+}  // namespace skema

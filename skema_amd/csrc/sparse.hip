@@ -1,0 +1,308 @@
+// Sparse kernels of the sketch path (sparse-sign maps, CSR windows).
+// Bandwidth-bound gathers: consecutive lanes own consecutive output rows of a
+// column-major result, so stores and the dense-operand loads coalesce, and
+// no two threads ever accumulate into the same element.
+#include "sparse.hpp"
+#include "util.hpp"
+
+namespace skema {
+namespace {
+
+unsigned blocks_for(int64_t n, int bs = 256) {
+  int64_t b = cdiv(n, bs);
+  if (b > (int64_t)1 << 30) b = (int64_t)1 << 30;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+__device__ __forceinline__ int64_t lower_bound_col(const int32_t* ci, int64_t lo, int64_t hi, int64_t key) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ci[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void put(const GemmOut& o, int64_t i, int64_t p, double v) {
+  if (p < o.nsplit) {
+    double* c = o.C1 + i + p * o.ldc1;
+    *c = (o.beta1 != 0.0) ? o.alpha1 * v + o.beta1 * *c : o.alpha1 * v;
+  } else {
+    double* c = o.C2 + i + (p - o.nsplit) * o.ldc2;
+    *c = (o.beta2 != 0.0) ? o.alpha2 * v + o.beta2 * *c : o.alpha2 * v;
+  }
+}
+
+__global__ void k_spmm_csr_rows(const int64_t* rp, const int32_t* ci, const double* cv, int64_t r0, int64_t R,
+                                int64_t c0, int64_t c1, bool windowed, double alpha, const double* B, int64_t ldb,
+                                int64_t N, double beta, double* C, int64_t ldc) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= R * N) return;
+  const int64_t p = g % R, j = g / R;
+  int64_t lo = rp[r0 + p], hi = rp[r0 + p + 1];
+  if (windowed) {
+    const int64_t a = lower_bound_col(ci, lo, hi, c0);
+    hi = lower_bound_col(ci, a, hi, c1);
+    lo = a;
+  }
+  const double* b = B + j * ldb - c0;
+  double acc = 0.0;
+  for (int64_t e = lo; e < hi; ++e) acc += cv[e] * b[ci[e]];
+  double* c = C + p + j * ldc;
+  *c = (beta != 0.0) ? alpha * acc + beta * *c : alpha * acc;
+}
+
+__global__ void k_spmm_dense_csrT(int64_t M, const double* B, int64_t ldb, const int64_t* rp, const int32_t* ci,
+                                  const double* cv, int64_t R, GemmOut o) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= M * R) return;
+  const int64_t i = g % M, p = g / M;
+  double acc = 0.0;
+  for (int64_t e = rp[p]; e < rp[p + 1]; ++e) acc += cv[e] * B[i + (int64_t)ci[e] * ldb];
+  put(o, i, p, acc);
+}
+
+// ---- CSR window -> CSC ----------------------------------------------------
+// flag bit 0: rowptr decreasing, bit 1: column index outside [0, cols).
+__global__ void k_validate_csr(const int64_t* rowptr, int64_t rows, const int32_t* col, int64_t cols, int* flag) {
+  const int64_t base = rowptr[0];
+  const int64_t nnz = rowptr[rows] - base;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x)
+    if (rowptr[i + 1] < rowptr[i]) atomicOr(flag, 1);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = col[base + e];
+    if (c < 0 || c >= cols) atomicOr(flag, 2);
+  }
+}
+
+__global__ void k_count_cols(const int64_t* rowptr, int64_t rows, const int32_t* col, int* counts) {
+  const int64_t base = rowptr[0];
+  const int64_t nnz = rowptr[rows] - base;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&counts[col[base + e]], 1);
+}
+
+// Exclusive scan of counts[0..n) into ptr[0..n] by one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan_counts(const int* counts, int64_t n, int64_t* ptr) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t chunk = cdiv(n, 1024);
+  const int64_t lo = t * chunk, hi = (lo + chunk < n) ? lo + chunk : n;
+  int64_t s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += counts[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int64_t v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = part[t] - s;  // exclusive prefix of this chunk
+  for (int64_t i = lo; i < hi; ++i) {
+    ptr[i] = run;
+    run += counts[i];
+  }
+  if (t == 1023) ptr[n] = part[1023];
+}
+
+__global__ void k_fill_csc(const int64_t* rowptr, int64_t rows, const int32_t* col, const double* val, int* cursor,
+                           const int64_t* ptr, int32_t* orow, double* oval) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+    const int32_t c = col[e];
+    const int64_t pos = ptr[c] + atomicAdd(&cursor[c], 1);
+    orow[pos] = (int32_t)i;
+    oval[pos] = val[e];
+  }
+}
+
+// Insertion sort of each column segment by (row, value): makes the order of
+// summation independent of the atomic placement above.
+__global__ void k_sort_segments(const int64_t* ptr, int64_t cols, int32_t* row, double* val) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  const int64_t lo = ptr[c], hi = ptr[c + 1];
+  for (int64_t a = lo + 1; a < hi; ++a) {
+    const int32_t r = row[a];
+    const double v = val[a];
+    int64_t b = a - 1;
+    while (b >= lo && (row[b] > r || (row[b] == r && val[b] > v))) {
+      row[b + 1] = row[b];
+      val[b + 1] = val[b];
+      --b;
+    }
+    row[b + 1] = r;
+    val[b + 1] = v;
+  }
+}
+
+__global__ void k_csc_left_dense(const int64_t* ptr, const int32_t* row, const double* val, int64_t cols,
+                                 const double* M, int64_t ldm, int64_t row0, int64_t k, double eta, double nu,
+                                 double* X, int64_t ldx) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= k * cols) return;
+  const int64_t p = g % k, j = g / k;
+  double acc = 0.0;
+  for (int64_t e = ptr[j]; e < ptr[j + 1]; ++e) acc += val[e] * M[p + (row0 + row[e]) * ldm];
+  double* x = X + p + j * ldx;
+  *x = (nu != 0.0) ? nu * *x + eta * acc : eta * acc;
+}
+
+__global__ void k_csc_left_sparse(const int64_t* ptr, const int32_t* row, const double* val, int64_t cols,
+                                  const int64_t* mrp, const int32_t* mci, const double* mv, int64_t row0, int64_t k,
+                                  double eta, double nu, double* X, int64_t ldx) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double* x = X + j * ldx;
+  if (nu != 1.0)
+    for (int64_t p = 0; p < k; ++p) x[p] = (nu != 0.0) ? nu * x[p] : 0.0;
+  for (int64_t e = ptr[j]; e < ptr[j + 1]; ++e) {
+    const double a = eta * val[e];
+    const int64_t c = row0 + row[e];
+    for (int64_t f = mrp[c]; f < mrp[c + 1]; ++f) x[mci[f]] += a * mv[f];
+  }
+}
+
+__global__ void k_csr_right_dense(const int64_t* rowptr, const int32_t* col, const double* val, int64_t rows,
+                                  const double* M, int64_t ldm, int64_t kk, GemmOut o) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= kk * rows) return;
+  const int64_t p = g % kk, i = g / kk;
+  double acc = 0.0;
+  for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) acc += val[e] * M[p + (int64_t)col[e] * ldm];
+  put(o, i, p, acc);
+}
+
+__global__ void k_csr_right_sparse(const int64_t* rowptr, const int32_t* col, const double* val, int64_t rows,
+                                   const int64_t* mrp, const int32_t* mci, const double* mv, int64_t kk, GemmOut o) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  // beta pass
+  for (int64_t p = 0; p < kk; ++p) {
+    if (p < o.nsplit) {
+      double* c = o.C1 + i + p * o.ldc1;
+      *c = (o.beta1 != 0.0) ? o.beta1 * *c : 0.0;
+    } else {
+      double* c = o.C2 + i + (p - o.nsplit) * o.ldc2;
+      *c = (o.beta2 != 0.0) ? o.beta2 * *c : 0.0;
+    }
+  }
+  for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+    const double a = val[e];
+    const int32_t j = col[e];
+    for (int64_t f = mrp[j]; f < mrp[j + 1]; ++f) {
+      const int64_t p = mci[f];
+      if (p < o.nsplit) o.C1[i + p * o.ldc1] += o.alpha1 * (a * mv[f]);
+      else o.C2[i + (p - o.nsplit) * o.ldc2] += o.alpha2 * (a * mv[f]);
+    }
+  }
+}
+
+}  // namespace
+
+void spmm_csr_rows(hipStream_t st, const Csr& S, int64_t r0, int64_t r1, int64_t c0, int64_t c1, double alpha,
+                   const double* B, int64_t ldb, int64_t N, double beta, double* C, int64_t ldc) {
+  const int64_t R = r1 - r0;
+  SKEMA_REQUIRE(r0 >= 0 && r1 <= S.rows && R >= 0 && c0 >= 0 && c1 <= S.cols && c0 <= c1, kErrShape,
+                "spmm_csr_rows: range outside the map");
+  if (R == 0 || N == 0) return;
+  const bool windowed = !(c0 == 0 && c1 == S.cols);
+  hipLaunchKernelGGL(k_spmm_csr_rows, dim3(blocks_for(R * N)), dim3(256), 0, st, S.rp(), S.ci(), S.v(), r0, R, c0,
+                     c1, windowed, alpha, B, ldb, N, beta, C, ldc);
+  check_launch("k_spmm_csr_rows");
+}
+
+void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, const Csr& S, const GemmOut& out) {
+  if (M == 0 || S.rows == 0) return;
+  hipLaunchKernelGGL(k_spmm_dense_csrT, dim3(blocks_for(M * S.rows)), dim3(256), 0, st, M, B, ldb, S.rp(), S.ci(),
+                     S.v(), S.rows, out);
+  check_launch("k_spmm_dense_csrT");
+}
+
+CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val,
+                            int64_t rows, int64_t cols, DevBuf& scratch) {
+  int64_t ends[2] = {0, 0};
+  SKEMA_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SKEMA_HIP(hipMemcpyAsync(&ends[1], rowptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  SKEMA_HIP(hipStreamSynchronize(st));
+  const int64_t nnz = ends[1] - ends[0];
+  SKEMA_REQUIRE(nnz >= 0 && nnz < ((int64_t)1 << 31), kErrArg, "CSR window: bad rowptr range");
+  {
+    // Reject malformed windows before any kernel indexes with them.
+    DevBuf flag(sizeof(int));
+    fill_i32(st, flag.as<int>(), 1, 0);
+    const int64_t work = nnz > rows ? nnz : rows;
+    const unsigned nb = blocks_for(work) > 65536 ? 65536 : blocks_for(work);
+    hipLaunchKernelGGL(k_validate_csr, dim3(nb), dim3(256), 0, st, rowptr, rows, col, cols, flag.as<int>());
+    check_launch("k_validate_csr");
+    int h = 0;
+    SKEMA_HIP(hipMemcpyAsync(&h, flag.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
+    SKEMA_HIP(hipStreamSynchronize(st));
+    SKEMA_REQUIRE(!(h & 1), kErrArg, "CSR window: rowptr is not non-decreasing");
+    SKEMA_REQUIRE(!(h & 2), kErrArg, "CSR window: column index outside [0, n)");
+  }
+  const size_t o_cnt = 0;
+  const size_t o_ptr = align_up((size_t)cols * sizeof(int) * 2, 256);
+  const size_t o_row = o_ptr + align_up((size_t)(cols + 1) * sizeof(int64_t), 256);
+  const size_t o_val = o_row + align_up((size_t)nnz * sizeof(int32_t), 256);
+  const size_t total = o_val + align_up((size_t)nnz * sizeof(double), 256);
+  scratch.reserve(total);
+  char* base = scratch.as<char>();
+  int* counts = reinterpret_cast<int*>(base + o_cnt);
+  int* cursor = counts + cols;
+  CscWindow w;
+  w.rows = rows;
+  w.cols = cols;
+  w.nnz = nnz;
+  w.ptr = reinterpret_cast<int64_t*>(base + o_ptr);
+  w.row = reinterpret_cast<int32_t*>(base + o_row);
+  w.val = reinterpret_cast<double*>(base + o_val);
+  fill_i32(st, counts, 2 * cols, 0);
+  if (nnz > 0) {
+    const unsigned nb = blocks_for(nnz) > 65536 ? 65536 : blocks_for(nnz);
+    hipLaunchKernelGGL(k_count_cols, dim3(nb), dim3(256), 0, st, rowptr, rows, col, counts);
+  }
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, cols, w.ptr);
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_fill_csc, dim3(blocks_for(rows)), dim3(256), 0, st, rowptr, rows, col, val, cursor, w.ptr,
+                       w.row, w.val);
+    hipLaunchKernelGGL(k_sort_segments, dim3(blocks_for(cols)), dim3(256), 0, st, w.ptr, cols, w.row, w.val);
+  }
+  check_launch("csr_window_to_csc");
+  return w;
+}
+
+void csc_left_dense(hipStream_t st, const CscWindow& H, const double* M, int64_t ldm, int64_t row0, int64_t k,
+                    double eta, double nu, double* X, int64_t ldx) {
+  if (k == 0 || H.cols == 0) return;
+  hipLaunchKernelGGL(k_csc_left_dense, dim3(blocks_for(k * H.cols)), dim3(256), 0, st, H.ptr, H.row, H.val, H.cols,
+                     M, ldm, row0, k, eta, nu, X, ldx);
+  check_launch("k_csc_left_dense");
+}
+
+void csc_left_sparse(hipStream_t st, const CscWindow& H, const Csr& Mt, int64_t row0, int64_t k, double eta,
+                     double nu, double* X, int64_t ldx) {
+  if (k == 0 || H.cols == 0) return;
+  hipLaunchKernelGGL(k_csc_left_sparse, dim3(blocks_for(H.cols)), dim3(256), 0, st, H.ptr, H.row, H.val, H.cols,
+                     Mt.rp(), Mt.ci(), Mt.v(), row0, k, eta, nu, X, ldx);
+  check_launch("k_csc_left_sparse");
+}
+
+void csr_right_dense(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val, int64_t rows,
+                     const double* M, int64_t ldm, int64_t kk, const GemmOut& out) {
+  if (rows == 0 || kk == 0) return;
+  hipLaunchKernelGGL(k_csr_right_dense, dim3(blocks_for(kk * rows)), dim3(256), 0, st, rowptr, col, val, rows, M,
+                     ldm, kk, out);
+  check_launch("k_csr_right_dense");
+}
+
+void csr_right_sparse(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val, int64_t rows,
+                      const Csr& Mt, int64_t kk, const GemmOut& out) {
+  if (rows == 0 || kk == 0) return;
+  hipLaunchKernelGGL(k_csr_right_sparse, dim3(blocks_for(rows)), dim3(256), 0, st, rowptr, col, val, rows, Mt.rp(),
+                     Mt.ci(), Mt.v(), kk, out);
+  check_launch("k_csr_right_sparse");
+}
+
+}  // namespace skema

@@ -1,0 +1,340 @@
+"""Pins the CPU oracle (CPU only, no GPU).
+
+What fixes each oracle function:
+  * the reference's own fixtures test/data/*_svals.txt (exact-recovery regime,
+    k = min(m, n)) for the whole path, dense and CSR, both map families;
+  * closed forms computed by an independent route (logical maps densified
+    and multiplied directly) for all four update() forms, the nu/eta
+    recurrence and window-size invariance;
+  * brute-force index loops on tiny inputs;
+  * LAPACK conventions: Q^T Q = I, QR = A, dlarfg's sign rule, dgels on a
+    triangular matrix = triangular solve, SVD invariants;
+  * scipy.spatial.distance for the RBF window;
+  * planted mistakes: mutated copies of oracle functions must fail the checks.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+from scipy.spatial.distance import cdist
+
+from oracle import skema_oracle as O
+
+
+def load(golden_dir, name):
+    return np.loadtxt(f"{golden_dir}/{name}")
+
+
+# ---------------------------------------------------------------- fixtures ---
+@pytest.mark.parametrize("model", ["gauss", "sparse-sign"])
+@pytest.mark.parametrize("issparse", [False, True])
+@pytest.mark.parametrize("window", [1, 5, 7, 32])
+def test_rectangular_fixture_exact_recovery(golden_dir, model, issparse, window):
+    A = load(golden_dir, "rectangular_dense.txt")
+    sv = load(golden_dir, "rectangular_svals.txt")
+    p = O.AlgParams(matrix_m=32, matrix_n=14, rank=10, sketch_range=14, sketch_core=29, window=window,
+                    dim_redux=model, issparse=issparse)
+    sk = O.SketchySVD(p)
+    sk.linear_update(sp.csr_matrix(A) if issparse else A)
+    U, S, V = sk.low_rank_approx()
+    assert np.max(np.abs(S - sv[:10])) <= 1e-13 * sv[0]
+    assert O.rel_err((U * S) @ V.T, A) <= 1e-13
+
+
+def test_sparse_fixture_equals_dense_fixture(golden_dir):
+    """rectangular_sparse.mtx (MatrixMarket coordinate) is the dense fixture."""
+    from scipy.io import mmread
+
+    A = load(golden_dir, "rectangular_dense.txt")
+    M = mmread(f"{golden_dir}/rectangular_sparse.mtx").toarray()
+    # the .txt copy carries 15 significant digits, the .mtx 17
+    assert np.allclose(M, A, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("model", ["gauss", "sparse-sign"])
+def test_symmetric_fixture_exact_recovery(golden_dir, model):
+    A = load(golden_dir, "symmetric_dense.txt")
+    sv = load(golden_dir, "symmetric_svals.txt")
+    p = O.AlgParams(matrix_m=27, matrix_n=27, rank=27, sketch_range=27, sketch_core=55, window=5, dim_redux=model)
+    sk = O.SketchySVD(p)
+    sk.linear_update(A)
+    U, S, V = sk.low_rank_approx()
+    assert np.max(np.abs(S - sv)) <= 1e-13 * sv[0]
+
+
+def test_symmetric_bin_fixture(golden_dir):
+    """symmetric_sparse.bin: Kokkos-Kernels binary CRS (int32 nrows, uint64
+    nnz, uint64 rowptr[n+1], int32 col[nnz], f64 val[nnz])."""
+    raw = open(f"{golden_dir}/symmetric_sparse.bin", "rb").read()
+    n = int(np.frombuffer(raw, np.int32, 1, 0)[0])
+    nnz = int(np.frombuffer(raw, np.uint64, 1, 4)[0])
+    rp = np.frombuffer(raw, np.uint64, n + 1, 12).astype(np.int64)
+    off = 12 + 8 * (n + 1)
+    col = np.frombuffer(raw, np.int32, nnz, off)
+    val = np.frombuffer(raw, np.float64, nnz, off + 4 * nnz)
+    M = sp.csr_matrix((val, col, rp), shape=(n, n)).toarray()
+    assert np.allclose(M, load(golden_dir, "symmetric_dense.txt"), rtol=1e-14, atol=1e-9)
+
+
+@pytest.mark.parametrize("window", [4, 25])
+def test_kernel_fixture_exact_recovery(golden_dir, window):
+    """kernel_data.txt with gamma = 1 (the --gamma default) -> kernel_svals.txt."""
+    X = load(golden_dir, "kernel_data.txt")
+    sv = load(golden_dir, "kernel_svals.txt")
+    p = O.AlgParams(matrix_m=25, matrix_n=25, rank=10, sketch_range=25, sketch_core=51, window=window,
+                    kernel_func="gaussrbf", kernel_gamma=1.0)
+    sk = O.SketchySVD(p)
+    sk.linear_update(X)
+    U, S, V = sk.low_rank_approx()
+    assert np.max(np.abs(S - sv[:10])) <= 1e-13 * sv[0]
+
+
+# ------------------------------------------------------------ closed forms ---
+def closed_form_sketch(A, maps, nu=1.0, eta=1.0, window=None):
+    """X, Y, Z from densified logical maps, windowed recurrence written out."""
+    Ad = A.toarray() if sp.issparse(A) else A
+    m = Ad.shape[0]
+    Ups, Om = O.logical_map(maps["Upsilon"]), O.logical_map(maps["Omega"])
+    Ph, Ps = O.logical_map(maps["Phi"]), O.logical_map(maps["Psi"])
+    w = window or m
+    X = np.zeros((Ups.shape[0], Ad.shape[1]))
+    Y = np.zeros((m, Om.shape[0]))
+    Z = np.zeros((Ph.shape[0], Ph.shape[0]))
+    for r0 in range(0, m, w):
+        r1 = min(m, r0 + w)
+        Aw = Ad[r0:r1]
+        X = nu * X + eta * (Ups[:, r0:r1] @ Aw)
+        Y[r0:r1] = nu * Y[r0:r1] + eta * (Aw @ Om.T)
+        Z = nu * Z + eta * (Ph[:, r0:r1] @ (Aw @ Ps.T))
+    return X, Y, Z
+
+
+FORMS = [("gauss", False), ("sparse-sign", False), ("gauss", True), ("sparse-sign", True)]
+
+
+@pytest.mark.parametrize("model,issparse", FORMS)
+@pytest.mark.parametrize("conv", ["rows", "cols"])
+def test_update_forms_match_closed_form(model, issparse, conv):
+    rng = np.random.default_rng(0)
+    m, n = 61, 23
+    A = rng.standard_normal((m, n))
+    if issparse:
+        A[rng.random((m, n)) < 0.7] = 0.0
+        Ain = sp.csr_matrix(A)
+    else:
+        Ain = A
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=3, window=9, dim_redux=model, issparse=issparse,
+                    sketch_nu=0.7, sketch_eta=1.9)
+    sk = O.SketchySVD(p, sparse_convention=conv)
+    sk.linear_update(Ain)
+    X, Y, Z = closed_form_sketch(A, {"Upsilon": sk.Upsilon, "Omega": sk.Omega, "Phi": sk.Phi, "Psi": sk.Psi},
+                                 nu=0.7, eta=1.9, window=9)
+    assert O.rel_err(sk.X, X) <= 1e-13
+    assert O.rel_err(sk.Y, Y) <= 1e-13
+    assert O.rel_err(sk.Z, Z) <= 1e-13
+
+
+def test_map_orientation_follows_reference():
+    """Stored shapes / init_transposed of src/Skema_SketchySVD.cpp:30-95."""
+    p = O.AlgParams(matrix_m=50, matrix_n=20, rank=2)
+    k, s = O.sketch_sizes(p)
+    assert (k, s) == (9, 19)
+    assert O.map_shapes(p)["Upsilon"] == ((9, 50), False)
+    p.issparse, p.dim_redux = True, "gauss"
+    sh = O.map_shapes(p)
+    assert sh["Upsilon"] == ((50, 9), True) and sh["Omega"] == ((20, 9), True)
+    assert sh["Phi"] == ((50, 19), True) and sh["Psi"] == ((20, 19), True)
+    p.dim_redux = "sparse-sign"
+    sh = O.map_shapes(p)
+    assert sh["Upsilon"] == ((9, 50), False) and sh["Omega"] == ((20, 9), True)
+    assert sh["Phi"] == ((19, 50), False) and sh["Psi"] == ((20, 19), True)
+    p2 = O.AlgParams(rank=3, sketch_range=7, sketch_core=5)
+    assert O.sketch_sizes(p2) == (7, 5)
+
+
+@pytest.mark.parametrize("model", ["gauss", "sparse-sign"])
+def test_window_size_invariance(model):
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((40, 17))
+    ref = None
+    for w in [1, 3, 16, 40, 100]:
+        p = O.AlgParams(matrix_m=40, matrix_n=17, rank=2, window=w, dim_redux=model)
+        sk = O.SketchySVD(p)
+        sk.linear_update(A)
+        if ref is None:
+            ref = (sk.X, sk.Y, sk.Z)
+        else:
+            for a, b in zip((sk.X, sk.Y, sk.Z), ref):
+                assert O.rel_err(a, b) <= 1e-14
+
+
+def test_brute_force_tiny():
+    """Index loops on a 6 x 5 matrix, two windows, nu/eta != 1."""
+    rng = np.random.default_rng(2)
+    m, n = 6, 5
+    A = rng.standard_normal((m, n))
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=1, sketch_range=2, sketch_core=3, window=4,
+                    sketch_nu=0.5, sketch_eta=2.0)
+    sk = O.SketchySVD(p)
+    sk.linear_update(A)
+    Ups, Om, Ph, Ps = (sk.Upsilon.data, sk.Omega.data, sk.Phi.data, sk.Psi.data)
+    X = [[0.0] * n for _ in range(2)]
+    Y = [[0.0] * 2 for _ in range(m)]
+    Z = [[0.0] * 3 for _ in range(3)]
+    for r0, r1 in [(0, 4), (4, 6)]:
+        for pp in range(2):
+            for j in range(n):
+                X[pp][j] = 0.5 * X[pp][j] + 2.0 * sum(Ups[pp][i] * A[i][j] for i in range(r0, r1))
+        for i in range(r0, r1):
+            for pp in range(2):
+                Y[i][pp] = 0.5 * Y[i][pp] + 2.0 * sum(A[i][j] * Om[pp][j] for j in range(n))
+        for a in range(3):
+            for b in range(3):
+                zab = 0.0
+                for i in range(r0, r1):
+                    for j in range(n):
+                        zab += Ph[a][i] * A[i][j] * Ps[b][j]
+                Z[a][b] = 0.5 * Z[a][b] + 2.0 * zab
+    assert O.rel_err(sk.X, np.array(X)) <= 1e-14
+    assert O.rel_err(sk.Y, np.array(Y)) <= 1e-14
+    assert O.rel_err(sk.Z, np.array(Z)) <= 1e-14
+
+
+# ------------------------------------------------------- LAPACK conventions ---
+def test_qr_conventions():
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((40, 7))
+    Q, R = O.qr_with_r(A.copy())
+    assert np.allclose(Q.T @ Q, np.eye(7), atol=1e-14)
+    assert O.rel_err(Q @ R, A) <= 1e-14
+    assert np.allclose(np.tril(R, -1), 0.0)
+    # dlarfg: beta = -sign(alpha) * ||x||  ->  R[0,0] = -sign(A[0,0]) ||A[:,0]||
+    assert R[0, 0] == pytest.approx(-math.copysign(np.linalg.norm(A[:, 0]), A[0, 0]), rel=1e-14)
+    assert O.rel_err(O.qr_inplace(A.copy()), Q) <= 1e-15
+    # rank-deficient input still gives an orthonormal Q
+    B = np.c_[A[:, :3], A[:, :3] @ rng.standard_normal((3, 4))]
+    Qb = O.qr_inplace(B)
+    assert np.allclose(Qb.T @ Qb, np.eye(7), atol=1e-13)
+
+
+def test_ls_on_triangular_is_back_substitution():
+    rng = np.random.default_rng(4)
+    T = np.triu(rng.standard_normal((9, 9))) + 5 * np.eye(9)
+    B = rng.standard_normal((9, 9))
+    assert O.rel_err(O.ls(T, B), sla.solve_triangular(T, B, lower=False)) <= 1e-14
+
+
+def test_svd_invariants():
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((12, 12))
+    U, S, Vt = O.svd(A)
+    assert np.all(np.diff(S) <= 0)
+    assert O.rel_err((U * S) @ Vt, A) <= 1e-14
+    assert np.allclose(U.T @ U, np.eye(12), atol=1e-14)
+    assert np.allclose(S, np.linalg.svd(A, compute_uv=False), rtol=1e-14)
+
+
+# -------------------------------------------------------------------- RBF ---
+def test_rbf_window_closed_form():
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((30, 4))
+    gamma = 0.37
+    K = O.gauss_rbf_compute(X[10:17], X, gamma, (10, 17))
+    ref = np.exp(-gamma * cdist(X[10:17], X, "sqeuclidean"))
+    assert np.max(np.abs(K - ref)) <= 1e-15
+    for jj in range(7):
+        assert K[jj, 10 + jj] == 1.0
+    full = O.gauss_rbf_compute(X, X, gamma, (0, 30))
+    assert np.array_equal(full, full.T) or np.max(np.abs(full - full.T)) <= 1e-16
+    assert np.all(O.gauss_rbf_compute(X[:3], X, 0.0, (0, 3)) == 1.0)
+
+
+# ------------------------------------------------------------- generators ---
+def test_gauss_map_moments_and_uniform_bounds():
+    G = O.gauss_map(100, 2000, 0)
+    assert abs(G.mean()) < 0.01 and abs(G.var() - 1.0) < 0.01
+    assert np.array_equal(G, O.gauss_map(100, 2000, 0))
+    assert not np.array_equal(G, O.gauss_map(100, 2000, 1))
+    U = O.gauss_map(50, 40, 3, "uniform")
+    a = math.sqrt(2 * math.log(50 * 40))
+    assert np.all(np.abs(U) < a) and U.max() > 0.9 * a and U.min() < -0.9 * a
+
+
+def test_sparse_sign_conventions():
+    R = O.sparse_sign_rows(40, 300, 1)
+    assert np.all(np.diff(R.indptr) == 8)
+    assert set(np.unique(R.data)) <= {-1.0, 1.0}
+    for i in range(40):
+        row = R.indices[R.indptr[i]:R.indptr[i + 1]]
+        assert np.all(np.diff(row) >= 0)
+    Cc = O.sparse_sign_cols(101, 500, 2).tocsc()
+    assert np.all(np.diff(Cc.indptr) == 8)
+    for j in range(500):
+        rows = Cc.indices[Cc.indptr[j]:Cc.indptr[j + 1]]
+        assert len(set(rows)) == 8
+    assert np.allclose(np.abs(Cc.data), 1 / math.sqrt(8))
+    assert abs(np.mean(Cc.data > 0) - 0.5) < 0.05
+    small = O.sparse_sign_rows(5, 3, 0)  # zeta = max(2, min(ncol, 8)) = 3
+    assert np.all(np.diff(small.indptr) == 3)
+
+
+# ------------------------------------------------------- planted mistakes ---
+def _mut_ignore_transposed_flag(monkeypatch):
+    orig = O.GaussDimRedux.lmap
+
+    def lmap(self, alpha, B, beta, transA="N", transB="N", idx=(0, 0)):
+        if sp.issparse(B):
+            return orig(self, alpha, B, beta, transA, transB, idx)
+        return alpha * (O._op(self.data, transA) @ O._op(B, transB))  # flip dropped
+
+    monkeypatch.setattr(O.GaussDimRedux, "lmap", lmap)
+
+
+def _mut_axpy_swapped(monkeypatch):
+    def axpy(eta, Y, nu, A, idx=(0, 0)):  # scales the new term instead of the state
+        if idx[0] == idx[1]:
+            Y[...] = nu * Y + eta * A
+        else:
+            Y[idx[0]:idx[1], :] = nu * Y[idx[0]:idx[1], :] + eta * A
+
+    monkeypatch.setattr(O.SketchySVD, "axpy", staticmethod(axpy))
+
+
+def _mut_window_off_by_one(monkeypatch):
+    orig = O.Window.get
+
+    def get(self, A, idx):
+        return orig(self, A, (idx[0], idx[1]) if idx[0] == 0 else (idx[0] - 1, idx[1] - 1))
+
+    monkeypatch.setattr(O.Window, "get", get)
+
+
+def _mut_rbf_sign(monkeypatch):
+    def compute(X, Y, gamma, offsets):
+        return np.exp(+gamma * cdist(X, Y, "sqeuclidean") * 0.0 - gamma * cdist(X, Y, "euclidean"))
+
+    monkeypatch.setattr(O, "gauss_rbf_compute", compute)
+
+
+def _mut_qr_no_sign_rule(monkeypatch):
+    def qr_with_r(Q):
+        q, r = np.linalg.qr(Q)
+        d = np.sign(np.diag(r))
+        return q * d, (r.T * d).T  # positive-diagonal convention
+
+    monkeypatch.setattr(O, "qr_with_r", qr_with_r)
+
+
+@pytest.mark.parametrize("mutant,check", [
+    (_mut_ignore_transposed_flag, lambda gd: test_rectangular_fixture_exact_recovery(gd, "gauss", True, 5)),
+    (_mut_axpy_swapped, lambda gd: test_update_forms_match_closed_form("gauss", False, "rows")),
+    (_mut_window_off_by_one, lambda gd: test_update_forms_match_closed_form("gauss", False, "rows")),
+    (_mut_rbf_sign, lambda gd: test_kernel_fixture_exact_recovery(gd, 4)),
+    (_mut_qr_no_sign_rule, lambda gd: test_qr_conventions()),
+])
+def test_planted_mistakes_are_caught(monkeypatch, golden_dir, mutant, check):
+    mutant(monkeypatch)
+    with pytest.raises((AssertionError, ValueError)):
+        check(golden_dir)
