@@ -429,6 +429,19 @@ class SketchySVD:
             z = self.Psi.rmap(one, w, zero, "N", "N")
         return x, y, z
 
+    def init_state(self):
+        """Zeroed X, Y, Z (src/Skema_SketchySVD.cpp:131-133)."""
+        self.X = np.zeros((self.range, self.ncol))
+        self.Y = np.zeros((self.nrow, self.range))
+        self.Z = np.zeros((self.core, self.core))
+
+    def update_window(self, H, idx):
+        """One pass of the window loop body (:179-189) on a given window H."""
+        x, y, z = self.update(H, idx)
+        self.axpy(self.nu, self.X, self.eta, x)
+        self.axpy(self.nu, self.Z, self.eta, z)
+        self.axpy(self.nu, self.Y, self.eta, y, idx)
+
     def linear_update(self, A, windows: Optional[int] = None):
         """src/Skema_SketchySVD.cpp:119-246.  ``windows`` (oracle-only) stops
         after that many windows -- used to time a bounded CPU sample."""

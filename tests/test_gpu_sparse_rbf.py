@@ -1,0 +1,197 @@
+"""GPU parity of the CSR-input forms and of the RBF window generator."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import skema_oracle as O
+from tests.gpu_helpers import TOL, compare_usv, dev, gpu_maps, host, to_gpu_params
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import skema_amd
+
+    skema_amd.load()
+    return skema_amd
+
+
+def _sparse(m, n, density, seed):
+    rng = np.random.default_rng(seed)
+    A = sp.random(m, n, density=density, random_state=rng, data_rvs=rng.standard_normal, format="csr")
+    A.sort_indices()
+    return A
+
+
+def _omaps(osk):
+    return {"Upsilon": osk.Upsilon, "Omega": osk.Omega, "Phi": osk.Phi, "Psi": osk.Psi}
+
+
+@pytest.mark.parametrize("model,conv", [("gauss", "rows"), ("sparse-sign", "rows"), ("sparse-sign", "cols")])
+@pytest.mark.parametrize("window,nu,eta", [(37, 1.0, 1.0), (400, 0.8, 1.2), (64, 0.5, 2.0)])
+def test_csr_sketch_parity(S, model, conv, window, nu, eta):
+    print(f"case csr sketch {model}/{conv} w={window} nu={nu} eta={eta}", flush=True)
+    m, n = 400, 150
+    A = _sparse(m, n, 0.05, 1)
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=5, window=window, dim_redux=model, issparse=True,
+                    sketch_nu=nu, sketch_eta=eta)
+    osk = O.SketchySVD(p, sparse_convention=conv)
+    osk.linear_update(A)
+    Xr, Yr, Zr = osk.X.copy(), osk.Y.copy(), osk.Z.copy()
+    Ur, Sr, Vr = osk.low_rank_approx()
+    g = S.SketchySVD(to_gpu_params(p), maps=gpu_maps(_omaps(osk)))
+    g.linear_update(S.CrsMatrix.from_scipy(A))
+    X, Y, Z = g.state()
+    assert O.rel_err(X, Xr) <= TOL
+    assert O.rel_err(Y, Yr) <= TOL
+    assert O.rel_err(Z, Zr) <= TOL
+    U, Sv, V = g.low_rank_approx()
+    compare_usv(U, Sv, V, Ur, Sr, Vr)
+
+
+def test_csr_fixture_exact_recovery(S, golden_dir):
+    """rectangular_sparse.mtx through the CSR path (maps stored as the
+    reference stores them for sparse input)."""
+    from scipy.io import mmread
+
+    print("case csr fixture", flush=True)
+    A = mmread(f"{golden_dir}/rectangular_sparse.mtx").tocsr()
+    sv = np.loadtxt(f"{golden_dir}/rectangular_svals.txt")
+    for model in ["gauss", "sparse-sign"]:
+        p = S.AlgParams(matrix_m=32, matrix_n=14, rank=10, sketch_range=14, sketch_core=29, window=6,
+                        dim_redux=model, issparse=True)
+        g = S.SketchySVD(p)
+        g.linear_update(S.CrsMatrix.from_scipy(A))
+        U, Sv, V = g.low_rank_approx()
+        assert np.max(np.abs(Sv - sv[:10])) <= 1e-13 * sv[0], model
+
+
+def test_csr_per_window_update(S):
+    print("case csr per-window", flush=True)
+    m, n = 300, 90
+    A = _sparse(m, n, 0.1, 2)
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=3, window=70, issparse=True)
+    osk = O.SketchySVD(p)
+    osk.linear_update(A)
+    Ad = S.CrsMatrix.from_scipy(A)
+    g = S.SketchySVD(to_gpu_params(p), maps=gpu_maps(_omaps(osk)))
+    for r0 in range(0, m, 70):
+        g.update(Ad, (r0, min(m, r0 + 70)))
+    X, Y, Z = g.state()
+    assert O.rel_err(X, osk.X) <= TOL and O.rel_err(Y, osk.Y) <= TOL and O.rel_err(Z, osk.Z) <= TOL
+
+
+def test_csr_empty_rows_and_columns(S):
+    """Ragged input: empty rows, an empty window, empty columns."""
+    print("case csr ragged", flush=True)
+    m, n = 120, 40
+    A = _sparse(m, n, 0.05, 3).tolil()
+    A[10:50, :] = 0.0
+    A[:, 5:9] = 0.0
+    A = A.tocsr()
+    A.eliminate_zeros()
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=3, window=20, issparse=True, dim_redux="sparse-sign")
+    osk = O.SketchySVD(p)
+    osk.linear_update(A)
+    g = S.SketchySVD(to_gpu_params(p), maps=gpu_maps(_omaps(osk)))
+    g.linear_update(S.CrsMatrix.from_scipy(A))
+    X, Y, Z = g.state()
+    assert O.rel_err(X, osk.X) <= TOL and O.rel_err(Y, osk.Y) <= TOL and O.rel_err(Z, osk.Z) <= TOL
+
+
+def test_csr_bad_column_index_is_rejected(S):
+    """A column index >= n is caught by the validation pass (an error, not a
+    fault): the pass only reads the col entries rowptr addresses."""
+    print("case csr bad column", flush=True)
+    m, n = 20, 10
+    A = _sparse(m, n, 0.3, 4)
+    Ad = S.CrsMatrix.from_scipy(A)
+    Ad.col[3] = n + 5
+    p = S.AlgParams(matrix_m=m, matrix_n=n, rank=2, window=5, issparse=True)
+    g = S.SketchySVD(p)
+    with pytest.raises(S.SkemaError) as e:
+        g.linear_update(Ad)
+    assert "column index" in str(e.value)
+
+
+def test_sparse_sign_dense_input_cols_convention(S):
+    print("case dense x sparse-sign cols", flush=True)
+    m, n = 256, 120
+    A = np.random.default_rng(5).standard_normal((m, n))
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=4, window=50, dim_redux="sparse-sign")
+    osk = O.SketchySVD(p, sparse_convention="cols")
+    osk.linear_update(A)
+    Ur, Sr, Vr = osk.low_rank_approx()
+    g = S.SketchySVD(to_gpu_params(p), maps=gpu_maps(_omaps(osk)))
+    g.linear_update(dev(A))
+    X, Y, Z = g.state()
+    assert O.rel_err(X, osk.X) <= TOL and O.rel_err(Y, osk.Y) <= TOL and O.rel_err(Z, osk.Z) <= TOL
+    U, Sv, V = g.low_rank_approx()
+    compare_usv(U, Sv, V, Ur, Sr, Vr)
+
+
+@pytest.mark.parametrize("transposed", [False, True])
+def test_sparse_lmap_rmap(S, transposed):
+    print(f"case sparse lmap/rmap transposed={transposed}", flush=True)
+    rng = np.random.default_rng(6)
+    nrow, ncol = 31, 200
+    D = O.sparse_sign_rows(nrow, ncol, 3)
+    om = O.SparseSignDimRedux(D, transposed)
+    gm = S.SparseSignDimRedux(nrow, ncol, init_transposed=transposed, csr=(D.indptr, D.indices, D.data))
+    for tA in ["N", "T"]:
+        eff = tA if not transposed else ("T" if tA == "N" else "N")
+        for idx in [(0, 0), (17, 140)]:
+            wc = idx[1] - idx[0] if idx[0] != idx[1] else ncol
+            B = rng.standard_normal((wc if eff == "N" else nrow, 9))
+            ref = om.lmap(0.75, B, 0.0, tA, "N", idx)
+            got = host(gm.lmap(0.75, dev(B), 0.0, tA, "N", idx))
+            assert O.rel_err(got, ref) <= TOL
+    for tB in ["N", "T"]:
+        A = rng.standard_normal((13, ncol if tB == "T" else nrow))
+        ref = om.rmap(1.25, A, 0.0, "N", tB)
+        got = host(gm.rmap(1.25, dev(A), 0.0, "N", tB))
+        assert O.rel_err(got, ref) <= TOL
+
+
+# ------------------------------------------------------------------- RBF ---
+def test_rbf_window_parity(S):
+    print("case rbf window", flush=True)
+    X = np.random.default_rng(7).standard_normal((333, 7))
+    Xd = dev(X)
+    for idx in [(0, 64), (100, 171), (332, 333)]:
+        ref = O.gauss_rbf_compute(X[idx[0]:idx[1]], X, 0.3, idx)
+        got = host(S.gauss_rbf_window(Xd, 0.3, idx))
+        assert np.max(np.abs(got - ref)) <= 1e-15
+
+
+@pytest.mark.parametrize("window", [4, 25])
+def test_kernel_fixture_exact_recovery(S, golden_dir, window):
+    print(f"case kernel fixture w={window}", flush=True)
+    X = np.loadtxt(f"{golden_dir}/kernel_data.txt")
+    sv = np.loadtxt(f"{golden_dir}/kernel_svals.txt")
+    p = S.AlgParams(matrix_m=25, matrix_n=25, rank=10, sketch_range=25, sketch_core=51, window=window,
+                    kernel_func="gaussrbf", kernel_gamma=1.0, nfeat=2)
+    g = S.SketchySVD(p)
+    g.linear_update(dev(X))
+    U, Sv, V = g.low_rank_approx()
+    assert np.max(np.abs(Sv - sv[:10])) <= 1e-13 * sv[0]
+
+
+def test_rbf_sketch_parity(S):
+    print("case rbf sketch", flush=True)
+    rng = np.random.default_rng(8)
+    n, d = 500, 16
+    X = rng.standard_normal((n, d)) + 2.0 * rng.standard_normal((5, d))[rng.integers(0, 5, n)]
+    p = O.AlgParams(matrix_m=n, matrix_n=n, rank=6, window=96, kernel_func="gaussrbf", kernel_gamma=1.0 / d)
+    osk = O.SketchySVD(p)
+    osk.linear_update(X)
+    Xr, Yr, Zr = osk.X.copy(), osk.Y.copy(), osk.Z.copy()
+    Ur, Sr, Vr = osk.low_rank_approx()
+    g = S.SketchySVD(to_gpu_params(p, nfeat=d), maps=gpu_maps(_omaps(osk)))
+    g.linear_update(dev(X))
+    Xg, Yg, Zg = g.state()
+    assert O.rel_err(Xg, Xr) <= TOL and O.rel_err(Yg, Yr) <= TOL and O.rel_err(Zg, Zr) <= TOL
+    U, Sv, V = g.low_rank_approx()
+    compare_usv(U, Sv, V, Ur, Sr, Vr)

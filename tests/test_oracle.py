@@ -338,3 +338,22 @@ def test_planted_mistakes_are_caught(monkeypatch, golden_dir, mutant, check):
     mutant(monkeypatch)
     with pytest.raises((AssertionError, ValueError)):
         check(golden_dir)
+
+
+# ---------------------------------------------------------- golden vectors ---
+def test_oracle_reproduces_committed_golden(golden_dir):
+    """toy_sketch.npz (written by tests/golden/make_golden.py) guards the
+    oracle against drift: maps, state and the factors must reproduce."""
+    g = np.load(f"{golden_dir}/toy_sketch.npz")
+    m, n, r, w, nu, eta = g["params"]
+    p = O.AlgParams(matrix_m=int(m), matrix_n=int(n), rank=int(r), window=int(w), sketch_nu=nu, sketch_eta=eta)
+    sk = O.SketchySVD(p)
+    for name in ("Upsilon", "Omega", "Phi", "Psi"):
+        assert np.array_equal(getattr(sk, name).data, g[name])
+    sk.linear_update(g["A"])
+    for name in ("X", "Y", "Z"):
+        assert O.rel_err(getattr(sk, name), g[name]) <= 1e-14
+    U, S, V = sk.low_rank_approx()
+    assert O.rel_err(S, g["S"]) <= 1e-14
+    Ua, Va = O.align_signs(U, V, g["U"], g["V"])
+    assert O.rel_err(Ua, g["U"]) <= 1e-13 and O.rel_err(Va, g["V"]) <= 1e-13
