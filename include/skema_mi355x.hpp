@@ -1,0 +1,284 @@
+// C++17 host layer over the C ABI in skema_mi355x.h, shaped like the
+// reference's own classes so Skema's driver code ports line for line:
+//   Skema::GaussDimRedux / SparseSignDimRedux  (src/Skema_DimRedux.hpp:95-308)
+//   Skema::SketchySVD<MatrixType, DimReduxT>   (src/Skema_SketchySVD.hpp:11-64)
+//   Skema::AlgParams (sketch fields)           (src/Skema_AlgParams.hpp:12-90)
+// Matrices are device-resident column-major views (the LayoutLeft analogue);
+// the library copies nothing it does not own and throws skema_mi355x::Error
+// (status + message) where the C ABI returns a status.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "skema_mi355x.h"
+
+namespace skema_mi355x {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int code) {
+  if (code != SKEMA_OK) throw Error(code, skema_last_error());
+}
+
+// ---- device memory ----------------------------------------------------------
+// Owning column-major FP64 matrix in device memory (rows x cols, ld = rows).
+class DeviceMatrix {
+ public:
+  DeviceMatrix() = default;
+  DeviceMatrix(int64_t rows, int64_t cols) : rows_(rows), cols_(cols), ld_(rows > 0 ? rows : 1) {
+    void* p = nullptr;
+    if (rows * cols > 0 && hipMalloc(&p, sizeof(double) * rows * cols) != hipSuccess)
+      throw Error(SKEMA_ERR_OOM, "DeviceMatrix: hipMalloc failed");
+    data_.reset(static_cast<double*>(p), [](double* q) { (void)hipFree(q); });
+    if (p) (void)hipMemset(p, 0, sizeof(double) * rows * cols);
+  }
+  static DeviceMatrix from_host(const double* colmajor, int64_t rows, int64_t cols, int64_t ld) {
+    DeviceMatrix m(rows, cols);
+    if (rows * cols > 0 &&
+        hipMemcpy2D(m.data(), sizeof(double) * rows, colmajor, sizeof(double) * ld, sizeof(double) * rows, cols,
+                    hipMemcpyHostToDevice) != hipSuccess)
+      throw Error(SKEMA_ERR_HIP, "DeviceMatrix: upload failed");
+    return m;
+  }
+  std::vector<double> to_host() const {
+    std::vector<double> h((size_t)(rows_ * cols_));
+    if (!h.empty() && hipMemcpy(h.data(), data(), sizeof(double) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      throw Error(SKEMA_ERR_HIP, "DeviceMatrix: download failed");
+    return h;
+  }
+  double* data() const { return data_.get(); }
+  int64_t extent(int i) const { return i == 0 ? rows_ : cols_; }
+  int64_t ld() const { return ld_; }
+
+ private:
+  std::shared_ptr<double> data_;
+  int64_t rows_ = 0, cols_ = 0, ld_ = 1;
+};
+
+// Device CSR (CrsMatrix<double,int,...,size_t> analogue, src/Skema_Utils.hpp:24-25).
+class CrsMatrix {
+ public:
+  CrsMatrix() = default;
+  CrsMatrix(int64_t m, int64_t n, const std::vector<int64_t>& rowptr, const std::vector<int32_t>& col,
+            const std::vector<double>& val)
+      : m_(m), n_(n), nnz_((int64_t)col.size()) {
+    rp_ = upload(rowptr);
+    ci_ = upload(col);
+    v_ = upload(val);
+  }
+  int64_t numRows() const { return m_; }
+  int64_t numCols() const { return n_; }
+  int64_t nnz() const { return nnz_; }
+  const int64_t* rowptr() const { return static_cast<const int64_t*>(rp_.get()); }
+  const int32_t* col() const { return static_cast<const int32_t*>(ci_.get()); }
+  const double* val() const { return static_cast<const double*>(v_.get()); }
+
+ private:
+  template <class T>
+  static std::shared_ptr<void> upload(const std::vector<T>& h) {
+    void* p = nullptr;
+    const size_t b = std::max<size_t>(h.size(), 1) * sizeof(T);
+    if (hipMalloc(&p, b) != hipSuccess) throw Error(SKEMA_ERR_OOM, "CrsMatrix: hipMalloc failed");
+    if (!h.empty() && hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      throw Error(SKEMA_ERR_HIP, "CrsMatrix: upload failed");
+    return std::shared_ptr<void>(p, [](void* q) { (void)hipFree(q); });
+  }
+  int64_t m_ = 0, n_ = 0, nnz_ = 0;
+  std::shared_ptr<void> rp_, ci_, v_;
+};
+
+// Dense point set for the Gauss RBF window (n points x d features).
+struct PointSet {
+  DeviceMatrix X;
+};
+
+// ---- context ------------------------------------------------------------------
+class Context {
+ public:
+  explicit Context(int device = 0) {
+    skema_ctx* c = nullptr;
+    check(skema_ctx_create(device, &c));
+    h_.reset(c, [](skema_ctx* q) { (void)skema_ctx_destroy(q); });
+  }
+  skema_ctx* get() const { return h_.get(); }
+
+ private:
+  std::shared_ptr<skema_ctx> h_;
+};
+
+// ---- parameters ----------------------------------------------------------------
+struct AlgParams {
+  int64_t matrix_m = 0, matrix_n = 0;
+  int64_t rank = 1, sketch_range = 1, sketch_core = 1, window = 1;
+  std::array<uint64_t, 4> seeds{{0, 1, 2, 3}};
+  double sketch_eta = 1.0, sketch_nu = 1.0;
+  int dim_redux = SKEMA_MAP_GAUSS;
+  bool issparse = false;
+  int kernel_func = SKEMA_KERNEL_NONE;
+  double kernel_gamma = 1.0;
+  int64_t nfeat = 0;
+  int precision = SKEMA_PREC_FP64;
+
+  skema_params to_c() const {
+    skema_params p;
+    skema_params_default(&p);
+    p.m = matrix_m;
+    p.n = matrix_n;
+    p.rank = rank;
+    p.range = sketch_range;
+    p.core = sketch_core;
+    p.window = window;
+    p.eta = sketch_eta;
+    p.nu = sketch_nu;
+    p.dim_redux = dim_redux;
+    p.issparse = issparse;
+    p.kernel = kernel_func;
+    p.gamma = kernel_gamma;
+    p.nfeat = nfeat;
+    p.precision = precision;
+    return p;
+  }
+  // src/Skema_SketchySVD.cpp:20-24
+  int64_t range() const { return sketch_range < rank ? 4 * rank + 1 : sketch_range; }
+  int64_t core() const { return sketch_core < rank ? 2 * range() + 1 : sketch_core; }
+};
+
+using range_type = std::pair<int64_t, int64_t>;
+
+// ---- DimRedux maps ----------------------------------------------------------------
+template <class Derived>
+class DimRedux {
+ public:
+  // lmap: alpha * op(map[:, idx]) * op(B) + beta * C  (src/Skema_DimRedux.hpp:60-69)
+  DeviceMatrix lmap(double alpha, const DeviceMatrix& B, double beta, char transA = 'N', char transB = 'N',
+                    range_type idx = {0, 0}) const {
+    const int64_t wc = idx.first != idx.second ? idx.second - idx.first : ncol_;
+    char eff = transA;
+    if (init_transposed_) eff = (transA == 'N' || transA == 'n') ? 'T' : 'N';
+    const int64_t M = (eff == 'N') ? nrow_ : wc;
+    const bool tb = !Derived::sparse && (transB == 'T' || transB == 't');
+    DeviceMatrix C(M, tb ? B.extent(0) : B.extent(1));
+    check(skema_map_lmap(h_.get(), alpha, B.data(), B.extent(0), B.extent(1), B.ld(), beta, transA, transB,
+                         idx.first, idx.second, C.data(), C.ld()));
+    return C;
+  }
+  // rmap: alpha * op(A) * op(map) + beta * C  (src/Skema_DimRedux.hpp:71-80)
+  DeviceMatrix rmap(double alpha, const DeviceMatrix& A, double beta, char transA = 'N', char transB = 'T',
+                    range_type idx = {0, 0}) const {
+    const bool ta = !Derived::sparse && (transA == 'T' || transA == 't');
+    DeviceMatrix C(ta ? A.extent(1) : A.extent(0), (transB == 'T' || transB == 't') ? nrow_ : ncol_);
+    check(skema_map_rmap(h_.get(), alpha, A.data(), A.extent(0), A.extent(1), A.ld(), beta, transA, transB,
+                         idx.first, idx.second, C.data(), C.ld()));
+    return C;
+  }
+  bool issparse() const { return Derived::sparse; }
+  int64_t nrows() const { return nrow_; }
+  int64_t ncols() const { return ncol_; }
+  skema_map* handle() const { return h_.get(); }
+
+ protected:
+  void adopt(skema_map* m, int64_t nrow, int64_t ncol, bool t) {
+    h_.reset(m, [](skema_map* q) { (void)skema_map_destroy(q); });
+    nrow_ = nrow;
+    ncol_ = ncol;
+    init_transposed_ = t;
+  }
+  std::shared_ptr<skema_map> h_;
+  int64_t nrow_ = 0, ncol_ = 0;
+  bool init_transposed_ = false;
+};
+
+// GaussDimRedux(nrow, ncol, seed, ..., init_transposed) (src/Skema_DimRedux.hpp:98-128)
+class GaussDimRedux : public DimRedux<GaussDimRedux> {
+ public:
+  static constexpr bool sparse = false;
+  GaussDimRedux(const Context& ctx, int64_t nrow, int64_t ncol, uint64_t seed, bool init_transposed = false,
+                int dist = SKEMA_DIST_NORMAL) {
+    skema_map* m = nullptr;
+    check(skema_map_gauss_seeded(ctx.get(), nrow, ncol, seed, dist, init_transposed, &m));
+    adopt(m, nrow, ncol, init_transposed);
+  }
+  GaussDimRedux(const Context& ctx, int64_t nrow, int64_t ncol, const double* host_colmajor,
+                bool init_transposed = false) {
+    skema_map* m = nullptr;
+    check(skema_map_dense(ctx.get(), nrow, ncol, host_colmajor, nrow, init_transposed, &m));
+    adopt(m, nrow, ncol, init_transposed);
+  }
+};
+
+// SparseSignDimRedux (src/Skema_DimRedux.hpp:191-268)
+class SparseSignDimRedux : public DimRedux<SparseSignDimRedux> {
+ public:
+  static constexpr bool sparse = true;
+  SparseSignDimRedux(const Context& ctx, int64_t nrow, int64_t ncol, uint64_t seed, bool init_transposed = false,
+                     int convention = SKEMA_SSIGN_ROWS, int64_t zeta = 0) {
+    skema_map* m = nullptr;
+    check(skema_map_sparse_sign_seeded(ctx.get(), nrow, ncol, seed, convention, zeta, init_transposed, &m));
+    adopt(m, nrow, ncol, init_transposed);
+  }
+};
+
+// ---- SketchySVD ------------------------------------------------------------------
+// MatrixType: DeviceMatrix (dense A), CrsMatrix (CSR A) or PointSet (RBF).
+template <class MatrixType, class DimReduxT>
+class SketchySVD {
+ public:
+  // ctor (src/Skema_SketchySVD.cpp:15-117): maps from params.seeds with the
+  // reference's storage orientation.
+  SketchySVD(const AlgParams& p, const Context& ctx = Context(0)) : p_(p), ctx_(ctx) {
+    const int64_t k = p.range(), s = p.core(), m = p.matrix_m, n = p.matrix_n;
+    const bool g = p.dim_redux == SKEMA_MAP_GAUSS;
+    const bool tU = p.issparse && g, tO = p.issparse;
+    maps_.emplace_back(make(tU ? m : k, tU ? k : m, p.seeds[0], tU));
+    maps_.emplace_back(make(tO ? n : k, tO ? k : n, p.seeds[1], tO));
+    maps_.emplace_back(make(tU ? m : s, tU ? s : m, p.seeds[2], tU));
+    maps_.emplace_back(make(tO ? n : s, tO ? s : n, p.seeds[3], tO));
+    skema_params cp = p.to_c();
+    skema_sketch* h = nullptr;
+    check(skema_sketch_create(ctx_.get(), &cp, maps_[0].handle(), maps_[1].handle(), maps_[2].handle(),
+                              maps_[3].handle(), &h));
+    h_.reset(h, [](skema_sketch* q) { (void)skema_sketch_destroy(q); });
+  }
+
+  // linear_update (src/Skema_SketchySVD.cpp:119-246)
+  void linear_update(const MatrixType& A) { update_all(A); }
+
+  // low_rank_approx (src/Skema_SketchySVD.cpp:568-664): U (m x r), S (r), V (n x r)
+  std::tuple<DeviceMatrix, std::vector<double>, DeviceMatrix> low_rank_approx() {
+    DeviceMatrix U(p_.matrix_m, p_.rank), V(p_.matrix_n, p_.rank), S(p_.rank, 1);
+    check(skema_low_rank_approx_device(h_.get(), U.data(), U.ld(), S.data(), V.data(), V.ld()));
+    return {U, S.to_host(), V};
+  }
+
+  std::string timings_json() const {
+    std::vector<char> buf(16384);
+    check(skema_sketch_timings_json(h_.get(), buf.data(), buf.size()));
+    return std::string(buf.data());
+  }
+  skema_sketch* handle() const { return h_.get(); }
+
+ private:
+  DimReduxT make(int64_t r, int64_t c, uint64_t seed, bool t) { return DimReduxT(ctx_, r, c, seed, t); }
+  void update_all(const DeviceMatrix& A) { check(skema_linear_update_dense(h_.get(), A.data(), A.ld())); }
+  void update_all(const CrsMatrix& A) {
+    check(skema_linear_update_csr(h_.get(), A.rowptr(), A.col(), A.val()));
+  }
+  void update_all(const PointSet& P) { check(skema_linear_update_rbf(h_.get(), P.X.data(), P.X.ld())); }
+
+  AlgParams p_;
+  Context ctx_;
+  std::vector<DimReduxT> maps_;
+  std::shared_ptr<skema_sketch> h_;
+};
+
+}  // namespace skema_mi355x

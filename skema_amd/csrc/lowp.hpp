@@ -1,5 +1,5 @@
 // Reduced-precision forms of the dense sketch contractions (FP32, BF16,
-// split-BF16, FP8 e4m3) and the fused RBF window generator.
+// split-BF16, FP8 e4m3).
 #pragma once
 #include "sketch.hpp"
 
@@ -7,8 +7,5 @@ namespace skema {
 
 void prepare_lowp_maps(Sketch& sk);
 void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows);
-
-bool rbf_fused_supported(const Sketch& sk);
-void update_rbf_window_fused(Sketch& sk, const double* points, int64_t ldp, int64_t row0, int64_t rows);
 
 }  // namespace skema

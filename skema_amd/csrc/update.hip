@@ -15,6 +15,7 @@
 #include "linalg.hpp"
 #include "lowp.hpp"
 #include "rbf.hpp"
+#include "rbf_fused.hpp"
 #include "sketch.hpp"
 #include "sparse.hpp"
 #include "util.hpp"
