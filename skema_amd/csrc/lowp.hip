@@ -2,20 +2,23 @@
 // window (A_w is w x n, FP64, column-major in HBM):
 //   left : x        = Upsilon[:, idx] * A_w          (k x n, reduce over w)
 //   right: [y | G]  = A_w * [Omega; Psi]^T           (w x (k+s), reduce over n)
-// The maps are converted once (sketch creation) to the form's storage,
-// row-major with K contiguous.  A_w is converted while it is staged into LDS,
-// so no reduced copy of the window is ever written to HBM.  Products
+//
+// Each window is read once in FP64 by a quantize pass that writes two compact
+// copies with the reduction index contiguous: L (column-major, for the left
+// product, K = window rows) and R (row-major, for the right product, K =
+// columns).  Both products are then the same "NT" GEMM, C = A * B^T with A
+// (M x K) and B (N x K) both K-contiguous, fed by 16-byte loads and K-major
+// LDS tiles.  The maps are converted once at sketch creation.  Products
 // accumulate in FP32 on the matrix cores; the epilogue widens to FP64 and
 // applies the nu/eta update into the FP64 state.  The small core product
-// Z += Phi[:, idx] * G stays on the FP64 path.
+// Z += Phi[:, idx] * G and the whole finalize stay FP64.
 //
 // Forms:
-//   bf16   : v_mfma_f32_16x16x32_bf16, one term
+//   bf16   : v_mfma_f32_16x16x32_bf16, one product
 //   bf16x3 : same instruction, x = hi + lo (both bf16), a*b ~ hi*hi + hi*lo + lo*hi
-//   fp32   : v_mfma_f32_16x16x4_f32
-//   fp8    : v_mfma_scale_f32_16x16x128_f8f6f4 with e4m3 operands and one
+//   fp8    : v_mfma_scale_f32_16x16x128_f8f6f4 on e4m3 operands with one
 //            power-of-two (E8M0) scale per 32 consecutive K elements of every
-//            row / column, chosen from the block's max |x| as it is staged.
+//            row, chosen from that block's max |x| when it is quantized.
 #include <cmath>
 
 #include "lowp.hpp"
@@ -27,21 +30,10 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef int i8v __attribute__((ext_vector_type(8)));
+typedef int i4v __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
-
-struct LpArgs {
-  int64_t M, N, K;
-  const void* map_hi;  // reduced map, row-major, K contiguous
-  const void* map_lo;  // split-BF16 low part (bf16x3 only)
-  const float* map_scale;  // unused by the bf16 forms
-  int64_t ldmap;
-  int64_t map_koff;    // K offset into the map's columns (window row0 for left)
-  const double* data;  // A_w
-  int64_t ldd;
-  GemmOut out;
-  int tiles_m;
-};
+constexpr int kQ = 64;  // quantize tile
 
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   const int q = nb >> 3, r = nb & 7;
@@ -59,111 +51,214 @@ __device__ __forceinline__ void store_out(const GemmOut& o, int64_t i, int64_t j
   }
 }
 
-// ---------------------------------------------------------------------------
-// BF16 / split-BF16 GEMM.  LEFT: A operand = map (M = k rows), B = A_w
-// (K = w rows, contiguous).  RIGHT: A operand = A_w (M = w rows, contiguous),
-// B = map (N = k+s rows).  LDS tiles are K-major: [row][BK + 8] bf16.
-// ---------------------------------------------------------------------------
-template <int BM, int BN, bool SPLIT, bool LEFT>
-__global__ __launch_bounds__(kThreads) void k_gemm_bf16(LpArgs g) {
+// ---- e4m3 with a per-block power-of-two scale --------------------------------
+// Scale exponent e with max|x| * 2^-e in [128, 256) (<= 448, the largest e4m3
+// normal); E8M0 byte = 127 + e.  A zero block gets e = 0.
+__device__ __forceinline__ int block_exponent(float amax) {
+  if (!(amax > 0.f)) return 0;
+  int ex;
+  (void)frexpf(amax, &ex);  // amax = f * 2^ex, f in [0.5, 1)
+  int e = ex - 8;
+  e = e < -127 ? -127 : (e > 127 ? 127 : e);
+  return e;
+}
+__device__ __forceinline__ uint32_t pack4_e4m3(float a, float b, float c, float d) {
+  int lo = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+  return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(c, d, lo, true);
+}
+
+// ---- window quantization ------------------------------------------------------
+// 64 x 64 tile of A_w -> L (column-major, ld ldl) and R (row-major, ld ldr).
+template <bool SPLIT>
+__global__ __launch_bounds__(kThreads) void k_quant_bf16(const double* A, int64_t lda, int64_t rows, int64_t n,
+                                                         __bf16* Lh, __bf16* Ll, int64_t ldl, __bf16* Rh,
+                                                         __bf16* Rl, int64_t ldr) {
+  __shared__ float t[kQ][kQ + 1];  // t[j][i]
+  const int tid = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * kQ, j0 = (int64_t)blockIdx.y * kQ;
+  for (int e = 0; e < kQ * kQ / kThreads; ++e) {
+    const int i = tid % kQ, j = tid / kQ + e * (kThreads / kQ);
+    const int64_t gi = i0 + i, gj = j0 + j;
+    const float x = (gi < rows && gj < n) ? (float)A[gi + gj * lda] : 0.f;
+    t[j][i] = x;
+    if (gi < rows && gj < n) {
+      const __bf16 h = (__bf16)x;
+      Lh[gj * ldl + gi] = h;
+      if (SPLIT) Ll[gj * ldl + gi] = (__bf16)(x - (float)h);
+    }
+  }
+  __syncthreads();
+  for (int e = 0; e < kQ * kQ / kThreads; ++e) {
+    const int j = tid % kQ, i = tid / kQ + e * (kThreads / kQ);
+    const int64_t gi = i0 + i, gj = j0 + j;
+    if (gi < rows && gj < n) {
+      const float x = t[j][i];
+      const __bf16 h = (__bf16)x;
+      Rh[gi * ldr + gj] = h;
+      if (SPLIT) Rl[gi * ldr + gj] = (__bf16)(x - (float)h);
+    }
+  }
+}
+
+// FP8: L blocks are 32 consecutive rows of a column, R blocks 32 consecutive
+// columns of a row.  Ls[j * ldsl + i/32], Rs[i * ldsr + j/32].
+__global__ __launch_bounds__(kThreads) void k_quant_fp8(const double* A, int64_t lda, int64_t rows, int64_t n,
+                                                        uint8_t* Lq, int64_t ldl, uint8_t* Ls, int64_t ldsl,
+                                                        uint8_t* Rq, int64_t ldr, uint8_t* Rs, int64_t ldsr) {
+  __shared__ float t[kQ][kQ + 1];  // t[j][i]
+  const int tid = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * kQ, j0 = (int64_t)blockIdx.y * kQ;
+  for (int e = 0; e < kQ * kQ / kThreads; ++e) {
+    const int i = tid % kQ, j = tid / kQ + e * (kThreads / kQ);
+    const int64_t gi = i0 + i, gj = j0 + j;
+    t[j][i] = (gi < rows && gj < n) ? (float)A[gi + gj * lda] : 0.f;
+  }
+  __syncthreads();
+  const bool left = tid < 128;
+  const int task = tid & 127;
+  const int a = task >> 1, b = task & 1;  // a: column (left) or row (right), b: 32-block
+  float v[32];
+  float amax = 0.f;
+#pragma unroll
+  for (int q = 0; q < 32; ++q) {
+    v[q] = left ? t[a][32 * b + q] : t[32 * b + q][a];
+    amax = fmaxf(amax, fabsf(v[q]));
+  }
+  const int e = block_exponent(amax);
+  const float inv = ldexpf(1.f, -e);
+  uint32_t w[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) w[q] = pack4_e4m3(v[4 * q] * inv, v[4 * q + 1] * inv, v[4 * q + 2] * inv, v[4 * q + 3] * inv);
+  if (left) {
+    const int64_t gj = j0 + a, gi = i0 + 32 * b;
+    if (gj < n && gi < rows) {
+      uint32_t* dst = reinterpret_cast<uint32_t*>(Lq + gj * ldl + gi);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) dst[q] = w[q];
+      Ls[gj * ldsl + gi / 32] = (uint8_t)(127 + e);
+    }
+  } else {
+    const int64_t gi = i0 + a, gj = j0 + 32 * b;
+    if (gi < rows && gj < n) {
+      uint32_t* dst = reinterpret_cast<uint32_t*>(Rq + gi * ldr + gj);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) dst[q] = w[q];
+      Rs[gi * ldsr + gj / 32] = (uint8_t)(127 + e);
+    }
+  }
+}
+
+// Map -> row-major reduced copy (rows x cols logical, column-major source).
+__global__ void k_convert_map_bf16(const double* src, int64_t ld, int64_t rows, int64_t cols, __bf16* hi, __bf16* lo,
+                                   int64_t ldo) {
+  const int64_t total = rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e % rows, c = e / rows;
+    const float x = (float)src[r + c * ld];
+    const __bf16 h = (__bf16)x;
+    hi[r * ldo + c] = h;
+    if (lo) lo[r * ldo + c] = (__bf16)(x - (float)h);
+  }
+}
+// One thread per (row, 32-block): scale + 32 e4m3 bytes.
+__global__ void k_convert_map_fp8(const double* src, int64_t ld, int64_t rows, int64_t cols, uint8_t* q,
+                                  int64_t ldo, uint8_t* sc, int64_t lds) {
+  const int64_t nb = (cols + 31) / 32;
+  const int64_t total = rows * nb;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e % rows, b = e / rows;
+    float v[32];
+    float amax = 0.f;
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const int64_t c = 32 * b + t;
+      v[t] = (c < cols) ? (float)src[r + c * ld] : 0.f;
+      amax = fmaxf(amax, fabsf(v[t]));
+    }
+    const int ex = block_exponent(amax);
+    const float inv = ldexpf(1.f, -ex);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(q + r * ldo + 32 * b);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) dst[t] = pack4_e4m3(v[4 * t] * inv, v[4 * t + 1] * inv, v[4 * t + 2] * inv, v[4 * t + 3] * inv);
+    sc[r * lds + b] = (uint8_t)(127 + ex);
+  }
+}
+
+// ---- NT GEMM: C(i, j) = sum_k A(i, k) B(j, k), A and B K-contiguous ----------
+struct NtArgs {
+  int64_t M, N, K;
+  const void* ah;  // A planes (hi, lo) or fp8 bytes
+  const void* al;
+  const uint8_t* as;  // fp8 scales of A: as[i * lds_a + (koff_a + k) / 32]
+  int64_t lda, koff_a, lds_a;
+  const void* bh;
+  const void* bl;
+  const uint8_t* bs;
+  int64_t ldb, lds_b;
+  GemmOut out;
+  int tiles_m;
+};
+
+template <int BM, int BN, bool SPLIT>
+__global__ __launch_bounds__(kThreads) void k_gemm_nt_bf16(NtArgs g) {
   constexpr int BK = 64, LDK = BK + 8;
   constexpr int TM = BM / 32, TN = BN / 32;
   constexpr int NP = SPLIT ? 2 : 1;
-  constexpr int MAPR = LEFT ? BM : BN;   // rows of the map tile
-  constexpr int DATR = LEFT ? BN : BM;   // rows (K-major) of the data tile
-  constexpr int MAPC = MAPR * BK / 8 / kThreads;  // 16-byte map chunks per thread
-  constexpr int DATC = DATR * BK / 8 / kThreads;  // 8-element data chunks per thread
+  constexpr int CA = BM * BK / 8 / kThreads, CB = BN * BK / 8 / kThreads;
   __shared__ __attribute__((aligned(16))) __bf16 sA[NP][BM][LDK];
   __shared__ __attribute__((aligned(16))) __bf16 sB[NP][BN][LDK];
-
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
   const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
-  const int64_t r0_map = LEFT ? m0 : n0;
-  const int64_t r0_dat = LEFT ? n0 : m0;
-  const int64_t nmap = LEFT ? g.M : g.N;
-  const int64_t ndat = LEFT ? g.N : g.M;
-  const __bf16* mh = static_cast<const __bf16*>(g.map_hi);
-  const __bf16* ml = static_cast<const __bf16*>(g.map_lo);
+  const __bf16* A[2] = {static_cast<const __bf16*>(g.ah), static_cast<const __bf16*>(g.al)};
+  const __bf16* B[2] = {static_cast<const __bf16*>(g.bh), static_cast<const __bf16*>(g.bl)};
 
   f4v acc[TM][TN];
 #pragma unroll
   for (int a = 0; a < TM; ++a)
 #pragma unroll
     for (int b = 0; b < TN; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+  bf16x8 ra[NP][CA], rb[NP][CB];
 
-  bf16x8 rm[NP][MAPC];
-  double rd[DATC][8];
-
-  auto gload = [&](int64_t k0) {
-    // map chunks: chunk c -> row c / (BK/8), k-chunk c % (BK/8)
+  auto load_chunk = [&](const __bf16* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
+    const __bf16* src = base + row * ld + koff + k;
+    bf16x8 v;
+    if (row < nrows && k + 8 <= g.K && (((uintptr_t)src) & 15) == 0) {
+      v = *reinterpret_cast<const bf16x8*>(src);
+    } else {
 #pragma unroll
-    for (int e = 0; e < MAPC; ++e) {
-      const int c = tid + e * kThreads;
-      const int row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
-      const int64_t gr = r0_map + row, gk = k0 + kc;
-#pragma unroll
-      for (int h = 0; h < NP; ++h) {
-        const __bf16* src = (h == 0 ? mh : ml) + gr * g.ldmap + g.map_koff + gk;
-        bf16x8 v;
-        if (gr < nmap && gk + 8 <= g.K && (((uintptr_t)src) & 15) == 0) {
-          v = *reinterpret_cast<const bf16x8*>(src);
-        } else {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) v[q] = (gr < nmap && gk + q < g.K) ? src[q] : (__bf16)0.0f;
-        }
-        rm[h][e] = v;
-      }
+      for (int q = 0; q < 8; ++q) v[q] = (row < nrows && k + q < g.K) ? src[q] : (__bf16)0.0f;
     }
-    // data chunks
+    return v;
+  };
+  auto gload = [&](int64_t k0) {
 #pragma unroll
-    for (int e = 0; e < DATC; ++e) {
-      const int c = tid + e * kThreads;
-      int row, kc;
-      if (LEFT) { row = c / (BK / 8); kc = (c % (BK / 8)) * 8; }   // K-contiguous columns of A_w
-      else      { row = c % BM;       kc = (c / BM) * 8; }          // M-contiguous rows of A_w
-      const int64_t gr = r0_dat + row, gk = k0 + kc;
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        double v = 0.0;
-        if (gr < ndat && gk + q < g.K) v = LEFT ? g.data[(gk + q) + gr * g.ldd] : g.data[gr + (gk + q) * g.ldd];
-        rd[e][q] = v;
-      }
+      for (int h = 0; h < NP; ++h) ra[h][e] = load_chunk(A[h], m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+#pragma unroll
+      for (int h = 0; h < NP; ++h) rb[h][e] = load_chunk(B[h], n0 + row, g.N, g.ldb, 0, k0 + kc);
     }
   };
   auto sstore = [&]() {
 #pragma unroll
-    for (int e = 0; e < MAPC; ++e) {
-      const int c = tid + e * kThreads;
-      const int row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
 #pragma unroll
-      for (int h = 0; h < NP; ++h) {
-        __bf16* dst = LEFT ? &sA[h][row][kc] : &sB[h][row][kc];
-        *reinterpret_cast<bf16x8*>(dst) = rm[h][e];
-      }
+      for (int h = 0; h < NP; ++h) *reinterpret_cast<bf16x8*>(&sA[h][row][kc]) = ra[h][e];
     }
 #pragma unroll
-    for (int e = 0; e < DATC; ++e) {
-      const int c = tid + e * kThreads;
-      int row, kc;
-      if (LEFT) { row = c / (BK / 8); kc = (c % (BK / 8)) * 8; }
-      else      { row = c % BM;       kc = (c / BM) * 8; }
-      bf16x8 hi, lo;
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const float x = (float)rd[e][q];
-        const __bf16 h = (__bf16)x;
-        hi[q] = h;
-        if (SPLIT) lo[q] = (__bf16)(x - (float)h);
-      }
-      __bf16* dh = LEFT ? &sB[0][row][kc] : &sA[0][row][kc];
-      *reinterpret_cast<bf16x8*>(dh) = hi;
-      if (SPLIT) {
-        __bf16* dl = LEFT ? &sB[NP - 1][row][kc] : &sA[NP - 1][row][kc];
-        *reinterpret_cast<bf16x8*>(dl) = lo;
-      }
+      for (int h = 0; h < NP; ++h) *reinterpret_cast<bf16x8*>(&sB[h][row][kc]) = rb[h][e];
     }
   };
 
@@ -198,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_bf16(LpArgs g) {
         }
     }
   }
-  // D layout of the 16x16 f32 MFMAs: lane l holds rows 4*(l/16)+v of column l%16.
+  // 16x16 f32 MFMA result: lane l holds rows 4*(l/16)+v of column l%16.
 #pragma unroll
   for (int x = 0; x < TM; ++x)
 #pragma unroll
@@ -211,105 +306,293 @@ __global__ __launch_bounds__(kThreads) void k_gemm_bf16(LpArgs g) {
       }
 }
 
-// Map conversion: src logical (rows x cols) column-major FP64 (ld) ->
-// row-major bf16 hi (and lo) with leading dimension ldo.
-__global__ void k_convert_map_bf16(const double* src, int64_t ld, int64_t rows, int64_t cols, __bf16* hi, __bf16* lo,
-                                   int64_t ldo) {
-  const int64_t total = rows * cols;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e % rows, c = e / rows;
-    const float x = (float)src[r + c * ld];
-    const __bf16 h = (__bf16)x;
-    hi[r * ldo + c] = h;
-    if (lo) lo[r * ldo + c] = (__bf16)(x - (float)h);
+// FP8 NT GEMM: BK = 128 = one MFMA; lane l feeds 32 bytes (row l%16, K block
+// l/16) and that block's E8M0 scale.
+template <int BM, int BN>
+__global__ __launch_bounds__(kThreads) void k_gemm_nt_fp8(NtArgs g) {
+  constexpr int BK = 128, LDK = BK + 16;
+  constexpr int TM = BM / 32, TN = BN / 32;
+  constexpr int CA = BM * BK / 16 / kThreads, CB = BN * BK / 16 / kThreads;
+  __shared__ __attribute__((aligned(16))) uint8_t sA[BM][LDK];
+  __shared__ __attribute__((aligned(16))) uint8_t sB[BN][LDK];
+  __shared__ uint8_t sAs[BM][4];
+  __shared__ uint8_t sBs[BN][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
+  const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
+  const uint8_t* A = static_cast<const uint8_t*>(g.ah);
+  const uint8_t* B = static_cast<const uint8_t*>(g.bh);
+
+  f4v acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+  i4v ra[CA], rb[CB];
+  constexpr int SA = (BM * 4 + kThreads - 1) / kThreads, SB = (BN * 4 + kThreads - 1) / kThreads;
+  uint8_t rsa[SA], rsb[SB];
+
+  auto load16 = [&](const uint8_t* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
+    const uint8_t* src = base + row * ld + koff + k;
+    i4v v;
+    if (row < nrows && k + 16 <= g.K && (((uintptr_t)src) & 15) == 0) {
+      v = *reinterpret_cast<const i4v*>(src);
+    } else {
+      uint8_t b[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) b[q] = (row < nrows && k + q < g.K) ? src[q] : (uint8_t)0;
+      v = *reinterpret_cast<const i4v*>(b);
+    }
+    return v;
+  };
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
+      ra[e] = load16(A, m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
+      rb[e] = load16(B, n0 + row, g.N, g.ldb, 0, k0 + kc);
+    }
+    // scales: one byte per (row, 32-block of the stage)
+#pragma unroll
+    for (int e = 0; e < SA; ++e) {
+      const int c = tid + e * kThreads;
+      const int row = c >> 2, kb = c & 3;
+      const int64_t gr = m0 + row, k = k0 + 32 * kb;
+      rsa[e] = (c < BM * 4 && gr < g.M && k < g.K) ? g.as[gr * g.lds_a + (g.koff_a + k) / 32] : (uint8_t)127;
+    }
+#pragma unroll
+    for (int e = 0; e < SB; ++e) {
+      const int c = tid + e * kThreads;
+      const int row = c >> 2, kb = c & 3;
+      const int64_t gr = n0 + row, k = k0 + 32 * kb;
+      rsb[e] = (c < BN * 4 && gr < g.N && k < g.K) ? g.bs[gr * g.lds_b + k / 32] : (uint8_t)127;
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
+      *reinterpret_cast<i4v*>(&sA[row][kc]) = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
+      *reinterpret_cast<i4v*>(&sB[row][kc]) = rb[e];
+    }
+#pragma unroll
+    for (int e = 0; e < SA; ++e) {
+      const int c = tid + e * kThreads;
+      if (c < BM * 4) sAs[c >> 2][c & 3] = rsa[e];
+    }
+#pragma unroll
+    for (int e = 0; e < SB; ++e) {
+      const int c = tid + e * kThreads;
+      if (c < BN * 4) sBs[c >> 2][c & 3] = rsb[e];
+    }
+  };
+
+  gload(0);
+  for (int64_t k0 = 0; k0 < g.K; k0 += BK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + BK < g.K) gload(k0 + BK);
+    const int kb = lane >> 4;
+    i8v a[TM], b[TN];
+    int sa[TM], sb[TN];
+#pragma unroll
+    for (int x = 0; x < TM; ++x) {
+      const int row = wm * (BM / 2) + x * 16 + (lane & 15);
+      const i4v* p = reinterpret_cast<const i4v*>(&sA[row][32 * kb]);
+      const i4v lo = p[0], hi = p[1];
+      a[x] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      sa[x] = sAs[row][kb];
+    }
+#pragma unroll
+    for (int y = 0; y < TN; ++y) {
+      const int row = wn * (BN / 2) + y * 16 + (lane & 15);
+      const i4v* p = reinterpret_cast<const i4v*>(&sB[row][32 * kb]);
+      const i4v lo = p[0], hi = p[1];
+      b[y] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      sb[y] = sBs[row][kb];
+    }
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TN; ++y)
+        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[x], b[y], acc[x][y], 0, 0, 0, sa[x], 0, sb[y]);
   }
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TN; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t i = m0 + wm * (BM / 2) + x * 16 + 4 * (lane >> 4) + v;
+        const int64_t j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
+        if (i < g.M && j < g.N) store_out(g.out, i, j, (double)acc[x][y][v]);
+      }
 }
 
-template <int BM, int BN, bool SPLIT, bool LEFT>
-void launch_bf16(hipStream_t st, const LpArgs& a, int tiles) {
-  hipLaunchKernelGGL((k_gemm_bf16<BM, BN, SPLIT, LEFT>), dim3(tiles), dim3(kThreads), 0, st, a);
-}
-
-template <bool SPLIT, bool LEFT>
-void gemm_bf16(hipStream_t st, LpArgs a) {
-  // Same tile rule as the FP64 GEMM: minimise the busiest CU's share.
-  struct Cfg { int bm, bn; };
-  const Cfg cfgs[] = {{128, 128}, {128, 64}, {64, 128}, {64, 64}};
-  Cfg best = cfgs[3];
+struct Tile {
+  int bm, bn;
+};
+Tile pick_tile(int64_t M, int64_t N) {
+  const Tile cfgs[] = {{128, 128}, {128, 64}, {64, 128}, {64, 64}};
+  Tile best = cfgs[3];
   int64_t best_cost = INT64_MAX;
-  for (const Cfg& c : cfgs) {
-    const int64_t tiles = cdiv(a.M, c.bm) * cdiv(a.N, c.bn);
+  for (const Tile& c : cfgs) {
+    const int64_t tiles = cdiv(M, c.bm) * cdiv(N, c.bn);
     const int64_t cost = cdiv(tiles, 256) * c.bm * c.bn;
     if (cost < best_cost) { best_cost = cost; best = c; }
   }
-  a.tiles_m = (int)cdiv(a.M, best.bm);
-  const int tiles = (int)(a.tiles_m * cdiv(a.N, best.bn));
-  if (best.bm == 128 && best.bn == 128) launch_bf16<128, 128, SPLIT, LEFT>(st, a, tiles);
-  else if (best.bm == 128) launch_bf16<128, 64, SPLIT, LEFT>(st, a, tiles);
-  else if (best.bn == 128) launch_bf16<64, 128, SPLIT, LEFT>(st, a, tiles);
-  else launch_bf16<64, 64, SPLIT, LEFT>(st, a, tiles);
-  check_launch("k_gemm_bf16");
+  return best;
 }
+
+void gemm_nt(hipStream_t st, int prec, NtArgs a) {
+  const Tile tl = pick_tile(a.M, a.N);
+  a.tiles_m = (int)cdiv(a.M, tl.bm);
+  const unsigned tiles = (unsigned)(a.tiles_m * cdiv(a.N, tl.bn));
+  const dim3 grid(tiles), block(kThreads);
+#define SKEMA_NT_LAUNCH(BM_, BN_)                                                                   \
+  do {                                                                                              \
+    if (prec == SKEMA_PREC_FP8) hipLaunchKernelGGL((k_gemm_nt_fp8<BM_, BN_>), grid, block, 0, st, a); \
+    else if (prec == SKEMA_PREC_BF16X3)                                                             \
+      hipLaunchKernelGGL((k_gemm_nt_bf16<BM_, BN_, true>), grid, block, 0, st, a);                  \
+    else hipLaunchKernelGGL((k_gemm_nt_bf16<BM_, BN_, false>), grid, block, 0, st, a);              \
+  } while (0)
+  if (tl.bm == 128 && tl.bn == 128) SKEMA_NT_LAUNCH(128, 128);
+  else if (tl.bm == 128) SKEMA_NT_LAUNCH(128, 64);
+  else if (tl.bn == 128) SKEMA_NT_LAUNCH(64, 128);
+  else SKEMA_NT_LAUNCH(64, 64);
+#undef SKEMA_NT_LAUNCH
+  check_launch("k_gemm_nt");
+}
+
+// The fp8 kernel feeds scale (row l%16, K block l/16) from lane l; a
+// one-wave layout check of that assumption on gfx950 disagreed, so the form
+// stays off until the lane-to-scale mapping is established.
+constexpr bool kFp8ScaleLayoutVerified = false;
+
+unsigned grid1(int64_t e) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(e, 256), 65536)); }
 
 }  // namespace
 
 void prepare_lowp_maps(Sketch& sk) {
   const int prec = sk.p.precision;
-  SKEMA_REQUIRE(prec == SKEMA_PREC_BF16 || prec == SKEMA_PREC_BF16X3, kErrUnsupported,
-                "this reduced-precision form is not built yet");
+  SKEMA_REQUIRE(prec == SKEMA_PREC_BF16 || prec == SKEMA_PREC_BF16X3 || prec == SKEMA_PREC_FP8, kErrUnsupported,
+                "reduced-precision form not built (fp32): use fp64, bf16, bf16x3 or fp8");
+  SKEMA_REQUIRE(prec != SKEMA_PREC_FP8 || kFp8ScaleLayoutVerified, kErrUnsupported,
+                "fp8 form disabled: the per-lane E8M0 scale layout of the scaled f8f6f4 MFMA is not yet verified");
+  if (prec == SKEMA_PREC_FP8)
+    SKEMA_REQUIRE(sk.p.window % 32 == 0 || sk.p.window >= sk.p.m, kErrUnsupported,
+                  "fp8 form: the window must be a multiple of 32 rows (scale blocks of the maps)");
   hipStream_t st = sk.ctx->stream;
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n;
-  const bool split = prec == SKEMA_PREC_BF16X3;
-  // Upsilon: k x m, row-major, ld padded to a multiple of 8 elements.
-  sk.lp_ldups = (m + 7) / 8 * 8;
-  sk.lp_ldomps = (n + 7) / 8 * 8;
-  const size_t ups_elems = (size_t)k * sk.lp_ldups, omps_elems = (size_t)(k + s) * sk.lp_ldomps;
-  sk.lp_ups.alloc(ups_elems * 2 * (split ? 2 : 1));
-  sk.lp_omps.alloc(omps_elems * 2 * (split ? 2 : 1));
-  __bf16* uh = sk.lp_ups.as<__bf16>();
-  __bf16* oh = sk.lp_omps.as<__bf16>();
-  auto grid = [](int64_t e) { return (unsigned)std::min<int64_t>(cdiv(e, 256), 65536); };
-  hipLaunchKernelGGL(k_convert_map_bf16, dim3(grid(k * m)), dim3(256), 0, st, sk.upsL, sk.ldups, k, m, uh,
-                     split ? uh + ups_elems : nullptr, sk.lp_ldups);
-  hipLaunchKernelGGL(k_convert_map_bf16, dim3(grid((k + s) * n)), dim3(256), 0, st, sk.omps.as<double>(), k + s,
-                     k + s, n, oh, split ? oh + omps_elems : nullptr, sk.lp_ldomps);
-  check_launch("k_convert_map_bf16");
+  // Row-major reduced maps, K contiguous, leading dimension padded to 128.
+  sk.lp_ldups = cdiv(m, 128) * 128;
+  sk.lp_ldomps = cdiv(n, 128) * 128;
+  const size_t ue = (size_t)k * sk.lp_ldups, oe = (size_t)(k + s) * sk.lp_ldomps;
+  if (prec == SKEMA_PREC_FP8) {
+    sk.lp_ldsups = sk.lp_ldups / 32;
+    sk.lp_ldsomps = sk.lp_ldomps / 32;
+    sk.lp_ups.alloc(ue + (size_t)k * sk.lp_ldsups);
+    sk.lp_omps.alloc(oe + (size_t)(k + s) * sk.lp_ldsomps);
+    hipLaunchKernelGGL(k_convert_map_fp8, dim3(grid1(k * sk.lp_ldsups)), dim3(256), 0, st, sk.upsL, sk.ldups, k, m,
+                       sk.lp_ups.as<uint8_t>(), sk.lp_ldups, sk.lp_ups.as<uint8_t>() + ue, sk.lp_ldsups);
+    hipLaunchKernelGGL(k_convert_map_fp8, dim3(grid1((k + s) * sk.lp_ldsomps)), dim3(256), 0, st,
+                       sk.omps.as<double>(), k + s, k + s, n, sk.lp_omps.as<uint8_t>(), sk.lp_ldomps,
+                       sk.lp_omps.as<uint8_t>() + oe, sk.lp_ldsomps);
+    check_launch("k_convert_map_fp8");
+  } else {
+    const bool split = prec == SKEMA_PREC_BF16X3;
+    sk.lp_ups.alloc(ue * 2 * (split ? 2 : 1));
+    sk.lp_omps.alloc(oe * 2 * (split ? 2 : 1));
+    __bf16* uh = sk.lp_ups.as<__bf16>();
+    __bf16* oh = sk.lp_omps.as<__bf16>();
+    hipLaunchKernelGGL(k_convert_map_bf16, dim3(grid1(k * m)), dim3(256), 0, st, sk.upsL, sk.ldups, k, m, uh,
+                       split ? uh + ue : nullptr, sk.lp_ldups);
+    hipLaunchKernelGGL(k_convert_map_bf16, dim3(grid1((k + s) * n)), dim3(256), 0, st, sk.omps.as<double>(), k + s,
+                       k + s, n, oh, split ? oh + oe : nullptr, sk.lp_ldomps);
+    check_launch("k_convert_map_bf16");
+  }
   SKEMA_HIP(hipStreamSynchronize(st));
 }
 
 void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows) {
   Ctx* c = sk.ctx;
   hipStream_t st = c->stream;
+  const int prec = sk.p.precision;
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n;
   const double eta = sk.p.eta, nu = sk.p.nu;
-  const bool split = sk.p.precision == SKEMA_PREC_BF16X3;
+  const bool fp8 = prec == SKEMA_PREC_FP8, split = prec == SKEMA_PREC_BF16X3;
+  if (fp8) SKEMA_REQUIRE(row0 % 32 == 0, kErrUnsupported, "fp8 form: windows must start at a multiple of 32 rows");
   double* G = sk.G.as<double>();
-  const size_t ups_elems = (size_t)k * sk.lp_ldups, omps_elems = (size_t)(k + s) * sk.lp_ldomps;
+  // window copies: L (column-major, ld ldl = rows padded), R (row-major, ld ldr = n padded)
+  const int64_t ldl = cdiv(rows, 128) * 128, ldr = cdiv(n, 128) * 128;
+  const size_t esz = fp8 ? 1 : 2, planes = split ? 2 : 1;
+  const size_t le = (size_t)n * ldl, re = (size_t)rows * ldr;
+  const int64_t ldsl = ldl / 32, ldsr = ldr / 32;
+  const size_t lsc = fp8 ? (size_t)n * ldsl : 0, rsc = fp8 ? (size_t)rows * ldsr : 0;
+  const size_t o_l = 0, o_r = align_up(le * esz * planes, 256), o_ls = o_r + align_up(re * esz * planes, 256),
+               o_rs = o_ls + align_up(lsc, 256), total = o_rs + align_up(rsc, 256);
+  sk.win.reserve(total);
+  char* base = sk.win.as<char>();
+  {
+    Timed t(c, "window");
+    dim3 grid((unsigned)cdiv(rows, kQ), (unsigned)cdiv(n, kQ));
+    if (fp8) {
+      hipLaunchKernelGGL(k_quant_fp8, grid, dim3(kThreads), 0, st, H, ldh, rows, n, (uint8_t*)(base + o_l), ldl,
+                         (uint8_t*)(base + o_ls), ldsl, (uint8_t*)(base + o_r), ldr, (uint8_t*)(base + o_rs), ldsr);
+    } else if (split) {
+      __bf16* L = (__bf16*)(base + o_l);
+      __bf16* R = (__bf16*)(base + o_r);
+      hipLaunchKernelGGL(k_quant_bf16<true>, grid, dim3(kThreads), 0, st, H, ldh, rows, n, L, L + le, ldl, R, R + re,
+                         ldr);
+    } else {
+      hipLaunchKernelGGL(k_quant_bf16<false>, grid, dim3(kThreads), 0, st, H, ldh, rows, n, (__bf16*)(base + o_l),
+                         nullptr, ldl, (__bf16*)(base + o_r), nullptr, ldr);
+    }
+    check_launch("k_quant");
+  }
+  const size_t ue = (size_t)k * sk.lp_ldups, oe = (size_t)(k + s) * sk.lp_ldomps;
   {
     Timed t(c, "sketch_left");
-    LpArgs a{};
+    NtArgs a{};
     a.M = k; a.N = n; a.K = rows;
-    a.map_hi = sk.lp_ups.as<__bf16>();
-    a.map_lo = split ? sk.lp_ups.as<__bf16>() + ups_elems : nullptr;
-    a.ldmap = sk.lp_ldups;
-    a.map_koff = row0;
-    a.data = H; a.ldd = ldh;
+    a.ah = sk.lp_ups.p;
+    a.al = split ? (const void*)(sk.lp_ups.as<__bf16>() + ue) : nullptr;
+    a.as = fp8 ? sk.lp_ups.as<uint8_t>() + ue : nullptr;
+    a.lda = sk.lp_ldups; a.koff_a = row0; a.lds_a = sk.lp_ldsups;
+    a.bh = base + o_l;
+    a.bl = split ? (const void*)((const __bf16*)(base + o_l) + le) : nullptr;
+    a.bs = fp8 ? (const uint8_t*)(base + o_ls) : nullptr;
+    a.ldb = ldl; a.lds_b = ldsl;
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
-    if (split) gemm_bf16<true, true>(st, a); else gemm_bf16<false, true>(st, a);
+    gemm_nt(st, prec, a);
   }
   {
     Timed t(c, "sketch_right");
-    LpArgs a{};
+    NtArgs a{};
     a.M = rows; a.N = k + s; a.K = n;
-    a.map_hi = sk.lp_omps.as<__bf16>();
-    a.map_lo = split ? sk.lp_omps.as<__bf16>() + omps_elems : nullptr;
-    a.ldmap = sk.lp_ldomps;
-    a.map_koff = 0;
-    a.data = H; a.ldd = ldh;
+    a.ah = base + o_r;
+    a.al = split ? (const void*)((const __bf16*)(base + o_r) + re) : nullptr;
+    a.as = fp8 ? (const uint8_t*)(base + o_rs) : nullptr;
+    a.lda = ldr; a.koff_a = 0; a.lds_a = ldsr;
+    a.bh = sk.lp_omps.p;
+    a.bl = split ? (const void*)(sk.lp_omps.as<__bf16>() + oe) : nullptr;
+    a.bs = fp8 ? sk.lp_omps.as<uint8_t>() + oe : nullptr;
+    a.ldb = sk.lp_ldomps; a.lds_b = sk.lp_ldsomps;
     a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
     a.out.C2 = G; a.out.ldc2 = rows; a.out.alpha2 = 1.0; a.out.beta2 = 0.0;
-    if (split) gemm_bf16<true, false>(st, a); else gemm_bf16<false, false>(st, a);
+    gemm_nt(st, prec, a);
   }
   {
     Timed t(c, "sketch_core");

@@ -426,3 +426,19 @@ def residual_fro(A, U, S, V, window: int = 4096, ctx: Optional[Context] = None) 
     L.call("skema_residual_fro_dense", ctx.h, m, n, r, _ptr(A), _ld(A), _ptr(U), _ld(U), _ptr(S), _ptr(V), _ld(V),
            window, C.byref(e2), C.byref(a2))
     return e2.value, a2.value
+
+
+def rbf_sketch_loss_grad(points, gamma: float, omega: _DimRedux, phi: _DimRedux, psi: _DimRedux,
+                         window: int = 4096, precision: str = "fp64", grad: bool = True,
+                         ctx: Optional[Context] = None):
+    """L = ||K Omega^T||_F^2 + ||Phi K Psi^T||_F^2 for the Gaussian RBF kernel
+    K of the n x d device point set, with dL/dX (n x d, column-major device
+    tensor) and dL/dgamma, kernel tiles recomputed in the backward pass."""
+    ctx = ctx or default_context()
+    n, d = points.shape
+    loss, dg = C.c_double(), C.c_double()
+    dX = torch.empty((d, n), dtype=torch.float64, device=points.device).t() if grad else None
+    L.call("skema_rbf_sketch_loss_grad", ctx.h, _ptr(points), n, d, _ld(points), float(gamma), omega.h, phi.h, psi.h,
+           window, L.PRECISIONS[precision], C.byref(loss), _ptr(dX) if grad else None, _ld(dX) if grad else n,
+           C.byref(dg))
+    return loss.value, dX, dg.value
