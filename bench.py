@@ -43,6 +43,10 @@ WORKLOADS = {
     # Gaussians, means N(0, 4I), covariance I, FP32 draws widened to FP64,
     # seed 5), gamma = 1/d.
     "rbf-100000x128": dict(m=100000, n=100000, d=128, rank=50, k=201, s=403, window=2048, clusters=10, seed=5),
+    # Loss L = ||K Omega^T||^2 + ||Phi K Psi^T||^2 of the RBF kernel of 32768
+    # points in d = 64 (same mixture, seed 9), gamma = 0.5/d, k = 201,
+    # s = 403, forward windows of 4096 rows; one step = loss + dL/dX + dL/dgamma.
+    "rbf-adjoint-32768x64": dict(m=32768, n=32768, d=64, k=201, s=403, window=4096, clusters=10, seed=9),
 }
 
 PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0, "fp8": 5000.0}
@@ -266,6 +270,58 @@ def bench_rbf(args, S, torch):
     print(json.dumps(out))
 
 
+def bench_adjoint(args, S, torch):
+    """Forward loss and the hand-written adjoint (dL/dX, dL/dgamma), FP64."""
+    cfg = WORKLOADS[args.workload]
+    X = make_points(cfg)
+    Xd = S.colmajor(torch.as_tensor(X, device="cuda"))
+    ctx = S.Context(0)
+    n, d, k, s = cfg["m"], cfg["d"], cfg["k"], cfg["s"]
+    gamma = 0.5 / d
+    om = S.GaussDimRedux(k, n, seed=0, ctx=ctx)
+    ph = S.GaussDimRedux(s, n, seed=1, ctx=ctx)
+    ps = S.GaussDimRedux(s, n, seed=2, ctx=ctx)
+
+    def run(grad, steps):
+        out = None
+        for _ in range(steps):
+            out = S.rbf_sketch_loss_grad(Xd, gamma, om, ph, ps, window=cfg["window"], grad=grad, ctx=ctx)
+        return out
+
+    run(True, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(False, args.steps)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / args.steps
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    loss, dX, dg = run(True, args.steps)
+    torch.cuda.synchronize()
+    t_all = (time.perf_counter() - t0) / args.steps
+    ctx.set_profiling(False)
+    cnt, tot = ctx.kernel_stats("rbf_adjoint")
+    adj_ms = tot / cnt
+    h = k + s
+    # tile pass: (F_I R_J^T) with inner 2h, the Gram over d, and H X_J over d
+    flops = float(n) * n * (4.0 * h + 4.0 * d)
+    achieved = flops / (adj_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": n / t_all, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_all * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generators; no dataset)",
+        "config": {"workload": f"{args.workload}: loss + adjoint of the fused RBF sketch, {n} points in d={d}, "
+                               f"gamma=0.5/d, k={k} s={s}, forward windows {cfg['window']}",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": "rbf_adjoint", "achieved": achieved, "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": achieved / 78.6, "traffic": None,
+                     "flops_per_launch": flops, "avg_launch_ms": adj_ms},
+        "forward_ms": t_fwd * 1e3, "loss": loss, "dgamma": dg, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,6 +344,8 @@ def main():
 
     if args.workload.startswith("sparse-"):
         return bench_sparse(args, S, torch)
+    if args.workload.startswith("rbf-adjoint"):
+        return bench_adjoint(args, S, torch)
     if args.workload.startswith("rbf-"):
         return bench_rbf(args, S, torch)
     cfg = WORKLOADS[args.workload]

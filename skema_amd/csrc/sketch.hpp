@@ -103,7 +103,7 @@ struct Sketch {
   Csr sp_omps, sp_ompsT;
   // Low-precision map copies (precision forms).
   DevBuf lp_ups, lp_omps, lp_phi, lp_scale;
-  int64_t lp_ldups = 0, lp_ldomps = 0, lp_ldsups = 0, lp_ldsomps = 0;
+  int64_t lp_ldups = 0, lp_ldomps = 0;
   // State.
   DevBuf X, Y, Z;  // k x n, m x k, s x s
   DevBuf G;        // window x s scratch: H * Psi^T

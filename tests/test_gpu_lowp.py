@@ -3,7 +3,8 @@
 Tolerances are normwise relative errors against the FP64 oracle on the same
 inputs and maps: single-term BF16 keeps 8 significant bits per operand,
 split-BF16 about 16 (then FP32 accumulation), FP32 24, e4m3 FP8 4 with one
-power-of-two scale per 32 elements.
+power-of-two scale per row of every operand (per row of a map, per row and
+per column of a window).
 """
 import numpy as np
 import pytest
@@ -78,7 +79,7 @@ def test_lowp_window_offsets_unaligned(S):
     map-load path."""
     print("case lowp unaligned windows", flush=True)
     A, p, osk = _case(203, 130, 29)
-    for form in ["bf16x3", "bf16"]:
+    for form in ["bf16x3", "bf16", "fp32", "fp8"]:
         g = _run(S, A, p, osk, form)
         X, Y, Z = g.state()
         assert O.rel_err(X, osk.X) <= TOLS[form]

@@ -214,6 +214,24 @@ def test_rbf_fused_odd_shapes(S, n, d, w, rank):
     assert O.rel_err(Xg, osk.X) <= TOL and O.rel_err(Yg, osk.Y) <= TOL and O.rel_err(Zg, osk.Z) <= TOL
 
 
+@pytest.mark.parametrize("n,d,w,rank", [(2600, 64, 300, 10), (1200, 20, 256, 60)])
+def test_rbf_fused_split_points_and_wide_sketches(S, n, d, w, rank):
+    """Point counts large enough that the right-hand product splits the
+    points over blocks (partial sums reduced afterwards), and k + s above 640 /
+    k above 208 so both fused kernels tile their columns over several blocks."""
+    print(f"case rbf fused large n={n} d={d} w={w} r={rank}", flush=True)
+    rng = np.random.default_rng(n + d)
+    X = rng.standard_normal((n, d)) + 2.0 * rng.standard_normal((10, d))[rng.integers(0, 10, n)]
+    p = O.AlgParams(matrix_m=n, matrix_n=n, rank=rank, window=w, kernel_func="gaussrbf", kernel_gamma=1.0 / d,
+                    sketch_nu=0.9, sketch_eta=1.1)
+    osk = O.SketchySVD(p)
+    osk.linear_update(X)
+    g = S.SketchySVD(to_gpu_params(p, nfeat=d), maps=gpu_maps(_omaps(osk)))
+    g.linear_update(dev(X))
+    Xg, Yg, Zg = g.state()
+    assert O.rel_err(Xg, osk.X) <= TOL and O.rel_err(Yg, osk.Y) <= TOL and O.rel_err(Zg, osk.Z) <= TOL
+
+
 def test_cli_driver_on_reference_fixtures(golden_dir, tmp_path):
     """The C++ driver (include/skema_mi355x.hpp layer) on the reference's own
     fixture files: MatrixMarket coordinate (dense and --sparse) and the
