@@ -358,13 +358,20 @@ void rbf_fused_right_f64(hipStream_t st, RbfArgs a, DevBuf* ws) {
   const int tn = pick_tiles(kTnChoices, cdiv(cdiv(a.N, 16), 8));
   const int64_t col_blocks = cdiv(a.N, 128 * tn);
   a.tiles_m = (int)cdiv(a.M, 32);
-  // split the points so the grid holds about one block per CU, each split
-  // at least 1024 points long
+  // Split the points over blocks: one 512-thread block fills a CU, so the
+  // time is about (rounds of 256 blocks) x (points per split), plus the
+  // partial-sum traffic of each extra split (M x N doubles written and read,
+  // worth about M N / 65536 points of block time).  Take the split count
+  // that minimises it, splits at least 1024 points long.
   const int64_t per_split = a.tiles_m * col_blocks;
   int64_t splits = 1;
-  if (ws && per_split < 256) {
-    splits = std::max<int64_t>(1, (256 + per_split / 2) / per_split);
-    splits = std::min<int64_t>(splits, std::max<int64_t>(1, a.K / 1024));
+  if (ws) {
+    int64_t best = INT64_MAX;
+    const int64_t max_splits = std::max<int64_t>(1, std::min<int64_t>(64, a.K / 1024));
+    for (int64_t sp = 1; sp <= max_splits; ++sp) {
+      const int64_t cost = cdiv(per_split * sp, 256) * cdiv(a.K, sp) + (sp > 1 ? sp * (a.M * a.N / 65536 + 1) : 0);
+      if (cost < best) { best = cost; splits = sp; }
+    }
   }
   a.k_per_split = cdiv(cdiv(a.K, splits), 16) * 16;
   splits = cdiv(a.K, a.k_per_split);
