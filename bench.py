@@ -33,6 +33,11 @@ METRIC = "rows/s of SketchySVD window update, mixed vs FP64 plain at equal rel. 
 WORKLOADS = {
     "dense-65536x16384": dict(m=65536, n=16384, rank=100, k=401, s=803, window=4096, p=720, decay=0.95,
                               noise_var=1e-4, seeds=(6, 7, 8)),
+    # The same matrix with the noise level read as a standard deviation
+    # (variance 1e-8): signal and noise energies are comparable, so the
+    # reduced forms' rounding shows in the rank-r error.
+    "dense-65536x16384-lownoise": dict(m=65536, n=16384, rank=100, k=401, s=803, window=4096, p=720, decay=0.95,
+                                       noise_var=1e-8, seeds=(6, 7, 8)),
     "dense-2000x1000": dict(m=2000, n=1000, rank=10, k=41, s=83, window=100, p=1000, decay=0.8,
                             noise_var=1e-6, seeds=(1, 2, 3)),
     # CSR A with density 1e-3 (uniform positions, duplicates merged, N(0,1)
@@ -76,6 +81,25 @@ def make_dense(cfg, torch):
     del U, V
     torch.cuda.synchronize()
     return At.t()
+
+
+def optimal_tail(A, r, torch, iters=8, over=100):
+    """sqrt(sum_{i>r} sigma_i^2) / ||A||_F by block subspace iteration on the
+    device (FP64 torch products; a measurement aid, not the sketch path):
+    ||A||^2 minus the top-r Ritz values squared of an (r + over)-dimensional
+    subspace refined `iters` times."""
+    n = A.shape[1]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(12345)
+    Q = torch.randn((n, r + over), dtype=torch.float64, device="cuda", generator=g)
+    Q, _ = torch.linalg.qr(A @ Q)
+    for _ in range(iters):
+        Z, _ = torch.linalg.qr(A.t() @ Q)
+        Q, _ = torch.linalg.qr(A @ Z)
+    sv = torch.linalg.svdvals(Q.t() @ A)
+    a2 = float((A * A).sum())
+    top = float((sv[:r] ** 2).sum())
+    return math.sqrt(max(a2 - top, 0.0) / a2)
 
 
 def dominant_kernel(cfg):
@@ -222,51 +246,89 @@ def bench_sparse(args, S, torch):
     print(json.dumps(out))
 
 
+def rbf_residual(S, Xd, gamma, U, Sv, V, window, ctx):
+    """||K - U diag(S) V^T||_F / ||K||_F with K streamed in materialised
+    windows (the parity generator), one window at a time."""
+    n = Xd.shape[0]
+    e2 = a2 = 0.0
+    for r0 in range(0, n, window):
+        r1 = min(n, r0 + window)
+        Kw = S.gauss_rbf_window(Xd, gamma, (r0, r1), ctx=ctx)
+        e, a = S.residual_fro(Kw, U[r0:r1], Sv, V, window=window, ctx=ctx)
+        e2 += e
+        a2 += a
+        del Kw
+    return math.sqrt(e2 / a2)
+
+
 def bench_rbf(args, S, torch):
-    """Fused RBF window + Gauss sketch (K never written to HBM)."""
+    """Fused RBF window + Gauss sketch (K never written to HBM), FP64 and
+    BF16 forms; value = the faster form whose rank-r error is within 1.05x of
+    the FP64 form's."""
     cfg = WORKLOADS[args.workload]
     X = make_points(cfg)
     Xd = S.colmajor(torch.as_tensor(X, device="cuda"))
     ctx = S.Context(0)
     n, d, k, s, w = cfg["m"], cfg["d"], cfg["k"], cfg["s"], cfg["window"]
-    p = S.AlgParams(matrix_m=n, matrix_n=n, rank=cfg["rank"], sketch_range=k, sketch_core=s, window=w,
-                    kernel_func="gaussrbf", kernel_gamma=1.0 / d, nfeat=d)
-    sk = S.SketchySVD(p, ctx=ctx)
-    for _ in range(args.warmup):
-        sk.linear_update(Xd)
-    torch.cuda.synchronize()
-    ctx.reset_stats()
-    ctx.set_profiling(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sk.linear_update(Xd)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    ctx.set_profiling(False)
-    stats = {kname: ctx.kernel_stats(kname) for kname in ("window", "rbf_left", "rbf_right", "sketch_core")}
-    U, Sv, V = sk.low_rank_approx()
-    full = w  # flops of a full window; the dominant kernel's average is over all windows
+    gamma = 1.0 / d
+    p64 = S.AlgParams(matrix_m=n, matrix_n=n, rank=cfg["rank"], sketch_range=k, sketch_core=s, window=w,
+                      kernel_func="gaussrbf", kernel_gamma=gamma, nfeat=d)
+    maps = S.make_maps(p64, ctx)
+    forms = [f for f in args.forms.split(",") if f in ("fp64", "bf16")] or ["fp64"]
+    if "fp64" not in forms:
+        forms.insert(0, "fp64")
+    res = {}
+    for form in forms:
+        print(f"[bench] rbf form {form}", file=sys.stderr, flush=True)
+        p = S.AlgParams(matrix_m=n, matrix_n=n, rank=cfg["rank"], sketch_range=k, sketch_core=s, window=w,
+                        kernel_func="gaussrbf", kernel_gamma=gamma, nfeat=d, precision=form)
+        sk = S.SketchySVD(p, maps=maps, ctx=ctx)
+        for _ in range(args.warmup):
+            sk.linear_update(Xd)
+        torch.cuda.synchronize()
+        ctx.reset_stats()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sk.linear_update(Xd)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        ctx.set_profiling(False)
+        stats = {kname: ctx.kernel_stats(kname) for kname in ("window", "rbf_left", "rbf_right", "sketch_core")}
+        U, Sv, V = sk.low_rank_approx(device=True)
+        err = rbf_residual(S, Xd, gamma, U, Sv, V, w, ctx)
+        sk.close()
+        res[form] = dict(form=form, ms=dt * 1e3, rows_per_s=n / dt, err=err,
+                         kernel_ms={kn: (v[1] / v[0] if v[0] else None) for kn, v in stats.items()},
+                         right_avg_ms=stats["rbf_right"][1] / stats["rbf_right"][0],
+                         svals_head=[float(x) for x in Sv[:5].cpu()])
+        print(f"[bench] rbf {form}: {dt * 1e3:.1f} ms/step, err {err:.6e}", file=sys.stderr, flush=True)
+    base = res["fp64"]
+    best = base
+    for f, r in res.items():
+        if f != "fp64" and r["err"] <= 1.05 * base["err"] and r["rows_per_s"] > best["rows_per_s"]:
+            best = r
     nwin = math.ceil(n / w)
     avg_rows = n / nwin
     # rbf_right per launch: sketch 2 w n (k+s) + Gram 2 w n d (algorithmic, one Gram)
     flops_alg = 2.0 * avg_rows * n * (k + s) + 2.0 * avg_rows * n * d
-    avg_ms = stats["rbf_right"][1] / stats["rbf_right"][0]
-    achieved = flops_alg / (avg_ms * 1e-3) / 1e12
+    achieved = flops_alg / (best["right_avg_ms"] * 1e-3) / 1e12
+    peak = PEAKS_TFLOPS[best["form"]]
     out = {
-        "metric": METRIC, "value": n / dt, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generators; no dataset)",
+        "metric": METRIC, "value": best["rows_per_s"], "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": DTYPE[best["form"]], "data": "synthetic (seeded generators; no dataset)",
         "config": {"workload": f"{args.workload}: Gaussian RBF of {n} points in d={d} (10-cluster mixture), "
                                f"gamma=1/d, r={cfg['rank']} k={k} s={s} window={w}, fused generator",
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": "rbf_right", "achieved": achieved, "peak": 78.6,
-                     "unit": "TFLOP/s", "frac": achieved / 78.6, "traffic": None,
-                     "flops_per_launch": flops_alg, "avg_launch_ms": avg_ms},
-        "kernel_ms": {kname: (v[1] / v[0] if v[0] else None) for kname, v in stats.items()},
-        "svals_head": [float(x) for x in Sv[:5]],
+                   "form": best["form"], "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": "rbf_right", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "flops_per_launch": flops_alg, "avg_launch_ms": best["right_avg_ms"]},
+        "forms": {f: {"rows_per_s": r["rows_per_s"], "ms_per_step": r["ms"], "rel_fro_err": r["err"],
+                      "err_vs_fp64": r["err"] / base["err"], "speedup_vs_fp64": r["rows_per_s"] / base["rows_per_s"],
+                      "kernel_ms": r["kernel_ms"], "svals_head": r["svals_head"]} for f, r in res.items()},
         "cpu_baseline": None,
     }
-    del full
     print(json.dumps(out))
 
 
@@ -371,6 +433,8 @@ def main():
     for form, r in results.items():
         if form != "fp64" and r["rel_fro_err"] <= 1.05 * base["rel_fro_err"] and r["rows_per_s"] > best["rows_per_s"]:
             best = r
+    print("[bench] optimal rank-r tail (subspace iteration)", file=sys.stderr, flush=True)
+    tail = optimal_tail(A, cfg["rank"], torch)
     name, flops = dominant_kernel(cfg)
     form = best["form"]
     mfma_flops = flops * MFMA_PER_FLOP[form]
@@ -394,9 +458,9 @@ def main():
         "dtype": DTYPE[form],
         "data": "synthetic (seeded generators; no dataset)",
         "config": {
-            "workload": f"{args.workload}: dense A = U diag(0.95^i) V^T + N(0,1e-4) noise, r={cfg['rank']} "
-                        f"k={cfg['k']} s={cfg['s']} window={cfg['window']}, Gaussian maps seeds 0-3, "
-                        f"A resident in HBM" if args.workload.startswith("dense-65536") else args.workload,
+            "workload": f"{args.workload}: dense A = U diag({cfg['decay']}^i) V^T + N(0, {cfg['noise_var']:g}) noise "
+                        f"(variance), r={cfg['rank']} k={cfg['k']} s={cfg['s']} window={cfg['window']}, "
+                        f"Gaussian maps seeds 0-3, A resident in HBM",
             "m": cfg["m"], "n": cfg["n"], "rank": cfg["rank"], "range": cfg["k"], "core": cfg["s"],
             "window": cfg["window"], "form": form, "parallelism": "single GPU",
         },
@@ -408,9 +472,11 @@ def main():
         "cpu_baseline": cpu,
         "forms": {f: {"rows_per_s": r["rows_per_s"], "ms_per_step": r["ms_per_step"],
                       "rel_fro_err": r["rel_fro_err"], "err_vs_fp64": r["rel_fro_err"] / base["rel_fro_err"],
+                      "err_vs_optimal": r["rel_fro_err"] / tail if tail > 0 else None,
                       "speedup_vs_fp64": r["rows_per_s"] / base["rows_per_s"], "kernel_ms": r["kernel_ms"]}
                   for f, r in results.items()},
         "flops_per_row": flops_per_row(cfg),
+        "optimal_rank_r_err": tail,
     }
     print(json.dumps(out))
 

@@ -17,6 +17,7 @@ from .sketchy import (  # noqa: F401
     gauss_rbf_window,
     make_maps,
     map_shapes,
+    rbf_sketch_loss_grad,
     residual_fro,
 )
 

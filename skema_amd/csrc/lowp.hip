@@ -10,8 +10,11 @@
 // (M x K) and B (N x K) both K-contiguous, fed by 16-byte loads and K-major
 // LDS tiles.  The maps are converted once at sketch creation.  Products
 // accumulate in FP32 on the matrix cores; the epilogue widens to FP64 and
-// applies the nu/eta update into the FP64 state.  The small core product
-// Z += Phi[:, idx] * G and the whole finalize stay FP64.
+// applies the nu/eta update into the FP64 state.  The right product writes G
+// (the window's A_w Psi^T block) straight in the core product's element type,
+// column-major, so the core product Z += Phi[:, idx] * G is one more NT GEMM
+// (BF16 for the bf16 and fp8 forms, split-BF16 and FP32 for theirs).  The
+// whole finalize stays FP64.
 //
 // Forms:
 //   fp32   : v_mfma_f32_16x16x4_f32 on FP32-rounded operands
@@ -24,6 +27,7 @@
 //            max |x| so it lands in [128, 256) before rounding; the FP32
 //            accumulator is multiplied by the two scales in the FP64 epilogue.
 #include <cmath>
+#include <type_traits>
 
 #include "lowp.hpp"
 #include "util.hpp"
@@ -227,357 +231,293 @@ struct NtArgs {
   const float* amax_b;  // fp8: same for the rows of B
   int64_t ldb;
   GemmOut out;
+  // Reduced output of the columns j >= out.nsplit (G of the right product):
+  // gtype 1 = bf16 (gh), 2 = bf16 hi + lo (gh, gl), 3 = fp32 (gh); element
+  // (i, j - nsplit) at i + (j - nsplit) * ldg.  gtype 0 writes out.C2.
+  int gtype;
+  void* gh;
+  void* gl;
+  int64_t ldg;
   int tiles_m;
+  int64_t k_per_split;
+  float* part;  // FP32 partials [split][M x N] when K is split
 };
 
-template <int BM, int BN, bool SPLIT>
-__global__ __launch_bounds__(kThreads) void k_gemm_nt_bf16(NtArgs g) {
-  constexpr int BK = 64, LDK = BK + 8;
+__device__ __forceinline__ void store_nt(const NtArgs& g, int64_t i, int64_t j, double v) {
+  if (g.gtype != 0 && j >= g.out.nsplit) {
+    const int64_t e = i + (j - g.out.nsplit) * g.ldg;
+    if (g.gtype == 3) {
+      static_cast<float*>(g.gh)[e] = (float)v;
+    } else {
+      const float x = (float)v;
+      const __bf16 h = (__bf16)x;
+      static_cast<__bf16*>(g.gh)[e] = h;
+      if (g.gtype == 2) static_cast<__bf16*>(g.gl)[e] = (__bf16)(x - (float)h);
+    }
+    return;
+  }
+  store_out(g.out, i, j, v);
+}
+
+// ---- the NT kernel -------------------------------------------------------------
+// Operand forms.  Every form stages BK elements of K per pipeline step as
+// 16-byte chunks into a K-contiguous LDS row of RS bytes (BK * E + 16 pad), and
+// feeds the matrix cores in sub-steps that read 16-byte fragments:
+//   bf16   : 16x16x32 bf16, lane l reads row l%16, K 8*(l/16) .. +7 of a 32-slice
+//   bf16x3 : same on hi and lo planes, three products per sub-step
+//   fp32   : 16x16x4 f32 x 4, lane l reads K 4*(l/16) .. +3 of a 16-slice and
+//            issues element q of A against element q of B (every K index of
+//            the slice is met once, so the sum is the same contraction)
+//   fp8    : 16x16x128 f8f6f4 with unit block scales, lane l reads 32 bytes at
+//            K 32*(l/16); the row scales multiply the sum in the epilogue.
+struct FormBf16 {
+  static constexpr int E = 1 * 2, BK = 64, NP = 1, KS = 32;
+  __device__ static void mma(f4v& acc, const uint8_t* const* pa, const uint8_t* const* pb, int lq) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa[0] + 16 * lq);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb[0] + 16 * lq);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  }
+};
+struct FormBf16x3 {
+  static constexpr int E = 2, BK = 32, NP = 2, KS = 32;
+  __device__ static void mma(f4v& acc, const uint8_t* const* pa, const uint8_t* const* pb, int lq) {
+    const bf16x8 ah = *reinterpret_cast<const bf16x8*>(pa[0] + 16 * lq);
+    const bf16x8 bh = *reinterpret_cast<const bf16x8*>(pb[0] + 16 * lq);
+    const bf16x8 al = *reinterpret_cast<const bf16x8*>(pa[1] + 16 * lq);
+    const bf16x8 bl = *reinterpret_cast<const bf16x8*>(pb[1] + 16 * lq);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+  }
+};
+struct FormF32 {
+  static constexpr int E = 4, BK = 32, NP = 1, KS = 16;
+  __device__ static void mma(f4v& acc, const uint8_t* const* pa, const uint8_t* const* pb, int lq) {
+    const f4v a = *reinterpret_cast<const f4v*>(pa[0] + 16 * lq);
+    const f4v b = *reinterpret_cast<const f4v*>(pb[0] + 16 * lq);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b[q], acc, 0, 0, 0);
+  }
+};
+struct FormFp8 {
+  static constexpr int E = 1, BK = 128, NP = 1, KS = 128;
+  __device__ static void mma(f4v& acc, const uint8_t* const* pa, const uint8_t* const* pb, int lq) {
+    const i4v* qa = reinterpret_cast<const i4v*>(pa[0] + 32 * lq);
+    const i4v* qb = reinterpret_cast<const i4v*>(pb[0] + 32 * lq);
+    const i4v a0 = qa[0], a1 = qa[1], b0 = qb[0], b1 = qb[1];
+    const i8v a = i8v{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+    const i8v b = i8v{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+    acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc, 0, 0, 0, 127, 0, 127);
+  }
+};
+
+template <class F>
+__device__ __forceinline__ double row_scale(const float* amax, int64_t r) {
+  if constexpr (std::is_same<F, FormFp8>::value) return ldexp(1.0, block_exponent(amax[r]));
+  else return 1.0;
+}
+
+// C(i, j) = sum_k A(i, k) B(j, k) over this block's K range (blockIdx.z of
+// g.splits), BM x BN per 256-thread block as 2 x 2 waves.  LDS is double
+// buffered: the chunks of step t+1 sit in registers while step t computes,
+// are stored into the other buffer, and the loads of step t+2 are issued;
+// one barrier per step.  With one split the epilogue writes the output;
+// otherwise the FP32 partial goes to g.part[z] and k_nt_reduce finishes.
+template <class F, int BM, int BN>
+__global__ __launch_bounds__(kThreads) void k_nt(NtArgs g) {
+  constexpr int BK = F::BK, E = F::E, NP = F::NP;
+  constexpr int RS = BK * E + 16;                 // LDS row stride (bytes)
+  constexpr int CPR = BK * E / 16;                // 16-byte chunks per row and step
+  constexpr int CA = BM * CPR / kThreads, CB = BN * CPR / kThreads;
   constexpr int TM = BM / 32, TN = BN / 32;
-  constexpr int NP = SPLIT ? 2 : 1;
-  constexpr int CA = BM * BK / 8 / kThreads, CB = BN * BK / 8 / kThreads;
-  __shared__ __attribute__((aligned(16))) __bf16 sA[NP][BM][LDK];
-  __shared__ __attribute__((aligned(16))) __bf16 sB[NP][BN][LDK];
+  static_assert(CA * kThreads == BM * CPR && CB * kThreads == BN * CPR, "tile / thread mismatch");
+  __shared__ __attribute__((aligned(16))) uint8_t sA[2][NP][BM * RS];
+  __shared__ __attribute__((aligned(16))) uint8_t sB[2][NP][BN * RS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
   const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
-  const __bf16* A[2] = {static_cast<const __bf16*>(g.ah), static_cast<const __bf16*>(g.al)};
-  const __bf16* B[2] = {static_cast<const __bf16*>(g.bh), static_cast<const __bf16*>(g.bl)};
+  const int64_t kbeg = (int64_t)blockIdx.z * g.k_per_split;
+  const int64_t kend = (g.K < kbeg + g.k_per_split) ? g.K : kbeg + g.k_per_split;
+  const int steps = (int)((kend - kbeg + BK - 1) / BK);
+  const uint8_t* Ap[2] = {static_cast<const uint8_t*>(g.ah), static_cast<const uint8_t*>(g.al)};
+  const uint8_t* Bp[2] = {static_cast<const uint8_t*>(g.bh), static_cast<const uint8_t*>(g.bl)};
 
   f4v acc[TM][TN];
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
-  bf16x8 ra[NP][CA], rb[NP][CB];
-
-  auto load_chunk = [&](const __bf16* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
-    const __bf16* src = base + row * ld + koff + k;
-    bf16x8 v;
-    if (row < nrows && k + 8 <= g.K && (((uintptr_t)src) & 15) == 0) {
-      v = *reinterpret_cast<const bf16x8*>(src);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = (row < nrows && k + q < g.K) ? src[q] : (__bf16)0.0f;
-    }
-    return v;
-  };
-  auto gload = [&](int64_t k0) {
-#pragma unroll
-    for (int e = 0; e < CA; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
-#pragma unroll
-      for (int h = 0; h < NP; ++h) ra[h][e] = load_chunk(A[h], m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
-    }
-#pragma unroll
-    for (int e = 0; e < CB; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
-#pragma unroll
-      for (int h = 0; h < NP; ++h) rb[h][e] = load_chunk(B[h], n0 + row, g.N, g.ldb, 0, k0 + kc);
-    }
-  };
-  auto sstore = [&]() {
-#pragma unroll
-    for (int e = 0; e < CA; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
-#pragma unroll
-      for (int h = 0; h < NP; ++h) *reinterpret_cast<bf16x8*>(&sA[h][row][kc]) = ra[h][e];
-    }
-#pragma unroll
-    for (int e = 0; e < CB; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
-#pragma unroll
-      for (int h = 0; h < NP; ++h) *reinterpret_cast<bf16x8*>(&sB[h][row][kc]) = rb[h][e];
-    }
-  };
-
-  gload(0);
-  for (int64_t k0 = 0; k0 < g.K; k0 += BK) {
-    __syncthreads();
-    sstore();
-    __syncthreads();
-    if (k0 + BK < g.K) gload(k0 + BK);
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += 32) {
-      const int kk = ks + 8 * (lane >> 4);
-      bf16x8 a[NP][TM], b[NP][TN];
-#pragma unroll
-      for (int h = 0; h < NP; ++h) {
-#pragma unroll
-        for (int x = 0; x < TM; ++x)
-          a[h][x] = *reinterpret_cast<const bf16x8*>(&sA[h][wm * (BM / 2) + x * 16 + (lane & 15)][kk]);
-#pragma unroll
-        for (int y = 0; y < TN; ++y)
-          b[h][y] = *reinterpret_cast<const bf16x8*>(&sB[h][wn * (BN / 2) + y * 16 + (lane & 15)][kk]);
-      }
-#pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y) {
-          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[0][y], acc[x][y], 0, 0, 0);
-          if (SPLIT) {
-            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[NP - 1][y], acc[x][y], 0, 0, 0);
-            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[NP - 1][x], b[0][y], acc[x][y], 0, 0, 0);
-          }
-        }
-    }
-  }
-  // 16x16 f32 MFMA result: lane l holds rows 4*(l/16)+v of column l%16.
 #pragma unroll
   for (int x = 0; x < TM; ++x)
 #pragma unroll
-    for (int y = 0; y < TN; ++y)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t i = m0 + wm * (BM / 2) + x * 16 + 4 * (lane >> 4) + v;
-        const int64_t j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
-        if (i < g.M && j < g.N) store_out(g.out, i, j, (double)acc[x][y][v]);
-      }
-}
+    for (int y = 0; y < TN; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+  i4v ra[NP][CA], rb[NP][CB];
 
-// FP32 NT GEMM on v_mfma_f32_16x16x4_f32.  Each lane reads 4 consecutive K
-// elements of its row (K quarter l/16 of a 16-wide chunk) with one 16-byte LDS
-// read and issues 4 MFMAs, element q of A with element q of B: every K index
-// of the chunk is met exactly once, so the sum is the same contraction.
-template <int BM, int BN>
-__global__ __launch_bounds__(kThreads) void k_gemm_nt_f32(NtArgs g) {
-  constexpr int BK = 32, LDK = BK + 4;
-  constexpr int TM = BM / 32, TN = BN / 32;
-  constexpr int CA = BM * BK / 4 / kThreads, CB = BN * BK / 4 / kThreads;
-  __shared__ __attribute__((aligned(16))) float sA[BM][LDK];
-  __shared__ __attribute__((aligned(16))) float sB[BN][LDK];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
-  const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
-  const float* A = static_cast<const float*>(g.ah);
-  const float* B = static_cast<const float*>(g.bh);
-
-  f4v acc[TM][TN];
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
-  f4v ra[CA], rb[CB];
-
-  auto load4 = [&](const float* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
-    const float* src = base + row * ld + koff + k;
-    f4v v;
-    if (row < nrows && k + 4 <= g.K && (((uintptr_t)src) & 15) == 0) {
-      v = *reinterpret_cast<const f4v*>(src);
-    } else {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = (row < nrows && k + q < g.K) ? src[q] : 0.f;
-    }
-    return v;
-  };
-  auto gload = [&](int64_t k0) {
-#pragma unroll
-    for (int e = 0; e < CA; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 4), kc = (c % (BK / 4)) * 4;
-      ra[e] = load4(A, m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
-    }
-#pragma unroll
-    for (int e = 0; e < CB; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 4), kc = (c % (BK / 4)) * 4;
-      rb[e] = load4(B, n0 + row, g.N, g.ldb, 0, k0 + kc);
-    }
-  };
-  auto sstore = [&]() {
-#pragma unroll
-    for (int e = 0; e < CA; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 4), kc = (c % (BK / 4)) * 4;
-      *reinterpret_cast<f4v*>(&sA[row][kc]) = ra[e];
-    }
-#pragma unroll
-    for (int e = 0; e < CB; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 4), kc = (c % (BK / 4)) * 4;
-      *reinterpret_cast<f4v*>(&sB[row][kc]) = rb[e];
-    }
-  };
-
-  gload(0);
-  for (int64_t k0 = 0; k0 < g.K; k0 += BK) {
-    __syncthreads();
-    sstore();
-    __syncthreads();
-    if (k0 + BK < g.K) gload(k0 + BK);
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += 16) {
-      const int kk = ks + 4 * (lane >> 4);
-      f4v a[TM], b[TN];
-#pragma unroll
-      for (int x = 0; x < TM; ++x) a[x] = *reinterpret_cast<const f4v*>(&sA[wm * (BM / 2) + x * 16 + (lane & 15)][kk]);
-#pragma unroll
-      for (int y = 0; y < TN; ++y) b[y] = *reinterpret_cast<const f4v*>(&sB[wn * (BN / 2) + y * 16 + (lane & 15)][kk]);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int x = 0; x < TM; ++x)
-#pragma unroll
-          for (int y = 0; y < TN; ++y)
-            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[x][q], b[y][q], acc[x][y], 0, 0, 0);
-    }
-  }
-#pragma unroll
-  for (int x = 0; x < TM; ++x)
-#pragma unroll
-    for (int y = 0; y < TN; ++y)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t i = m0 + wm * (BM / 2) + x * 16 + 4 * (lane >> 4) + v;
-        const int64_t j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
-        if (i < g.M && j < g.N) store_out(g.out, i, j, (double)acc[x][y][v]);
-      }
-}
-
-// FP8 NT GEMM: BK = 128 = one MFMA; lane l feeds 32 bytes (row l%16, K
-// indices 32*(l/16) .. +31).  Block scales are all 2^0 (E8M0 127); the row
-// scales of A and B multiply the FP32 sum in the epilogue.
-template <int BM, int BN>
-__global__ __launch_bounds__(kThreads) void k_gemm_nt_fp8(NtArgs g) {
-  constexpr int BK = 128, LDK = BK + 16;
-  constexpr int TM = BM / 32, TN = BN / 32;
-  constexpr int CA = BM * BK / 16 / kThreads, CB = BN * BK / 16 / kThreads;
-  __shared__ __attribute__((aligned(16))) uint8_t sA[BM][LDK];
-  __shared__ __attribute__((aligned(16))) uint8_t sB[BN][LDK];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
-  const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
-  const uint8_t* A = static_cast<const uint8_t*>(g.ah);
-  const uint8_t* B = static_cast<const uint8_t*>(g.bh);
-
-  f4v acc[TM][TN];
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
-  i4v ra[CA], rb[CB];
-
+  // 16 bytes of row `row` at element k (absolute), zero past kend / nrows.
   auto load16 = [&](const uint8_t* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
-    const uint8_t* src = base + row * ld + koff + k;
+    const uint8_t* src = base + (row * ld + koff + k) * E;
     i4v v;
-    if (row < nrows && k + 16 <= g.K && (((uintptr_t)src) & 15) == 0) {
+    if (row < nrows && k + 16 / E <= kend && (((uintptr_t)src) & 15) == 0) {
       v = *reinterpret_cast<const i4v*>(src);
     } else {
-      uint8_t b[16];
+      uint8_t bytes[16];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) b[q] = (row < nrows && k + q < g.K) ? src[q] : (uint8_t)0;
-      v = *reinterpret_cast<const i4v*>(b);
+      for (int q = 0; q < 16; ++q) bytes[q] = (row < nrows && k + q / E < kend) ? src[q] : (uint8_t)0;
+      v = *reinterpret_cast<const i4v*>(bytes);
     }
     return v;
   };
   auto gload = [&](int64_t k0) {
 #pragma unroll
     for (int e = 0; e < CA; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
-      ra[e] = load16(A, m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
+      const int c = tid + e * kThreads, row = c / CPR, kc = (c % CPR) * (16 / E);
+#pragma unroll
+      for (int h = 0; h < NP; ++h) ra[h][e] = load16(Ap[h], m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
     }
 #pragma unroll
     for (int e = 0; e < CB; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
-      rb[e] = load16(B, n0 + row, g.N, g.ldb, 0, k0 + kc);
+      const int c = tid + e * kThreads, row = c / CPR, kc = (c % CPR) * (16 / E);
+#pragma unroll
+      for (int h = 0; h < NP; ++h) rb[h][e] = load16(Bp[h], n0 + row, g.N, g.ldb, 0, k0 + kc);
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](int b) {
 #pragma unroll
     for (int e = 0; e < CA; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
-      *reinterpret_cast<i4v*>(&sA[row][kc]) = ra[e];
+      const int c = tid + e * kThreads, row = c / CPR, cb = (c % CPR) * 16;
+#pragma unroll
+      for (int h = 0; h < NP; ++h) *reinterpret_cast<i4v*>(&sA[b][h][row * RS + cb]) = ra[h][e];
     }
 #pragma unroll
     for (int e = 0; e < CB; ++e) {
-      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
-      *reinterpret_cast<i4v*>(&sB[row][kc]) = rb[e];
+      const int c = tid + e * kThreads, row = c / CPR, cb = (c % CPR) * 16;
+#pragma unroll
+      for (int h = 0; h < NP; ++h) *reinterpret_cast<i4v*>(&sB[b][h][row * RS + cb]) = rb[h][e];
+    }
+  };
+  auto compute = [&](int b) {
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += F::KS) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x) {
+        const uint8_t* pa[NP];
+#pragma unroll
+        for (int h = 0; h < NP; ++h) pa[h] = &sA[b][h][(wm * (BM / 2) + x * 16 + l16) * RS + ks * E];
+#pragma unroll
+        for (int y = 0; y < TN; ++y) {
+          const uint8_t* pb[NP];
+#pragma unroll
+          for (int h = 0; h < NP; ++h) pb[h] = &sB[b][h][(wn * (BN / 2) + y * 16 + l16) * RS + ks * E];
+          F::mma(acc[x][y], pa, pb, lq);
+        }
+      }
     }
   };
 
-  gload(0);
-  for (int64_t k0 = 0; k0 < g.K; k0 += BK) {
-    __syncthreads();
-    sstore();
-    __syncthreads();
-    if (k0 + BK < g.K) gload(k0 + BK);
-    const int kb = lane >> 4;
-    i8v a[TM], b[TN];
-#pragma unroll
-    for (int x = 0; x < TM; ++x) {
-      const i4v* p = reinterpret_cast<const i4v*>(&sA[wm * (BM / 2) + x * 16 + (lane & 15)][32 * kb]);
-      const i4v lo = p[0], hi = p[1];
-      a[x] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  gload(kbeg);
+  sstore(0);
+  if (steps > 1) gload(kbeg + BK);
+  __syncthreads();
+  for (int st = 0; st < steps; ++st) {
+    const int b = st & 1;
+    compute(b);
+    if (st + 1 < steps) {
+      sstore(b ^ 1);
+      if (st + 2 < steps) gload(kbeg + (int64_t)(st + 2) * BK);
     }
-#pragma unroll
-    for (int y = 0; y < TN; ++y) {
-      const i4v* p = reinterpret_cast<const i4v*>(&sB[wn * (BN / 2) + y * 16 + (lane & 15)][32 * kb]);
-      const i4v lo = p[0], hi = p[1];
-      b[y] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-    }
-#pragma unroll
-    for (int x = 0; x < TM; ++x)
-#pragma unroll
-      for (int y = 0; y < TN; ++y)
-        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[x], b[y], acc[x][y], 0, 0, 0, 127, 0, 127);
+    __syncthreads();
   }
+
+  // 16x16 f32 MFMA result: lane l holds rows 4*(l/16)+v of column l%16.
   double sb[TN];
 #pragma unroll
   for (int y = 0; y < TN; ++y) {
-    const int64_t j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
-    sb[y] = (j < g.N) ? ldexp(1.0, block_exponent(g.amax_b[j])) : 0.0;
+    const int64_t j = n0 + wn * (BN / 2) + y * 16 + l16;
+    sb[y] = (j < g.N) ? row_scale<F>(g.amax_b, j) : 0.0;
   }
 #pragma unroll
   for (int x = 0; x < TM; ++x)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t i = m0 + wm * (BM / 2) + x * 16 + 4 * (lane >> 4) + v;
+      const int64_t i = m0 + wm * (BM / 2) + x * 16 + 4 * lq + v;
       if (i >= g.M) continue;
-      const double sa = ldexp(1.0, block_exponent(g.amax_a[i]));
+      const double sa = row_scale<F>(g.amax_a, i);
 #pragma unroll
       for (int y = 0; y < TN; ++y) {
-        const int64_t j = n0 + wn * (BN / 2) + y * 16 + (lane & 15);
-        if (j < g.N) store_out(g.out, i, j, (double)acc[x][y][v] * sa * sb[y]);
+        const int64_t j = n0 + wn * (BN / 2) + y * 16 + l16;
+        if (j >= g.N) continue;
+        const double val = (double)acc[x][y][v] * sa * sb[y];
+        if (g.part) g.part[(int64_t)blockIdx.z * g.M * g.N + i + j * g.M] = (float)val;
+        else store_nt(g, i, j, val);
       }
     }
 }
 
-struct Tile {
-  int bm, bn;
+// Sum of the FP32 partials of the K splits (in FP64, fixed order), then the
+// epilogue.
+__global__ void k_nt_reduce(NtArgs g, int splits) {
+  const int64_t total = g.M * g.N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int z = 0; z < splits; ++z) v += (double)g.part[(int64_t)z * total + e];
+    store_nt(g, e % g.M, e / g.M, v);
+  }
+}
+
+// Tile and K-split of one NT product.  Candidate tiles are 64/128 x 64/128;
+// a launch is modelled as rounds of `res` co-resident blocks per CU over 256
+// CUs, each round costing BM * BN * K/splits, plus FP32 partial traffic.
+struct NtPlan {
+  int bm, bn, splits;
 };
-Tile pick_tile(int64_t M, int64_t N) {
-  const Tile cfgs[] = {{128, 128}, {128, 64}, {64, 128}, {64, 64}};
-  Tile best = cfgs[3];
-  int64_t best_cost = INT64_MAX;
-  for (const Tile& c : cfgs) {
+NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
+  const NtPlan cands[] = {{128, 128, 1}, {128, 64, 1}, {64, 128, 1}, {64, 64, 1}};
+  NtPlan best = cands[3];
+  double best_cost = 1e300;
+  const int kstep = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
+  for (NtPlan c : cands) {
     const int64_t tiles = cdiv(M, c.bm) * cdiv(N, c.bn);
-    const int64_t cost = cdiv(tiles, 256) * c.bm * c.bn;
-    if (cost < best_cost) { best_cost = cost; best = c; }
+    const int res = (c.bm * c.bn >= 128 * 128) ? 2 : 4;  // blocks per CU the LDS admits
+    for (int sp = 1; sp <= 8; sp *= 2) {
+      if (K / sp < 4 * kstep) break;
+      const double rounds = (double)cdiv(tiles * sp, 256 * res);
+      const double cost = rounds * c.bm * c.bn * (double)cdiv(K, sp) * res +
+                          (sp > 1 ? 8.0 * sp * (double)M * N : 0.0);
+      if (cost < best_cost) { best_cost = cost; best = c; best.splits = sp; }
+    }
   }
   return best;
 }
 
-void gemm_nt(hipStream_t st, int prec, NtArgs a) {
-  const Tile tl = pick_tile(a.M, a.N);
-  a.tiles_m = (int)cdiv(a.M, tl.bm);
-  const unsigned tiles = (unsigned)(a.tiles_m * cdiv(a.N, tl.bn));
-  const dim3 grid(tiles), block(kThreads);
-#define SKEMA_NT_LAUNCH(BM_, BN_)                                                                   \
-  do {                                                                                              \
-    if (prec == SKEMA_PREC_FP8) hipLaunchKernelGGL((k_gemm_nt_fp8<BM_, BN_>), grid, block, 0, st, a); \
-    else if (prec == SKEMA_PREC_FP32) hipLaunchKernelGGL((k_gemm_nt_f32<BM_, BN_>), grid, block, 0, st, a); \
-    else if (prec == SKEMA_PREC_BF16X3)                                                             \
-      hipLaunchKernelGGL((k_gemm_nt_bf16<BM_, BN_, true>), grid, block, 0, st, a);                  \
-    else hipLaunchKernelGGL((k_gemm_nt_bf16<BM_, BN_, false>), grid, block, 0, st, a);              \
-  } while (0)
-  if (tl.bm == 128 && tl.bn == 128) SKEMA_NT_LAUNCH(128, 128);
-  else if (tl.bm == 128) SKEMA_NT_LAUNCH(128, 64);
-  else if (tl.bn == 128) SKEMA_NT_LAUNCH(64, 128);
-  else SKEMA_NT_LAUNCH(64, 64);
-#undef SKEMA_NT_LAUNCH
-  check_launch("k_gemm_nt");
+template <class F>
+void launch_nt(hipStream_t st, const NtArgs& a, int bm, int bn, dim3 grid) {
+  if (bm == 128 && bn == 128) hipLaunchKernelGGL((k_nt<F, 128, 128>), grid, dim3(kThreads), 0, st, a);
+  else if (bm == 128) hipLaunchKernelGGL((k_nt<F, 128, 64>), grid, dim3(kThreads), 0, st, a);
+  else if (bn == 128) hipLaunchKernelGGL((k_nt<F, 64, 128>), grid, dim3(kThreads), 0, st, a);
+  else hipLaunchKernelGGL((k_nt<F, 64, 64>), grid, dim3(kThreads), 0, st, a);
+}
+
+void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
+  const NtPlan pl = plan_nt(prec, a.M, a.N, a.K);
+  const int kstep = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
+  int splits = ws ? pl.splits : 1;
+  a.k_per_split = cdiv(cdiv(a.K, splits), kstep) * kstep;
+  splits = (int)cdiv(a.K, a.k_per_split);
+  a.part = nullptr;
+  if (splits > 1) {
+    ws->reserve((size_t)splits * a.M * a.N * sizeof(float));
+    a.part = ws->as<float>();
+  }
+  a.tiles_m = (int)cdiv(a.M, pl.bm);
+  const dim3 grid((unsigned)(a.tiles_m * cdiv(a.N, pl.bn)), 1, (unsigned)splits);
+  if (prec == SKEMA_PREC_FP8) launch_nt<FormFp8>(st, a, pl.bm, pl.bn, grid);
+  else if (prec == SKEMA_PREC_FP32) launch_nt<FormF32>(st, a, pl.bm, pl.bn, grid);
+  else if (prec == SKEMA_PREC_BF16X3) launch_nt<FormBf16x3>(st, a, pl.bm, pl.bn, grid);
+  else launch_nt<FormBf16>(st, a, pl.bm, pl.bn, grid);
+  check_launch("k_nt");
+  if (splits > 1) {
+    const int blocks = (int)std::min<int64_t>(cdiv(a.M * a.N, 256), 8192);
+    hipLaunchKernelGGL(k_nt_reduce, dim3(blocks), dim3(256), 0, st, a, splits);
+    check_launch("k_nt_reduce");
+  }
 }
 
 unsigned grid1(int64_t e) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(e, 256), 65536)); }
@@ -625,6 +565,21 @@ void prepare_lowp_maps(Sketch& sk) {
                        k + s, k + s, n, oh, split ? oh + oe : nullptr, sk.lp_ldomps);
     check_launch("k_convert_map<bf16>");
   }
+  // Phi (s x m) row-major for the core product: fp32 for the fp32 form,
+  // bf16 hi + lo for split-BF16, bf16 otherwise.
+  if (prec == SKEMA_PREC_FP32) {
+    sk.lp_phi.alloc((size_t)s * sk.lp_ldups * 4);
+    hipLaunchKernelGGL(k_convert_map<float>, dim3(grid1(s * m)), dim3(256), 0, st, sk.phiL, sk.ldphi, s, m,
+                       sk.lp_phi.as<float>(), (float*)nullptr, sk.lp_ldups);
+  } else {
+    const bool split = prec == SKEMA_PREC_BF16X3;
+    const size_t pe = (size_t)s * sk.lp_ldups;
+    sk.lp_phi.alloc(pe * 2 * (split ? 2 : 1));
+    __bf16* ph = sk.lp_phi.as<__bf16>();
+    hipLaunchKernelGGL(k_convert_map<__bf16>, dim3(grid1(s * m)), dim3(256), 0, st, sk.phiL, sk.ldphi, s, m, ph,
+                       split ? ph + pe : nullptr, sk.lp_ldups);
+  }
+  check_launch("k_convert_map (Phi)");
   SKEMA_HIP(hipStreamSynchronize(st));
 }
 
@@ -635,15 +590,19 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n;
   const double eta = sk.p.eta, nu = sk.p.nu;
   const bool fp8 = prec == SKEMA_PREC_FP8, split = prec == SKEMA_PREC_BF16X3, f32 = prec == SKEMA_PREC_FP32;
-  double* G = sk.G.as<double>();
   // window copies: L (column-major, ld ldl = rows padded), R (row-major, ld ldr = n padded);
-  // fp8 adds the row and column max |x| of the window (floats)
+  // fp8 adds the row and column max |x| of the window (floats); G (rows x s,
+  // column-major, ld ldl) in the core product's element type
   const int64_t ldl = cdiv(rows, 128) * 128, ldr = cdiv(n, 128) * 128;
   const size_t esz = fp8 ? 1 : (f32 ? 4 : 2), planes = split ? 2 : 1;
-  const size_t le = (size_t)n * ldl, re = (size_t)rows * ldr;
+  const int core_prec = f32 ? SKEMA_PREC_FP32 : (split ? SKEMA_PREC_BF16X3 : SKEMA_PREC_BF16);
+  const int gtype = f32 ? 3 : (split ? 2 : 1);
+  const size_t gsz = f32 ? 4 : 2;
+  const size_t le = (size_t)n * ldl, re = (size_t)rows * ldr, ge = (size_t)s * ldl;
   const size_t o_l = 0, o_r = align_up(le * esz * planes, 256), o_rmax = o_r + align_up(re * esz * planes, 256),
                o_cmax = o_rmax + (fp8 ? align_up((size_t)rows * 4, 256) : 0),
-               total = o_cmax + (fp8 ? align_up((size_t)n * 4, 256) : 0);
+               o_g = o_cmax + (fp8 ? align_up((size_t)n * 4, 256) : 0),
+               total = o_g + align_up(ge * gsz * planes, 256);
   sk.win.reserve(total);
   char* base = sk.win.as<char>();
   float* rmax = (float*)(base + o_rmax);
@@ -687,7 +646,7 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
     a.ldb = ldl;
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
-    gemm_nt(st, prec, a);
+    gemm_nt(st, prec, a, &c->gws.buf);
   }
   {
     Timed t(c, "sketch_right");
@@ -702,14 +661,38 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
     a.amax_b = fp8 ? (const float*)(sk.lp_omps.as<uint8_t>() + align_up(oe, 256)) : nullptr;
     a.ldb = sk.lp_ldomps;
     a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
-    a.out.C2 = G; a.out.ldc2 = rows; a.out.alpha2 = 1.0; a.out.beta2 = 0.0;
-    gemm_nt(st, prec, a);
+    a.gtype = gtype; a.gh = base + o_g; a.gl = base + o_g + ge * gsz; a.ldg = ldl;
+    gemm_nt(st, prec, a, &c->gws.buf);
   }
   {
     Timed t(c, "sketch_core");
-    gemm_f64(st, 'N', 'N', s, s, rows, eta, sk.phiL + row0 * sk.ldphi, sk.ldphi, G, rows, nu, sk.Z.as<double>(), s,
-             &c->gws);
+    const size_t pe = (size_t)s * sk.lp_ldups;
+    NtArgs a{};
+    a.M = s; a.N = s; a.K = rows;
+    a.ah = sk.lp_phi.p;
+    a.al = split ? (const void*)(sk.lp_phi.as<__bf16>() + pe) : nullptr;
+    a.lda = sk.lp_ldups; a.koff_a = row0;
+    a.bh = base + o_g;
+    a.bl = split ? (const void*)(base + o_g + ge * gsz) : nullptr;
+    a.ldb = ldl;
+    a.out.C1 = sk.Z.as<double>(); a.out.ldc1 = s; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = s;
+    a.out.C2 = a.out.C1; a.out.ldc2 = s;
+    gemm_nt(st, core_prec, a, &c->gws.buf);
   }
+}
+
+void lowp_core_bf16(Sketch& sk, const __bf16* G, int64_t ldg, int64_t row0, int64_t rows) {
+  SKEMA_REQUIRE(sk.p.precision == SKEMA_PREC_BF16, kErrUnsupported, "bf16 core product needs the bf16 maps");
+  const int64_t s = sk.s;
+  NtArgs a{};
+  a.M = s; a.N = s; a.K = rows;
+  a.ah = sk.lp_phi.p;
+  a.lda = sk.lp_ldups; a.koff_a = row0;
+  a.bh = G;
+  a.ldb = ldg;
+  a.out.C1 = sk.Z.as<double>(); a.out.ldc1 = s; a.out.alpha1 = sk.p.eta; a.out.beta1 = sk.p.nu; a.out.nsplit = s;
+  a.out.C2 = a.out.C1; a.out.ldc2 = s;
+  gemm_nt(sk.ctx->stream, SKEMA_PREC_BF16, a, &sk.ctx->gws.buf);
 }
 
 }  // namespace skema

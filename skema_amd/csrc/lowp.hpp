@@ -7,5 +7,8 @@ namespace skema {
 
 void prepare_lowp_maps(Sketch& sk);
 void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows);
+// Z <- nu Z + eta Phi[:, row0:row0+rows] G for a BF16 G (rows x s,
+// column-major, ld ldg) on the BF16 NT GEMM (bf16 form maps).
+void lowp_core_bf16(Sketch& sk, const __bf16* G, int64_t ldg, int64_t row0, int64_t rows);
 
 }  // namespace skema

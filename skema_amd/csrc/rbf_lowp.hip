@@ -1,0 +1,477 @@
+// BF16 form of the fused Gaussian-RBF window + sketch (BF16 Gram, FP32
+// accumulation, FP32 exponent).  The point set is rounded to BF16 once per
+// window update (row-major, features contiguous, padded to DP = 32, 64 or
+// 128) together with the squared norms of the rounded points, so
+// D_ij = |x~_i|^2 + |x~_j|^2 - 2 x~_i.x~_j is the squared distance of the
+// rounded points up to FP32 accumulation.  The rounding errors of the d
+// products are independent, so the Gram carries a relative error of about
+// sqrt(d) 2^-8 |x|^2 / D, which gamma = O(1/d) turns into ~0.3% of K for the
+// mixture data — the order of rounding K itself to BF16 for the sketch MFMAs.
+//   left : X       <- nu X + eta * Upsilon[:, idx] K_w     (k x n)
+//   right: [y | G]  =  K_w [Omega; Psi]^T                   (w x (k+s))
+// then Z <- nu Z + eta Phi[:, idx] G on the BF16 NT GEMM.  K never leaves
+// registers/LDS.
+#include <cmath>
+
+#include "lowp.hpp"
+#include "rbf_fused.hpp"
+#include "rbf_lowp.hpp"
+#include "util.hpp"
+
+namespace skema {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef int i4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = bid & 7, local = bid >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
+__device__ __forceinline__ void store_out(const GemmOut& o, int64_t i, int64_t j, double v) {
+  if (j < o.nsplit) {
+    double* c = o.C1 + i + j * o.ldc1;
+    *c = (o.beta1 != 0.0) ? o.alpha1 * v + o.beta1 * *c : o.alpha1 * v;
+  } else {
+    double* c = o.C2 + i + (j - o.nsplit) * o.ldc2;
+    *c = (o.beta2 != 0.0) ? o.alpha2 * v + o.beta2 * *c : o.alpha2 * v;
+  }
+}
+
+// exp(-gamma * max(D, 0)) in FP32, the window's own diagonal exactly 1
+// (src/Skema_Kernel.cpp:49-74).
+__device__ __forceinline__ float rbf_value_f(float ni, float nj, float g, float gamma, bool diag) {
+  if (diag) return 1.f;
+  float D = ni + nj - 2.f * g;
+  D = D > 0.f ? D : 0.f;
+  return __expf(-gamma * D);
+}
+
+struct RbfLpArgs {
+  int64_t M = 0, N = 0, K = 0;   // product shape
+  const __bf16* P = nullptr;     // rounded points, n x ldpb row-major
+  int64_t ldpb = 0;
+  const float* nrm = nullptr;    // |x~_j|^2
+  float gamma = 0.f;
+  int64_t row0 = 0;
+  const __bf16* OP = nullptr;    // right: [Omega; Psi] row-major ((k+s) x ldop)
+  int64_t ldop = 0;
+  const __bf16* U = nullptr;     // left: Upsilon row-major (k x ldu)
+  int64_t ldu = 0;
+  GemmOut out;                   // right: columns < nsplit -> out.C1 (Y rows)
+  __bf16* G = nullptr;           // right: columns >= nsplit as bf16, col-major ld ldg
+  int64_t ldg = 0;
+  int tiles_m = 0;
+  int64_t k_per_split = 0;
+  float* part = nullptr;         // right: FP32 partials [split][M x N]
+};
+
+__device__ __forceinline__ void store_right(const RbfLpArgs& g, int64_t i, int64_t j, double v) {
+  if (j >= g.out.nsplit) g.G[i + (j - g.out.nsplit) * g.ldg] = (__bf16)(float)v;
+  else store_out(g.out, i, j, v);
+}
+
+// 16 bytes (8 bf16) of a row-major bf16 matrix at element e, elements past
+// `lim` (exclusive) read as zero.
+__device__ __forceinline__ bf16x8 load8(const __bf16* row, int64_t e, int64_t lim) {
+  const __bf16* src = row + e;
+  if (e + 8 <= lim && (((uintptr_t)src) & 15) == 0) return *reinterpret_cast<const bf16x8*>(src);
+  bf16x8 v;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = (e + q < lim) ? src[q] : (__bf16)0.f;
+  return v;
+}
+
+// ---- points -> BF16 rows + norms of the rounded points --------------------------
+__global__ void k_points_bf16(const double* X, int64_t ldp, int64_t n, int64_t d, int64_t ldpb, __bf16* P,
+                              float* nrm) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float s = 0.f;
+  for (int64_t f0 = 0; f0 < ldpb; f0 += 8) {
+    bf16x8 v;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t f = f0 + q;
+      const __bf16 h = (f < d) ? (__bf16)(float)X[j + f * ldp] : (__bf16)0.f;
+      const float hf = (float)h;
+      s += hf * hf;
+      v[q] = h;
+    }
+    *reinterpret_cast<bf16x8*>(P + j * ldpb + f0) = v;
+  }
+  nrm[j] = s;
+}
+
+// ---- right -----------------------------------------------------------------------
+// 512 threads own 64 window rows x 128*TN columns.  Per step of 32 points:
+// wave w forms the Gram tile (row tile w%4, point tile w/4) over all DP
+// features (its window rows' features in registers, the points' from LDS),
+// exponentiates its four values per lane and writes K (bf16) to LDS; the
+// sketch MFMAs of wave w cover all 64 rows x its 16*TN columns with the
+// [Omega; Psi] operands loaded from global memory straight into registers.
+// Pipelined one step apart (sketch of t, Gram + exp of t+1, points of t+2
+// staged, t+3 loaded), double-buffered LDS, one barrier per step.
+constexpr int kThreadsR = 512;
+
+template <int DP, int TN>
+__global__ __launch_bounds__(kThreadsR) void k_rbf_right_bf16(RbfLpArgs g) {
+  constexpr int BM = 64, BK = 32, PS = DP + 8, CHUNKS = BK * DP / 8;
+  __shared__ __attribute__((aligned(16))) __bf16 Xs[2][BK][PS];
+  __shared__ __attribute__((aligned(16))) __bf16 Ks[2][BM][BK + 8];
+  __shared__ float NJ[4][BK];
+  __shared__ float NI[BM];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
+  const int split = t / g.tiles_m;
+  const int64_t cw0 = (int64_t)blockIdx.y * (128 * TN) + (int64_t)wave * (16 * TN);
+  const int64_t kbeg = (int64_t)split * g.k_per_split;
+  const int64_t kend = (g.K < kbeg + g.k_per_split) ? g.K : kbeg + g.k_per_split;
+  const int64_t steps = (kend - kbeg + BK - 1) / BK;
+  const int rt = wave & 3, pt = wave >> 2;
+
+  bf16x8 xa[DP / 32];  // Gram A operand: window row m0 + 16 rt + l16, features 32c + 8 lq ..
+  {
+    const int64_t i = m0 + 16 * rt + l16;
+#pragma unroll
+    for (int c = 0; c < DP / 32; ++c)
+      xa[c] = (i < g.M) ? load8(g.P + (g.row0 + i) * g.ldpb, 32 * c + 8 * lq, g.ldpb) : bf16x8{};
+  }
+  if (tid < BM) NI[tid] = (m0 + tid < g.M) ? g.nrm[g.row0 + m0 + tid] : 0.f;
+
+  bf16x8 xr = {};
+  float njr = 0.f;
+  auto load_points = [&](int64_t u) {
+    const int64_t j0 = kbeg + u * BK;
+    if (tid < CHUNKS) {
+      const int jj = tid / (DP / 8), fc = tid % (DP / 8);
+      const int64_t j = j0 + jj;
+      xr = (j < kend) ? load8(g.P + j * g.ldpb, 8 * fc, g.ldpb) : bf16x8{};
+    }
+    njr = (tid < BK && j0 + tid < kend) ? g.nrm[j0 + tid] : 0.f;
+  };
+  auto store_points = [&](int b, int64_t u) {
+    if (tid < CHUNKS) *reinterpret_cast<bf16x8*>(&Xs[b][tid / (DP / 8)][8 * (tid % (DP / 8))]) = xr;
+    if (tid < BK) NJ[u & 3][tid] = njr;
+  };
+  auto gram_exp = [&](int b, int64_t u) {  // Xs[b] (step u) -> Ks[b]
+    f4v gr = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < DP / 32; ++c) {
+      const bf16x8 bx = *reinterpret_cast<const bf16x8*>(&Xs[b][16 * pt + l16][32 * c + 8 * lq]);
+      gr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa[c], bx, gr, 0, 0, 0);
+    }
+    const int jl = 16 * pt + l16;
+    const int64_t j = kbeg + u * BK + jl;
+    const float nj = NJ[u & 3][jl];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int il = 16 * rt + 4 * lq + v;
+      const int64_t i = m0 + il;
+      float kv = 0.f;
+      if (i < g.M && j < kend) kv = rbf_value_f(NI[il], nj, gr[v], g.gamma, g.row0 + i == j);
+      Ks[b][il][jl] = (__bf16)kv;
+    }
+  };
+  bf16x8 op[TN], opn[TN];
+  auto load_op = [&](bf16x8 (&dst)[TN], int64_t u) {
+    const int64_t j = kbeg + u * BK + 8 * lq;
+#pragma unroll
+    for (int y = 0; y < TN; ++y) {
+      const int64_t c = cw0 + 16 * y + l16;
+      dst[y] = (c < g.N && j < kend) ? load8(g.OP + c * g.ldop, j, kend) : bf16x8{};
+    }
+  };
+
+  f4v acc[4][TN];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < TN; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  load_points(0); store_points(0, 0);
+  load_points(1); store_points(1, 1);
+  load_op(op, 0);
+  __syncthreads();
+  gram_exp(0, 0);
+  load_points(2);
+  __syncthreads();
+  for (int64_t u = 0; u < steps; ++u) {
+    const int b = (int)(u & 1), nb = b ^ 1;
+    if (u + 1 < steps) load_op(opn, u + 1);
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&Ks[b][16 * x + l16][8 * lq]);
+#pragma unroll
+      for (int y = 0; y < TN; ++y) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, op[y], acc[x][y], 0, 0, 0);
+    }
+    if (u + 1 < steps) gram_exp(nb, u + 1);  // points of step u+1 sit in Xs[nb]
+    store_points(b, u + 2);                  // points of step u+2 -> Xs[b]
+    load_points(u + 3);
+    __syncthreads();
+#pragma unroll
+    for (int y = 0; y < TN; ++y) op[y] = opn[y];
+  }
+  // 16x16 f32 result: lane l holds rows 4*(l/16)+v of column l%16.
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < TN; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t i = m0 + 16 * x + 4 * lq + v;
+        const int64_t j = cw0 + 16 * y + l16;
+        if (i < g.M && j < g.N) {
+          if (g.part) g.part[(int64_t)split * g.M * g.N + i + j * g.M] = acc[x][y][v];
+          else store_right(g, i, j, (double)acc[x][y][v]);
+        }
+      }
+}
+
+__global__ void k_rbf_right_bf16_reduce(RbfLpArgs g, int splits) {
+  const int64_t total = g.M * g.N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int z = 0; z < splits; ++z) v += (double)g.part[(int64_t)z * total + e];
+    store_right(g, e % g.M, e / g.M, v);
+  }
+}
+
+// ---- left ------------------------------------------------------------------------
+// 256 threads own 64 points (wave w: points 16w..16w+15, features in
+// registers) and 16*TP map rows.  Per step of 32 window rows each wave forms
+// its two 16x16 Gram tiles (rows 0-15 and 16-31 of the step), exponentiates
+// them in registers and packs the 8 values of each lane into the B operand of
+// one sketch MFMA: K-slot q of lane l stands for window row
+// 16*(q/4) + 4*(l/16) + q%4, which is exactly where the f32 result layout put
+// them, and the Upsilon A operand is read from LDS in the same row order.
+template <int DP, int TP>
+__global__ __launch_bounds__(256) void k_rbf_left_bf16(RbfLpArgs g) {
+  constexpr int BK = 32, PS = DP + 8, PR = 16 * TP, US = BK + 8;
+  constexpr int XCH = BK * DP / 8, UCH = PR * BK / 8;  // 16-byte chunks per step
+  constexpr int XR = (XCH + 255) / 256, UR = (UCH + 255) / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 Xw[2][BK][PS];
+  __shared__ __attribute__((aligned(16))) __bf16 Us[2][PR][US];
+  __shared__ float NI[2][BK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t n0 = (int64_t)t * 64;
+  const int64_t p0 = (int64_t)blockIdx.y * PR;
+  const int64_t steps = (g.K + BK - 1) / BK;
+  const int64_t jp = n0 + 16 * wave + l16;
+
+  bf16x8 xb[DP / 32];
+#pragma unroll
+  for (int c = 0; c < DP / 32; ++c) xb[c] = (jp < g.N) ? load8(g.P + jp * g.ldpb, 32 * c + 8 * lq, g.ldpb) : bf16x8{};
+  const float nj = (jp < g.N) ? g.nrm[jp] : 0.f;
+
+  bf16x8 xr[XR], ur[UR];
+  float nir = 0.f;
+  auto load_step = [&](int64_t u) {
+    const int64_t i0 = u * BK;
+#pragma unroll
+    for (int q = 0; q < XR; ++q) {
+      const int c = tid + 256 * q;
+      const int ii = c / (DP / 8), fc = c % (DP / 8);
+      const int64_t i = i0 + ii;
+      xr[q] = (c < XCH && i < g.K) ? load8(g.P + (g.row0 + i) * g.ldpb, 8 * fc, g.ldpb) : bf16x8{};
+    }
+#pragma unroll
+    for (int q = 0; q < UR; ++q) {
+      const int c = tid + 256 * q;
+      const int pp = c / (BK / 8), ic = c % (BK / 8);
+      const int64_t p = p0 + pp;
+      ur[q] = (c < UCH && p < g.M) ? load8(g.U + p * g.ldu + g.row0, i0 + 8 * ic, g.K) : bf16x8{};
+    }
+    nir = (tid < BK && i0 + tid < g.K) ? g.nrm[g.row0 + i0 + tid] : 0.f;
+  };
+  auto store_step = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < XR; ++q) {
+      const int c = tid + 256 * q;
+      if (c < XCH) *reinterpret_cast<bf16x8*>(&Xw[b][c / (DP / 8)][8 * (c % (DP / 8))]) = xr[q];
+    }
+#pragma unroll
+    for (int q = 0; q < UR; ++q) {
+      const int c = tid + 256 * q;
+      if (c < UCH) *reinterpret_cast<bf16x8*>(&Us[b][c / (BK / 8)][8 * (c % (BK / 8))]) = ur[q];
+    }
+    if (tid < BK) NI[b][tid] = nir;
+  };
+
+  f4v acc[TP];
+#pragma unroll
+  for (int y = 0; y < TP; ++y) acc[y] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  load_step(0);
+  store_step(0);
+  load_step(1);
+  __syncthreads();
+  for (int64_t u = 0; u < steps; ++u) {
+    const int b = (int)(u & 1);
+    const int64_t i0 = u * BK;
+    f4v gr[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      gr[x] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < DP / 32; ++c) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&Xw[b][16 * x + l16][32 * c + 8 * lq]);
+        gr[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[c], gr[x], 0, 0, 0);
+      }
+    }
+    bf16x8 kb;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int il = 16 * (q >> 2) + 4 * lq + (q & 3);
+      const int64_t i = i0 + il;
+      float kv = 0.f;
+      if (i < g.K && jp < g.N) kv = rbf_value_f(NI[b][il], nj, gr[q >> 2][q & 3], g.gamma, g.row0 + i == jp);
+      kb[q] = (__bf16)kv;
+    }
+#pragma unroll
+    for (int y = 0; y < TP; ++y) {
+      const __bf16* row = &Us[b][16 * y + l16][0];
+      const bf16x4 lo = *reinterpret_cast<const bf16x4*>(row + 4 * lq);
+      const bf16x4 hi = *reinterpret_cast<const bf16x4*>(row + 16 + 4 * lq);
+      const bf16x8 a = bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      acc[y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, kb, acc[y], 0, 0, 0);
+    }
+    store_step(b ^ 1);
+    load_step(u + 2);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int y = 0; y < TP; ++y)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t p = p0 + 16 * y + 4 * lq + v;
+      if (p < g.M && jp < g.N) store_out(g.out, p, jp, (double)acc[y][v]);
+    }
+}
+
+constexpr int kTnChoices[] = {1, 2, 3, 4, 5};
+constexpr int kTpChoices[] = {1, 2, 4, 8, 13};
+
+template <size_t L>
+int pick(const int (&choices)[L], int64_t need) {
+  for (int c : choices)
+    if (c >= need) return c;
+  return choices[L - 1];
+}
+
+template <int DP>
+void launch_right(hipStream_t st, const RbfLpArgs& a, dim3 grid, int tn) {
+  switch (tn) {
+    case 1: hipLaunchKernelGGL((k_rbf_right_bf16<DP, 1>), grid, dim3(kThreadsR), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_rbf_right_bf16<DP, 2>), grid, dim3(kThreadsR), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_rbf_right_bf16<DP, 3>), grid, dim3(kThreadsR), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_rbf_right_bf16<DP, 4>), grid, dim3(kThreadsR), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_rbf_right_bf16<DP, 5>), grid, dim3(kThreadsR), 0, st, a); break;
+  }
+}
+template <int DP>
+void launch_left(hipStream_t st, const RbfLpArgs& a, dim3 grid, int tp) {
+  switch (tp) {
+    case 1: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 1>), grid, dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 2>), grid, dim3(256), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 4>), grid, dim3(256), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 8>), grid, dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 13>), grid, dim3(256), 0, st, a); break;
+  }
+}
+
+}  // namespace
+
+bool rbf_bf16_supported(const Sketch& sk) {
+  return !sk.sparse_maps && sk.p.precision == SKEMA_PREC_BF16 && sk.p.nfeat >= 1 && sk.p.nfeat <= kRbfMaxFeat;
+}
+
+void update_rbf_window_bf16(Sketch& sk, const double* points, int64_t ldp, int64_t row0, int64_t rows) {
+  Ctx* c = sk.ctx;
+  hipStream_t st = c->stream;
+  const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n, d = sk.p.nfeat;
+  const double eta = sk.p.eta, nu = sk.p.nu;
+  const int dp = d <= 32 ? 32 : (d <= 64 ? 64 : 128);
+  const int64_t ldg = cdiv(rows, 128) * 128;
+  const size_t o_p = 0, o_n = align_up((size_t)n * dp * 2, 256), o_g = o_n + align_up((size_t)n * 4, 256),
+               total = o_g + align_up((size_t)s * ldg * 2, 256);
+  sk.win.reserve(total);
+  char* base = sk.win.as<char>();
+  __bf16* P = (__bf16*)(base + o_p);
+  float* nrm = (float*)(base + o_n);
+  __bf16* G = (__bf16*)(base + o_g);
+  {
+    Timed t(c, "window");
+    hipLaunchKernelGGL(k_points_bf16, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, points, ldp, n, d,
+                       (int64_t)dp, P, nrm);
+    check_launch("k_points_bf16");
+  }
+  RbfLpArgs base_args;
+  base_args.P = P; base_args.ldpb = dp; base_args.nrm = nrm; base_args.gamma = (float)sk.p.gamma;
+  base_args.row0 = row0;
+  {
+    Timed t(c, "rbf_left");
+    RbfLpArgs a = base_args;
+    a.M = k; a.N = n; a.K = rows;
+    a.U = sk.lp_ups.as<__bf16>(); a.ldu = sk.lp_ldups;
+    a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
+    a.out.C2 = a.out.C1; a.out.ldc2 = k;
+    const int tp = pick(kTpChoices, cdiv(k, 16));
+    const dim3 grid((unsigned)cdiv(n, 64), (unsigned)cdiv(k, 16 * tp));
+    if (dp == 32) launch_left<32>(st, a, grid, tp);
+    else if (dp == 64) launch_left<64>(st, a, grid, tp);
+    else launch_left<128>(st, a, grid, tp);
+    check_launch("k_rbf_left_bf16");
+  }
+  {
+    Timed t(c, "rbf_right");
+    RbfLpArgs a = base_args;
+    a.M = rows; a.N = k + s; a.K = n;
+    a.OP = sk.lp_omps.as<__bf16>(); a.ldop = sk.lp_ldomps;
+    a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
+    a.G = G; a.ldg = ldg;
+    const int tn = pick(kTnChoices, cdiv(cdiv(a.N, 16), 8));
+    const int64_t col_blocks = cdiv(a.N, 128 * tn);
+    a.tiles_m = (int)cdiv(a.M, 64);
+    // one 512-thread block per CU: choose the split of the points that
+    // minimises rounds x points per split plus the partial traffic
+    const int64_t per_split = a.tiles_m * col_blocks;
+    int64_t splits = 1, best = INT64_MAX;
+    const int64_t max_splits = std::max<int64_t>(1, std::min<int64_t>(64, a.K / 1024));
+    for (int64_t sp = 1; sp <= max_splits; ++sp) {
+      const int64_t cost = cdiv(per_split * sp, 256) * cdiv(a.K, sp) + (sp > 1 ? sp * (a.M * a.N / 65536 + 1) : 0);
+      if (cost < best) { best = cost; splits = sp; }
+    }
+    a.k_per_split = cdiv(cdiv(a.K, splits), 32) * 32;
+    splits = cdiv(a.K, a.k_per_split);
+    a.part = nullptr;
+    if (splits > 1) {
+      c->gws.buf.reserve((size_t)splits * a.M * a.N * sizeof(float));
+      a.part = c->gws.buf.as<float>();
+    }
+    const dim3 grid((unsigned)(a.tiles_m * splits), (unsigned)col_blocks);
+    if (dp == 32) launch_right<32>(st, a, grid, tn);
+    else if (dp == 64) launch_right<64>(st, a, grid, tn);
+    else launch_right<128>(st, a, grid, tn);
+    check_launch("k_rbf_right_bf16");
+    if (splits > 1) {
+      const int blocks = (int)std::min<int64_t>(cdiv(a.M * a.N, 256), 8192);
+      hipLaunchKernelGGL(k_rbf_right_bf16_reduce, dim3(blocks), dim3(256), 0, st, a, (int)splits);
+      check_launch("k_rbf_right_bf16_reduce");
+    }
+  }
+  {
+    Timed t(c, "sketch_core");
+    lowp_core_bf16(sk, G, ldg, row0, rows);
+  }
+}
+
+}  // namespace skema

@@ -102,7 +102,7 @@ struct Sketch {
   // Sparse maps: [Omega; Psi] as CSR (rows p) and as CSR of its transpose.
   Csr sp_omps, sp_ompsT;
   // Low-precision map copies (precision forms).
-  DevBuf lp_ups, lp_omps, lp_phi, lp_scale;
+  DevBuf lp_ups, lp_omps, lp_phi;
   int64_t lp_ldups = 0, lp_ldomps = 0;
   // State.
   DevBuf X, Y, Z;  // k x n, m x k, s x s

@@ -16,6 +16,7 @@
 #include "lowp.hpp"
 #include "rbf.hpp"
 #include "rbf_fused.hpp"
+#include "rbf_lowp.hpp"
 #include "sketch.hpp"
 #include "sparse.hpp"
 #include "util.hpp"
@@ -241,6 +242,10 @@ void update_rbf_window(Sketch& sk, const double* points, int64_t ldp, int64_t ro
   const int64_t n = sk.p.n;
   if (!sk.sparse_maps && rbf_fused_supported(sk)) {
     update_rbf_window_fused(sk, points, ldp, row0, rows);
+    return;
+  }
+  if (rbf_bf16_supported(sk)) {
+    update_rbf_window_bf16(sk, points, ldp, row0, rows);
     return;
   }
   sk.win.reserve((size_t)rows * n * sizeof(double));

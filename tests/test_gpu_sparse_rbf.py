@@ -232,6 +232,37 @@ def test_rbf_fused_split_points_and_wide_sketches(S, n, d, w, rank):
     assert O.rel_err(Xg, osk.X) <= TOL and O.rel_err(Yg, osk.Y) <= TOL and O.rel_err(Zg, osk.Z) <= TOL
 
 
+# BF16 form of the fused generator: BF16 Gram (one product, FP32
+# accumulation), FP32 exponent, BF16 sketch MFMAs with FP32 accumulation.
+# Rounding K and the maps to BF16 (8 significant bits) and the Gram's
+# rounding errors (independent over the d products) leave the sketches within
+# a few 1e-3 normwise of the FP64 oracle; 2e-2 is the bound used for the dense
+# BF16 form as well.
+BF16_TOL = 2e-2
+
+
+@pytest.mark.parametrize("n,d,w,rank", [(333, 37, 50, 4), (257, 128, 64, 6), (130, 3, 17, 3), (2600, 64, 300, 10),
+                                        (1200, 20, 256, 60)])
+def test_rbf_bf16_form_against_oracle(S, n, d, w, rank):
+    print(f"case rbf bf16 n={n} d={d} w={w} r={rank}", flush=True)
+    rng = np.random.default_rng(n + d + 1)
+    X = rng.standard_normal((n, d)) + 2.0 * rng.standard_normal((10, d))[rng.integers(0, 10, n)]
+    p = O.AlgParams(matrix_m=n, matrix_n=n, rank=rank, window=w, kernel_func="gaussrbf", kernel_gamma=1.0 / d,
+                    sketch_nu=0.9, sketch_eta=1.1)
+    osk = O.SketchySVD(p)
+    osk.linear_update(X)
+    Ur, Sr, Vr = osk.low_rank_approx()
+    g = S.SketchySVD(to_gpu_params(p, nfeat=d, precision="bf16"), maps=gpu_maps(_omaps(osk)))
+    g.linear_update(dev(X))
+    Xg, Yg, Zg = g.state()
+    assert O.rel_err(Xg, osk.X) <= BF16_TOL, O.rel_err(Xg, osk.X)
+    assert O.rel_err(Yg, osk.Y) <= BF16_TOL, O.rel_err(Yg, osk.Y)
+    assert O.rel_err(Zg, osk.Z) <= 2 * BF16_TOL, O.rel_err(Zg, osk.Z)
+    U, Sv, V = g.low_rank_approx()
+    # the leading singular values of the approximation agree to the same order
+    assert np.max(np.abs(Sv[:rank] - Sr[:rank])) <= BF16_TOL * Sr[0], (Sv[:rank], Sr[:rank])
+
+
 def test_cli_driver_on_reference_fixtures(golden_dir, tmp_path):
     """The C++ driver (include/skema_mi355x.hpp layer) on the reference's own
     fixture files: MatrixMarket coordinate (dense and --sparse) and the
