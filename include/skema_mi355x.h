@@ -56,11 +56,16 @@ extern "C" {
 #define SKEMA_KERNEL_NONE 0
 #define SKEMA_KERNEL_GAUSSRBF 1
 
-/* Arithmetic of the sketch contractions (the finalize is always FP64). */
+/* Arithmetic of the sketch contractions (the finalize is always FP64).
+ * Dense input takes every form; the fused RBF generator takes FP64 and BF16
+ * (BF16 Gram and sketch, FP32 exponent and accumulation); CSR input and
+ * sparse-sign maps are FP64.  The state X, Y, Z is FP64 in every form. */
 #define SKEMA_PREC_FP64 0   /* plain: FP64 MFMA, the reference's arithmetic   */
 #define SKEMA_PREC_FP32 1   /* FP32 inputs and accumulation                   */
 #define SKEMA_PREC_BF16X3 2 /* split-BF16 (hi*hi + hi*lo + lo*hi), FP32 acc.  */
-#define SKEMA_PREC_FP8 3    /* FP8 e4m3 with per-row/col scales, FP32 acc.    */
+#define SKEMA_PREC_FP8 3    /* FP8 e4m3, one power-of-two scale per operand
+                               row (map rows, window rows and columns), FP32
+                               accumulation; core product in BF16           */
 #define SKEMA_PREC_BF16 4   /* single-term BF16, FP32 acc.                    */
 
 /* Seeded Gauss map distributions. */
@@ -116,8 +121,8 @@ int skema_ctx_synchronize(skema_ctx* ctx);
  * group; see skema_ctx_kernel_stats. */
 int skema_ctx_set_profiling(skema_ctx* ctx, int enable);
 /* Average/total milliseconds and launch count recorded under `name` since
- * profiling was enabled (names: "sketch_left", "sketch_right", "sketch_core",
- * "sketch_sparse", "rbf_fused", ...). */
+ * profiling was enabled (names: "window", "sketch_left", "sketch_right",
+ * "sketch_core", "rbf_left", "rbf_right", "rbf_adjoint"). */
 int skema_ctx_kernel_stats(skema_ctx* ctx, const char* name, int64_t* count, double* total_ms);
 int skema_ctx_reset_stats(skema_ctx* ctx);
 

@@ -23,7 +23,7 @@
 namespace skema {
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 
 struct AdjArgs {
   const double* X;
@@ -34,172 +34,230 @@ struct AdjArgs {
   int64_t h;
   int64_t jblocks_per_split;
   double* part_hx;  // [split][n x d]
-  double* part_rs;  // [split][2][n]
+  double* part_rs;  // [split][n]
   double* part_dg;  // [block]
 };
 
-// Tile 64 x 64 (rows I of H, points J), 2 x 2 waves of 32 x 32.
+// Tile pass over (I, J) pairs of 64 rows x 64 points (512 threads).  The block owns I and
+// walks the J tiles of its split; everything is formed transposed (rows j,
+// columns i) because the f64 16x16x4 result layout (lane l: column l%16,
+// rows l/16 + 4v) then serves directly as the B operand of the H X product
+// with K = j.  Per J tile:
+//   S^T = R_J F_I^T over the 2h columns of F, streamed through
+//         double-buffered LDS chunks of 16 (one barrier per chunk); wave w
+//         owns j tile w%4 and i tiles 2(w/4), 2(w/4)+1,
+//   the Gram^T tile over the d features (X_I in registers, X_J in LDS),
+//   H^T = 2 S^T o K^T with its row sums and sum H o D in registers,
+//   H^T through LDS, and (H X)^T += X_J^T H^T with wave w owning feature
+//   tiles w%4 (+4, ...) and the same two i tiles, accumulated over all J.
+// Neither K nor H is ever written to HBM.
 template <int DMAX>
 __global__ __launch_bounds__(kThreads) void k_rbf_adjoint_f64(AdjArgs g) {
-  constexpr int BM = 64, BN = 64, BKC = 16, TF = DMAX / 4, ND = DMAX / 32;
-  __shared__ double Fa[BKC][BM + 2];
-  __shared__ double Fb[BKC][BN + 2];
-  __shared__ double Xj[DMAX][BN + 1];
-  __shared__ double Hs[BN][BM + 2];
-  __shared__ double red[4];
+  constexpr int BM = 64, BJ = 64, BKC = 16, TF = DMAX / 4, FT = DMAX / 16;
+  constexpr int FTW = (FT + 3) / 4;  // feature tiles per wave in the H X product
+  __shared__ double Rc[2][BKC][BJ + 2];
+  __shared__ double Fc[2][BKC][BM + 2];
+  __shared__ double XJ[DMAX][BJ + 1];
+  __shared__ double Hs[BJ][BM + 2];
+  __shared__ double red[BM];
+  __shared__ double redg[8];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * BM;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int jt = wave & 3, ih = wave >> 2;
+  const int64_t i0 = (int64_t)blockIdx.x * BM;
   const int split = blockIdx.y;
-  const int64_t nblk = cdiv(g.n, BN);
+  const int64_t nblk = cdiv(g.n, BJ);
   const int64_t jb0 = (int64_t)split * g.jblocks_per_split;
   const int64_t jb1 = (jb0 + g.jblocks_per_split < nblk) ? jb0 + g.jblocks_per_split : nblk;
   const int64_t two_h = 2 * g.h;
+  const int nch = (int)((two_h + BKC - 1) / BKC);
 
-  // Gram A fragments of the block's rows, and their norms.
-  double xa[2][TF];
-  double ni[2][4];
+  // Gram B operand: X_I rows i = i0 + 16 it + l16 (it = 2 ih, 2 ih + 1),
+  // features 4t + lq; held in registers up to 64 features, re-read from
+  // global memory (cache-resident, once per J tile) above that.
+  constexpr int TFR = DMAX <= 64 ? TF : 1;
+  double xi[2][TFR];
+  double ni[2];
+  int64_t irow[2];
 #pragma unroll
-  for (int x = 0; x < 2; ++x) {
-    const int64_t i = m0 + wm * 32 + x * 16 + (lane & 15);
+  for (int u = 0; u < 2; ++u) {
+    const int64_t i = i0 + 16 * (2 * ih + u) + l16;
+    irow[u] = i;
+    if constexpr (DMAX <= 64) {
 #pragma unroll
-    for (int t = 0; t < TF; ++t) {
-      const int64_t f = 4 * t + (lane >> 4);
-      xa[x][t] = (i < g.n && f < g.d) ? g.X[i + f * g.ldp] : 0.0;
+      for (int t = 0; t < TF; ++t) {
+        const int64_t f = 4 * t + lq;
+        xi[u][t] = (i < g.n && f < g.d) ? g.X[i + f * g.ldp] : 0.0;
+      }
     }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t r = m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * v;
-      ni[x][v] = (r < g.n) ? g.nrm[r] : 0.0;
-    }
+    ni[u] = (i < g.n) ? g.nrm[i] : 0.0;
   }
-  d4 hx[2][ND];
+  auto xi_at = [&](int u, int t) -> double {
+    if constexpr (DMAX <= 64) {
+      return xi[u][t];
+    } else {
+      const int64_t f = 4 * t + lq;
+      return (irow[u] < g.n && f < g.d) ? g.X[irow[u] + f * g.ldp] : 0.0;
+    }
+  };
+  d4 hx[FTW][2];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int q = 0; q < FTW; ++q)
 #pragma unroll
-    for (int y = 0; y < ND; ++y) hx[x][y] = d4{0.0, 0.0, 0.0, 0.0};
-  double rs[2][4] = {};
+    for (int u = 0; u < 2; ++u) hx[q][u] = d4{0.0, 0.0, 0.0, 0.0};
+  double rs[2] = {0.0, 0.0};
   double dg = 0.0;
 
-  for (int64_t jb = jb0; jb < jb1; ++jb) {
-    const int64_t n0 = jb * BN;
-    __syncthreads();
-    for (int e = tid; e < DMAX * BN; e += kThreads) {
-      const int jj = e % BN, f = e / BN;
-      const int64_t j = n0 + jj;
-      Xj[f][jj] = (j < g.n && f < g.d) ? g.X[j + (int64_t)f * g.ldp] : 0.0;
+  // chunk c of R_J (columns of F with halves swapped) and F_I: 4 doubles per thread
+  double cr[2], cf[2];
+  auto load_chunk = [&](int64_t j0, int t) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + q * kThreads;  // 0 .. 1023
+      const int cc = e / 64, r = e % 64;
+      const int64_t c = (int64_t)t * BKC + cc;
+      const int64_t crs = (c < g.h) ? c + g.h : c - g.h;
+      const int64_t j = j0 + r, i = i0 + r;
+      cr[q] = (c < two_h && j < g.n) ? g.F[j + crs * g.n] : 0.0;
+      cf[q] = (c < two_h && i < g.n) ? g.F[i + c * g.n] : 0.0;
     }
+  };
+  auto store_chunk = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + q * kThreads;
+      Rc[b][e / 64][e % 64] = cr[q];
+      Fc[b][e / 64][e % 64] = cf[q];
+    }
+  };
+
+  for (int64_t jb = jb0; jb < jb1; ++jb) {
+    const int64_t j0 = jb * BJ;
+    __syncthreads();  // previous tile's readers of XJ / Hs / Rc / Fc are done
+    for (int e = tid; e < DMAX * BJ; e += kThreads) {
+      const int jj = e % BJ, f = e / BJ;
+      const int64_t j = j0 + jj;
+      XJ[f][jj] = (j < g.n && f < g.d) ? g.X[j + (int64_t)f * g.ldp] : 0.0;
+    }
+    load_chunk(j0, 0);
+    store_chunk(0);
+    if (nch > 1) load_chunk(j0, 1);
     __syncthreads();
-    // Gram and S = F_I R_J^T on the same 2 x 2 wave tiling.
-    d4 gr[2][2], sa[2][2];
+
+    d4 sa[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+    for (int t = 0; t < nch; ++t) {
+      const int b = t & 1;
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+      for (int kk = 0; kk < BKC; kk += 4) {
+        const double a = Rc[b][kk + lq][16 * jt + l16];
 #pragma unroll
-      for (int y = 0; y < 2; ++y) { gr[x][y] = d4{0.0, 0.0, 0.0, 0.0}; sa[x][y] = gr[x][y]; }
+        for (int u = 0; u < 2; ++u)
+          sa[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Fc[b][kk + lq][16 * (2 * ih + u) + l16], sa[u], 0, 0, 0);
+      }
+      if (t + 1 < nch) {
+        store_chunk(b ^ 1);
+        if (t + 2 < nch) load_chunk(j0, t + 2);
+      }
+      __syncthreads();
+    }
+
+    // Gram^T (rows j of tile jt, columns i of tiles 2ih, 2ih+1)
+    d4 gr[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
     for (int t = 0; t < TF; ++t) {
       if (4 * t < g.d) {
+        const double a = XJ[4 * t + lq][16 * jt + l16];
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
-          const double b = Xj[4 * t + (lane >> 4)][wn * 32 + y * 16 + (lane & 15)];
-#pragma unroll
-          for (int x = 0; x < 2; ++x) gr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[x][t], b, gr[x][y], 0, 0, 0);
-        }
+        for (int u = 0; u < 2; ++u) gr[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xi_at(u, t), gr[u], 0, 0, 0);
       }
     }
-    for (int64_t c0 = 0; c0 < two_h; c0 += BKC) {
-      __syncthreads();
-      for (int e = tid; e < BKC * BM; e += kThreads) {
-        const int i = e % BM, cc = e / BM;
-        const int64_t gi = m0 + i, c = c0 + cc;
-        Fa[cc][i] = (gi < g.n && c < two_h) ? g.F[gi + c * g.n] : 0.0;
-        const int64_t gj = n0 + i;
-        const int64_t cr = (c < g.h) ? c + g.h : c - g.h;
-        Fb[cc][i] = (gj < g.n && c < two_h) ? g.F[gj + cr * g.n] : 0.0;
-      }
-      __syncthreads();
+    // H^T = 2 S^T o K^T, row sums over j, sum H o D; H^T -> LDS
 #pragma unroll
-      for (int kk = 0; kk < BKC; kk += 4) {
-        double a[2], b[2];
+    for (int u = 0; u < 2; ++u) {
+      const int il = 16 * (2 * ih + u) + l16;
+      const int64_t i = i0 + il;
 #pragma unroll
-        for (int x = 0; x < 2; ++x) a[x] = Fa[kk + (lane >> 4)][wm * 32 + x * 16 + (lane & 15)];
-#pragma unroll
-        for (int y = 0; y < 2; ++y) b[y] = Fb[kk + (lane >> 4)][wn * 32 + y * 16 + (lane & 15)];
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-          for (int y = 0; y < 2; ++y) sa[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], sa[x][y], 0, 0, 0);
-      }
-    }
-    // H = 2 S o K, its row sums and sum H o D, and H into LDS for H X_J.
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-      const int jl = wn * 32 + y * 16 + (lane & 15);
-      const int64_t j = n0 + jl;
-      const double nj = (j < g.n) ? g.nrm[j] : 0.0;
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int il = wm * 32 + x * 16 + (lane >> 4) + 4 * v;
-          const int64_t i = m0 + il;
-          double hv = 0.0;
-          if (i < g.n && j < g.n) {
-            double D = ni[x][v] + nj - 2.0 * gr[x][y][v];
-            D = (D > 0.0 && i != j) ? D : 0.0;
-            const double kv = (i == j) ? 1.0 : exp(-g.gamma * D);
-            hv = 2.0 * sa[x][y][v] * kv;
-            dg += hv * D;
-          }
-          rs[x][v] += hv;
-          Hs[jl][il] = hv;
+      for (int v = 0; v < 4; ++v) {
+        const int jl = 16 * jt + lq + 4 * v;
+        const int64_t j = j0 + jl;
+        double hv = 0.0;
+        if (i < g.n && j < g.n) {
+          double D = ni[u] + g.nrm[j] - 2.0 * gr[u][v];
+          D = (D > 0.0 && i != j) ? D : 0.0;
+          const double kv = (i == j) ? 1.0 : exp(-g.gamma * D);
+          hv = 2.0 * sa[u][v] * kv;
+          dg += hv * D;
         }
+        rs[u] += hv;
+        Hs[jl][il] = hv;
+      }
     }
     __syncthreads();
-    // HX_I += H (64 x 64) X_J (64 x d)
+    // (H X)^T(f, i) += sum_j X_J(j, f) H^T(j, i)
 #pragma unroll
-    for (int kk = 0; kk < BN; kk += 4) {
-      double a[2];
+    for (int kk = 0; kk < BJ; kk += 4) {
+      double bh[2];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) a[x] = Hs[kk + (lane >> 4)][wm * 32 + x * 16 + (lane & 15)];
+      for (int u = 0; u < 2; ++u) bh[u] = Hs[kk + lq][16 * (2 * ih + u) + l16];
 #pragma unroll
-      for (int y = 0; y < ND; ++y) {
-        const int f = wn * (DMAX / 2) + y * 16 + (lane & 15);
-        const double b = Xj[f][kk + (lane >> 4)];
+      for (int q = 0; q < FTW; ++q) {
+        const int ft = jt + 4 * q;
+        if (ft < FT) {
+          const double a = XJ[16 * ft + l16][kk + lq];
 #pragma unroll
-        for (int x = 0; x < 2; ++x) hx[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b, hx[x][y], 0, 0, 0);
+          for (int u = 0; u < 2; ++u) hx[q][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bh[u], hx[q][u], 0, 0, 0);
+        }
       }
     }
   }
-  // partial H X of this split
+
+  // (H X)^T result: lane l holds column i (l%16) and feature rows 16 ft + l/16 + 4v
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int q = 0; q < FTW; ++q) {
+    const int ft = jt + 4 * q;
+    if (ft >= FT) continue;
 #pragma unroll
-    for (int y = 0; y < ND; ++y)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t i = m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * v;
-        const int64_t f = wn * (DMAX / 2) + y * 16 + (lane & 15);
-        if (i < g.n && f < g.d) g.part_hx[(int64_t)split * g.n * g.d + i + f * g.n] = hx[x][y][v];
+        const int64_t i = i0 + 16 * (2 * ih + u) + l16;
+        const int64_t f = 16 * ft + lq + 4 * v;
+        if (i < g.n && f < g.d) g.part_hx[(int64_t)split * g.n * g.d + i + f * g.n] = hx[q][u][v];
       }
-  // row sums: reduce the 16 lanes sharing a row, one partial per column-wave
+  }
+  // row sums: over the 4 lane groups of a column, then over the 4 j-tile waves
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int u = 0; u < 2; ++u) {
+    double r = rs[u];
+    r += __shfl_xor(r, 16);
+    r += __shfl_xor(r, 32);
+    rs[u] = r;
+  }
+  __syncthreads();
+  if (jt == 0 && lq == 0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double r = rs[x][v];
+    for (int u = 0; u < 2; ++u) red[16 * (2 * ih + u) + l16] = 0.0;
+  }
+  __syncthreads();
+  for (int w = 0; w < 4; ++w) {  // fixed order over the j-tile waves
+    if (jt == w && lq == 0) {
 #pragma unroll
-      for (int o = 1; o < 16; o <<= 1) r += __shfl_xor(r, o);
-      const int64_t i = m0 + wm * 32 + x * 16 + (lane >> 4) + 4 * v;
-      if ((lane & 15) == 0 && i < g.n) g.part_rs[((int64_t)split * 2 + wn) * g.n + i] = r;
+      for (int u = 0; u < 2; ++u) red[16 * (2 * ih + u) + l16] += rs[u];
     }
+    __syncthreads();
+  }
+  if (tid < BM && i0 + tid < g.n) g.part_rs[(int64_t)split * g.n + i0 + tid] = red[tid];
   // sum H o D over the block
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) dg += __shfl_xor(dg, o);
-  if (lane == 0) red[wave] = dg;
+  if (lane == 0) redg[wave] = dg;
   __syncthreads();
-  if (tid == 0) g.part_dg[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (tid == 0) {
+    double sdg = 0.0;
+    for (int w = 0; w < 8; ++w) sdg += redg[w];
+    g.part_dg[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sdg;
+  }
 }
 
 // dX = -2 gamma (rowsum o X - HX), deterministic sums over the splits.
@@ -211,7 +269,7 @@ __global__ void k_adjoint_finish(const double* X, int64_t ldp, int64_t n, int64_
     double hx = 0.0, rs = 0.0;
     for (int s = 0; s < splits; ++s) {
       hx += part_hx[(int64_t)s * total + e];
-      rs += part_rs[((int64_t)s * 2 + 0) * n + i] + part_rs[((int64_t)s * 2 + 1) * n + i];
+      rs += part_rs[(int64_t)s * n + i];
     }
     dX[i + f * ldg] = -2.0 * gamma * (rs * X[i + f * ldp] - hx);
   }
@@ -271,13 +329,14 @@ void rbf_sketch_loss_grad(Ctx* c, const double* X, int64_t n, int64_t d, int64_t
     return;
   }
   // tile pass
+  // one 512-thread block per CU: split the J tiles until the grid covers
+  // the 256 CUs
   const int64_t iblocks = cdiv(n, 64), jblocks = cdiv(n, 64);
-  int splits = (int)std::max<int64_t>(1, std::min<int64_t>(jblocks, cdiv(1024, iblocks)));
+  int splits = (int)std::max<int64_t>(1, std::min<int64_t>(jblocks, cdiv(256, iblocks)));
   const int64_t per = cdiv(jblocks, splits);
   splits = (int)cdiv(jblocks, per);
-  DevBuf phx((size_t)splits * n * d * sizeof(double)), prs((size_t)splits * 2 * n * sizeof(double)),
+  DevBuf phx((size_t)splits * n * d * sizeof(double)), prs((size_t)splits * n * sizeof(double)),
       pdg((size_t)splits * iblocks * sizeof(double));
-  fill_f64(st, prs.as<double>(), splits * 2 * n, 0.0);
   AdjArgs g;
   g.X = X; g.ldp = ldp; g.d = d; g.n = n; g.nrm = nrm.as<double>(); g.gamma = gamma; g.F = Fp; g.h = h;
   g.jblocks_per_split = per; g.part_hx = phx.as<double>(); g.part_rs = prs.as<double>(); g.part_dg = pdg.as<double>();

@@ -1,0 +1,10 @@
+// Whole-matrix pass of linear_update over CSR input (closed form of the
+// window recurrence from a zero state; see csr_pass.hip).
+#pragma once
+#include "sketch.hpp"
+
+namespace skema {
+
+void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const double* val);
+
+}  // namespace skema

@@ -39,6 +39,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef int i8v __attribute__((ext_vector_type(8)));
 typedef int i4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
 constexpr int kQ = 64;  // quantize tile
@@ -243,6 +245,11 @@ struct NtArgs {
   float* part;  // FP32 partials [split][M x N] when K is split
 };
 
+// Rows i0 .. i0+3 of column j (the four accumulator values of one lane in
+// the 16x16 f32 result layout): contiguous in every column-major output, so
+// G goes out as one 8-byte (bf16) or 16-byte (fp32) store and the FP64
+// outputs as one 32-byte read-modify-write when aligned.
+__device__ __forceinline__ void store_nt4(const NtArgs& g, int64_t i0, int64_t j, const double (&v)[4]);
 __device__ __forceinline__ void store_nt(const NtArgs& g, int64_t i, int64_t j, double v) {
   if (g.gtype != 0 && j >= g.out.nsplit) {
     const int64_t e = i + (j - g.out.nsplit) * g.ldg;
@@ -259,6 +266,63 @@ __device__ __forceinline__ void store_nt(const NtArgs& g, int64_t i, int64_t j, 
   store_out(g.out, i, j, v);
 }
 
+__device__ __forceinline__ void store_nt4(const NtArgs& g, int64_t i0, int64_t j, const double (&v)[4]) {
+  if (i0 + 4 > g.M) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (i0 + q < g.M) store_nt(g, i0 + q, j, v[q]);
+    return;
+  }
+  if (g.gtype != 0 && j >= g.out.nsplit) {
+    const int64_t e = i0 + (j - g.out.nsplit) * g.ldg;
+    if (g.gtype == 3) {
+      float* p = static_cast<float*>(g.gh) + e;
+      if ((((uintptr_t)p) & 15) == 0) {
+        *reinterpret_cast<f4v*>(p) = f4v{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p[q] = (float)v[q];
+      }
+      return;
+    }
+    bf16x4 h, l;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float x = (float)v[q];
+      h[q] = (__bf16)x;
+      l[q] = (__bf16)(x - (float)h[q]);
+    }
+    __bf16* ph = static_cast<__bf16*>(g.gh) + e;
+    const bool al = (((uintptr_t)ph) & 7) == 0;
+    if (al) *reinterpret_cast<bf16x4*>(ph) = h;
+    else for (int q = 0; q < 4; ++q) ph[q] = h[q];
+    if (g.gtype == 2) {
+      __bf16* pl = static_cast<__bf16*>(g.gl) + e;
+      if ((((uintptr_t)pl) & 7) == 0) *reinterpret_cast<bf16x4*>(pl) = l;
+      else for (int q = 0; q < 4; ++q) pl[q] = l[q];
+    }
+    return;
+  }
+  const GemmOut& o = g.out;
+  double* c;
+  double alpha, beta;
+  if (j < o.nsplit) { c = o.C1 + i0 + j * o.ldc1; alpha = o.alpha1; beta = o.beta1; }
+  else { c = o.C2 + i0 + (j - o.nsplit) * o.ldc2; alpha = o.alpha2; beta = o.beta2; }
+  if ((((uintptr_t)c) & 31) == 0) {
+    d4v* cv = reinterpret_cast<d4v*>(c);
+    if (beta != 0.0) {
+      const d4v old = *cv;
+      *cv = d4v{alpha * v[0] + beta * old[0], alpha * v[1] + beta * old[1], alpha * v[2] + beta * old[2],
+                alpha * v[3] + beta * old[3]};
+    } else {
+      *cv = d4v{alpha * v[0], alpha * v[1], alpha * v[2], alpha * v[3]};
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c[q] = (beta != 0.0) ? alpha * v[q] + beta * c[q] : alpha * v[q];
+  }
+}
+
 // ---- the NT kernel -------------------------------------------------------------
 // Operand forms.  Every form stages BK elements of K per pipeline step as
 // 16-byte chunks into a K-contiguous LDS row of RS bytes (BK * E + 16 pad), and
@@ -271,42 +335,46 @@ __device__ __forceinline__ void store_nt(const NtArgs& g, int64_t i, int64_t j, 
 //   fp8    : 16x16x128 f8f6f4 with unit block scales, lane l reads 32 bytes at
 //            K 32*(l/16); the row scales multiply the sum in the epilogue.
 struct FormBf16 {
-  static constexpr int E = 1 * 2, BK = 64, NP = 1, KS = 32;
-  __device__ static void mma(f4v& acc, const uint8_t* const* pa, const uint8_t* const* pb, int lq) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa[0] + 16 * lq);
-    const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb[0] + 16 * lq);
+  static constexpr int E = 2, BK = 64, NP = 1, KS = 32;
+  using Frag = bf16x8;
+  __device__ static Frag load(const uint8_t* const* p, int lq) { return *reinterpret_cast<const bf16x8*>(p[0] + 16 * lq); }
+  __device__ static void mma(f4v& acc, const Frag& a, const Frag& b) {
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
   }
 };
+struct SplitFrag {
+  bf16x8 h, l;
+};
 struct FormBf16x3 {
   static constexpr int E = 2, BK = 32, NP = 2, KS = 32;
-  __device__ static void mma(f4v& acc, const uint8_t* const* pa, const uint8_t* const* pb, int lq) {
-    const bf16x8 ah = *reinterpret_cast<const bf16x8*>(pa[0] + 16 * lq);
-    const bf16x8 bh = *reinterpret_cast<const bf16x8*>(pb[0] + 16 * lq);
-    const bf16x8 al = *reinterpret_cast<const bf16x8*>(pa[1] + 16 * lq);
-    const bf16x8 bl = *reinterpret_cast<const bf16x8*>(pb[1] + 16 * lq);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+  using Frag = SplitFrag;
+  __device__ static Frag load(const uint8_t* const* p, int lq) {
+    return SplitFrag{*reinterpret_cast<const bf16x8*>(p[0] + 16 * lq), *reinterpret_cast<const bf16x8*>(p[1] + 16 * lq)};
+  }
+  __device__ static void mma(f4v& acc, const Frag& a, const Frag& b) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.h, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.l, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.l, b.h, acc, 0, 0, 0);
   }
 };
 struct FormF32 {
   static constexpr int E = 4, BK = 32, NP = 1, KS = 16;
-  __device__ static void mma(f4v& acc, const uint8_t* const* pa, const uint8_t* const* pb, int lq) {
-    const f4v a = *reinterpret_cast<const f4v*>(pa[0] + 16 * lq);
-    const f4v b = *reinterpret_cast<const f4v*>(pb[0] + 16 * lq);
+  using Frag = f4v;
+  __device__ static Frag load(const uint8_t* const* p, int lq) { return *reinterpret_cast<const f4v*>(p[0] + 16 * lq); }
+  __device__ static void mma(f4v& acc, const Frag& a, const Frag& b) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b[q], acc, 0, 0, 0);
   }
 };
 struct FormFp8 {
   static constexpr int E = 1, BK = 128, NP = 1, KS = 128;
-  __device__ static void mma(f4v& acc, const uint8_t* const* pa, const uint8_t* const* pb, int lq) {
-    const i4v* qa = reinterpret_cast<const i4v*>(pa[0] + 32 * lq);
-    const i4v* qb = reinterpret_cast<const i4v*>(pb[0] + 32 * lq);
-    const i4v a0 = qa[0], a1 = qa[1], b0 = qb[0], b1 = qb[1];
-    const i8v a = i8v{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
-    const i8v b = i8v{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+  using Frag = i8v;
+  __device__ static Frag load(const uint8_t* const* p, int lq) {
+    const i4v* q = reinterpret_cast<const i4v*>(p[0] + 32 * lq);
+    const i4v x0 = q[0], x1 = q[1];
+    return i8v{x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+  }
+  __device__ static void mma(f4v& acc, const Frag& a, const Frag& b) {
     acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc, 0, 0, 0, 127, 0, 127);
   }
 };
@@ -323,16 +391,16 @@ __device__ __forceinline__ double row_scale(const float* amax, int64_t r) {
 // are stored into the other buffer, and the loads of step t+2 are issued;
 // one barrier per step.  With one split the epilogue writes the output;
 // otherwise the FP32 partial goes to g.part[z] and k_nt_reduce finishes.
-template <class F, int BM, int BN>
+template <class F, int BM, int BN, bool DB>
 __global__ __launch_bounds__(kThreads) void k_nt(NtArgs g) {
-  constexpr int BK = F::BK, E = F::E, NP = F::NP;
+  constexpr int BK = F::BK, E = F::E, NP = F::NP, NB = DB ? 2 : 1;
   constexpr int RS = BK * E + 16;                 // LDS row stride (bytes)
   constexpr int CPR = BK * E / 16;                // 16-byte chunks per row and step
   constexpr int CA = BM * CPR / kThreads, CB = BN * CPR / kThreads;
   constexpr int TM = BM / 32, TN = BN / 32;
   static_assert(CA * kThreads == BM * CPR && CB * kThreads == BN * CPR, "tile / thread mismatch");
-  __shared__ __attribute__((aligned(16))) uint8_t sA[2][NP][BM * RS];
-  __shared__ __attribute__((aligned(16))) uint8_t sB[2][NP][BN * RS];
+  __shared__ __attribute__((aligned(16))) uint8_t sA[NB][NP][BM * RS];
+  __shared__ __attribute__((aligned(16))) uint8_t sB[NB][NP][BN * RS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
@@ -396,34 +464,52 @@ __global__ __launch_bounds__(kThreads) void k_nt(NtArgs g) {
   auto compute = [&](int b) {
 #pragma unroll
     for (int ks = 0; ks < BK; ks += F::KS) {
+      typename F::Frag fa[TM], fb[TN];
 #pragma unroll
       for (int x = 0; x < TM; ++x) {
         const uint8_t* pa[NP];
 #pragma unroll
         for (int h = 0; h < NP; ++h) pa[h] = &sA[b][h][(wm * (BM / 2) + x * 16 + l16) * RS + ks * E];
-#pragma unroll
-        for (int y = 0; y < TN; ++y) {
-          const uint8_t* pb[NP];
-#pragma unroll
-          for (int h = 0; h < NP; ++h) pb[h] = &sB[b][h][(wn * (BN / 2) + y * 16 + l16) * RS + ks * E];
-          F::mma(acc[x][y], pa, pb, lq);
-        }
+        fa[x] = F::load(pa, lq);
       }
+#pragma unroll
+      for (int y = 0; y < TN; ++y) {
+        const uint8_t* pb[NP];
+#pragma unroll
+        for (int h = 0; h < NP; ++h) pb[h] = &sB[b][h][(wn * (BN / 2) + y * 16 + l16) * RS + ks * E];
+        fb[y] = F::load(pb, lq);
+      }
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y) F::mma(acc[x][y], fa[x], fb[y]);
     }
   };
 
-  gload(kbeg);
-  sstore(0);
-  if (steps > 1) gload(kbeg + BK);
-  __syncthreads();
-  for (int st = 0; st < steps; ++st) {
-    const int b = st & 1;
-    compute(b);
-    if (st + 1 < steps) {
-      sstore(b ^ 1);
-      if (st + 2 < steps) gload(kbeg + (int64_t)(st + 2) * BK);
-    }
+  if constexpr (DB) {
+    gload(kbeg);
+    sstore(0);
+    if (steps > 1) gload(kbeg + BK);
     __syncthreads();
+    for (int st = 0; st < steps; ++st) {
+      const int b = st & 1;
+      compute(b);
+      if (st + 1 < steps) {
+        sstore(b ^ 1);
+        if (st + 2 < steps) gload(kbeg + (int64_t)(st + 2) * BK);
+      }
+      __syncthreads();
+    }
+  } else {
+    // single buffer: more blocks fit per CU, whose waves hide each other's loads
+    gload(kbeg);
+    for (int st = 0; st < steps; ++st) {
+      __syncthreads();
+      sstore(0);
+      __syncthreads();
+      if (st + 1 < steps) gload(kbeg + (int64_t)(st + 1) * BK);
+      compute(0);
+    }
   }
 
   // 16x16 f32 MFMA result: lane l holds rows 4*(l/16)+v of column l%16.
@@ -434,21 +520,29 @@ __global__ __launch_bounds__(kThreads) void k_nt(NtArgs g) {
     sb[y] = (j < g.N) ? row_scale<F>(g.amax_b, j) : 0.0;
   }
 #pragma unroll
-  for (int x = 0; x < TM; ++x)
+  for (int x = 0; x < TM; ++x) {
+    const int64_t i0 = m0 + wm * (BM / 2) + x * 16 + 4 * lq;
+    if (i0 >= g.M) continue;
+    double sa[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t i = m0 + wm * (BM / 2) + x * 16 + 4 * lq + v;
-      if (i >= g.M) continue;
-      const double sa = row_scale<F>(g.amax_a, i);
+    for (int v = 0; v < 4; ++v) sa[v] = (i0 + v < g.M) ? row_scale<F>(g.amax_a, i0 + v) : 0.0;
 #pragma unroll
-      for (int y = 0; y < TN; ++y) {
-        const int64_t j = n0 + wn * (BN / 2) + y * 16 + l16;
-        if (j >= g.N) continue;
-        const double val = (double)acc[x][y][v] * sa * sb[y];
-        if (g.part) g.part[(int64_t)blockIdx.z * g.M * g.N + i + j * g.M] = (float)val;
-        else store_nt(g, i, j, val);
+    for (int y = 0; y < TN; ++y) {
+      const int64_t j = n0 + wn * (BN / 2) + y * 16 + l16;
+      if (j >= g.N) continue;
+      double val[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) val[v] = (double)acc[x][y][v] * sa[v] * sb[y];
+      if (g.part) {
+        float* p = g.part + (int64_t)blockIdx.z * g.M * g.N + j * g.M;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (i0 + v < g.M) p[i0 + v] = (float)val[v];
+      } else {
+        store_nt4(g, i0, j, val);
       }
     }
+  }
 }
 
 // Sum of the FP32 partials of the K splits (in FP64, fixed order), then the
@@ -467,9 +561,10 @@ __global__ void k_nt_reduce(NtArgs g, int splits) {
 // CUs, each round costing BM * BN * K/splits, plus FP32 partial traffic.
 struct NtPlan {
   int bm, bn, splits;
+  bool db;
 };
 NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
-  const NtPlan cands[] = {{128, 128, 1}, {128, 64, 1}, {64, 128, 1}, {64, 64, 1}};
+  const NtPlan cands[] = {{128, 128, 1, true}, {128, 64, 1, true}, {64, 128, 1, true}, {64, 64, 1, true}};
   NtPlan best = cands[3];
   double best_cost = 1e300;
   const int kstep = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
@@ -487,12 +582,17 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   return best;
 }
 
+template <class F, bool DB>
+void launch_nt_db(hipStream_t st, const NtArgs& a, int bm, int bn, dim3 grid) {
+  if (bm == 128 && bn == 128) hipLaunchKernelGGL((k_nt<F, 128, 128, DB>), grid, dim3(kThreads), 0, st, a);
+  else if (bm == 128) hipLaunchKernelGGL((k_nt<F, 128, 64, DB>), grid, dim3(kThreads), 0, st, a);
+  else if (bn == 128) hipLaunchKernelGGL((k_nt<F, 64, 128, DB>), grid, dim3(kThreads), 0, st, a);
+  else hipLaunchKernelGGL((k_nt<F, 64, 64, DB>), grid, dim3(kThreads), 0, st, a);
+}
 template <class F>
-void launch_nt(hipStream_t st, const NtArgs& a, int bm, int bn, dim3 grid) {
-  if (bm == 128 && bn == 128) hipLaunchKernelGGL((k_nt<F, 128, 128>), grid, dim3(kThreads), 0, st, a);
-  else if (bm == 128) hipLaunchKernelGGL((k_nt<F, 128, 64>), grid, dim3(kThreads), 0, st, a);
-  else if (bn == 128) hipLaunchKernelGGL((k_nt<F, 64, 128>), grid, dim3(kThreads), 0, st, a);
-  else hipLaunchKernelGGL((k_nt<F, 64, 64>), grid, dim3(kThreads), 0, st, a);
+void launch_nt(hipStream_t st, const NtArgs& a, const NtPlan& pl, dim3 grid) {
+  if (pl.db) launch_nt_db<F, true>(st, a, pl.bm, pl.bn, grid);
+  else launch_nt_db<F, false>(st, a, pl.bm, pl.bn, grid);
 }
 
 void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
@@ -508,10 +608,10 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
   }
   a.tiles_m = (int)cdiv(a.M, pl.bm);
   const dim3 grid((unsigned)(a.tiles_m * cdiv(a.N, pl.bn)), 1, (unsigned)splits);
-  if (prec == SKEMA_PREC_FP8) launch_nt<FormFp8>(st, a, pl.bm, pl.bn, grid);
-  else if (prec == SKEMA_PREC_FP32) launch_nt<FormF32>(st, a, pl.bm, pl.bn, grid);
-  else if (prec == SKEMA_PREC_BF16X3) launch_nt<FormBf16x3>(st, a, pl.bm, pl.bn, grid);
-  else launch_nt<FormBf16>(st, a, pl.bm, pl.bn, grid);
+  if (prec == SKEMA_PREC_FP8) launch_nt<FormFp8>(st, a, pl, grid);
+  else if (prec == SKEMA_PREC_FP32) launch_nt<FormF32>(st, a, pl, grid);
+  else if (prec == SKEMA_PREC_BF16X3) launch_nt<FormBf16x3>(st, a, pl, grid);
+  else launch_nt<FormBf16>(st, a, pl, grid);
   check_launch("k_nt");
   if (splits > 1) {
     const int blocks = (int)std::min<int64_t>(cdiv(a.M * a.N, 256), 8192);

@@ -218,22 +218,37 @@ __global__ __launch_bounds__(kThreadsR) void k_rbf_right_bf16(RbfLpArgs g) {
 #pragma unroll
     for (int y = 0; y < TN; ++y) op[y] = opn[y];
   }
-  // 16x16 f32 result: lane l holds rows 4*(l/16)+v of column l%16.
+  // 16x16 f32 result: lane l holds rows 4*(l/16)+v of column l%16, i.e. four
+  // consecutive rows of one column-major column: stored as one vector when
+  // all four rows exist.
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
-    for (int y = 0; y < TN; ++y)
+    for (int y = 0; y < TN; ++y) {
+      const int64_t i0 = m0 + 16 * x + 4 * lq;
+      const int64_t j = cw0 + 16 * y + l16;
+      if (i0 >= g.M || j >= g.N) continue;
+      if (g.part) {
+        float* p = g.part + (int64_t)split * g.M * g.N + j * g.M;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t i = m0 + 16 * x + 4 * lq + v;
-        const int64_t j = cw0 + 16 * y + l16;
-        if (i < g.M && j < g.N) {
-          if (g.part) g.part[(int64_t)split * g.M * g.N + i + j * g.M] = acc[x][y][v];
-          else store_right(g, i, j, (double)acc[x][y][v]);
-        }
+        for (int v = 0; v < 4; ++v)
+          if (i0 + v < g.M) p[i0 + v] = acc[x][y][v];
+      } else if (i0 + 4 <= g.M && j >= g.out.nsplit) {
+        __bf16* pg = g.G + i0 + (j - g.out.nsplit) * g.ldg;
+        const bf16x4 h = bf16x4{(__bf16)acc[x][y][0], (__bf16)acc[x][y][1], (__bf16)acc[x][y][2],
+                                (__bf16)acc[x][y][3]};
+        if ((((uintptr_t)pg) & 7) == 0) *reinterpret_cast<bf16x4*>(pg) = h;
+        else for (int v = 0; v < 4; ++v) pg[v] = h[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (i0 + v < g.M) store_right(g, i0 + v, j, (double)acc[x][y][v]);
       }
+    }
 }
 
+// Sum of the FP32 partials of the point splits (fixed order), then the
+// epilogue; consecutive threads own consecutive rows, so the stores coalesce.
 __global__ void k_rbf_right_bf16_reduce(RbfLpArgs g, int splits) {
   const int64_t total = g.M * g.N;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {

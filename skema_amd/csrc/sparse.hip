@@ -117,23 +117,64 @@ __global__ void k_fill_csc(const int64_t* rowptr, int64_t rows, const int32_t* c
   }
 }
 
-// Insertion sort of each column segment by (row, value): makes the order of
-// summation independent of the atomic placement above.
-__global__ void k_sort_segments(const int64_t* ptr, int64_t cols, int32_t* row, double* val) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
-  const int64_t lo = ptr[c], hi = ptr[c + 1];
-  for (int64_t a = lo + 1; a < hi; ++a) {
-    const int32_t r = row[a];
-    const double v = val[a];
-    int64_t b = a - 1;
-    while (b >= lo && (row[b] > r || (row[b] == r && val[b] > v))) {
-      row[b + 1] = row[b];
-      val[b + 1] = val[b];
-      --b;
+// Sort of each column segment by (row, value): makes the order of summation
+// independent of the atomic placement above.  One 64-thread block per column:
+// a bitonic sort in LDS for up to 1024 entries, insertion sort by one thread
+// above that.
+constexpr int kSortMax = 1024;
+__global__ __launch_bounds__(64) void k_sort_columns(const int64_t* ptr, int32_t* row, double* val) {
+  __shared__ int32_t kr[kSortMax];
+  __shared__ double kv[kSortMax];
+  const int tid = threadIdx.x;
+  const int64_t c = blockIdx.x;
+  const int64_t lo = ptr[c], hi = ptr[c + 1], len = hi - lo;
+  if (len <= 1) return;
+  if (len > kSortMax) {
+    if (tid != 0) return;
+    for (int64_t a = lo + 1; a < hi; ++a) {
+      const int32_t r = row[a];
+      const double v = val[a];
+      int64_t b = a - 1;
+      while (b >= lo && (row[b] > r || (row[b] == r && val[b] > v))) {
+        row[b + 1] = row[b];
+        val[b + 1] = val[b];
+        --b;
+      }
+      row[b + 1] = r;
+      val[b + 1] = v;
     }
-    row[b + 1] = r;
-    val[b + 1] = v;
+    return;
+  }
+  int n = 1;
+  while (n < len) n <<= 1;
+  for (int t = tid; t < n; t += 64) {
+    const bool in = t < len;
+    kr[t] = in ? row[lo + t] : INT32_MAX;
+    kv[t] = in ? val[lo + t] : INFINITY;
+  }
+  __syncthreads();
+  for (int k = 2; k <= n; k <<= 1)
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int t = tid; t < n; t += 64) {
+        const int u = t ^ jj;
+        if (u > t) {
+          const bool up = (t & k) == 0;
+          const bool gt = kr[t] > kr[u] || (kr[t] == kr[u] && kv[t] > kv[u]);
+          if (gt == up) {
+            const int32_t r = kr[t];
+            kr[t] = kr[u];
+            kr[u] = r;
+            const double v = kv[t];
+            kv[t] = kv[u];
+            kv[u] = v;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int t = tid; t < len; t += 64) {
+    row[lo + t] = kr[t];
+    val[lo + t] = kv[t];
   }
 }
 
@@ -221,7 +262,7 @@ void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, co
 }
 
 CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val,
-                            int64_t rows, int64_t cols, DevBuf& scratch) {
+                            int64_t rows, int64_t cols, DevBuf& scratch, bool sorted) {
   int64_t ends[2] = {0, 0};
   SKEMA_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SKEMA_HIP(hipMemcpyAsync(&ends[1], rowptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -267,7 +308,7 @@ CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t
   if (nnz > 0) {
     hipLaunchKernelGGL(k_fill_csc, dim3(blocks_for(rows)), dim3(256), 0, st, rowptr, rows, col, val, cursor, w.ptr,
                        w.row, w.val);
-    hipLaunchKernelGGL(k_sort_segments, dim3(blocks_for(cols)), dim3(256), 0, st, w.ptr, cols, w.row, w.val);
+    if (sorted) hipLaunchKernelGGL(k_sort_columns, dim3((unsigned)cols), dim3(64), 0, st, w.ptr, w.row, w.val);
   }
   check_launch("csr_window_to_csc");
   return w;

@@ -1,5 +1,5 @@
-// Sparse products of the sketch path.  All are deterministic: every output
-// element is owned by one thread that sums its terms in a fixed order.
+// Sparse products of the windowed sketch path.  All are deterministic: every
+// output element is owned by one thread that sums its terms in a fixed order.
 #pragma once
 #include "common.hpp"
 #include "gemm.hpp"
@@ -23,8 +23,11 @@ struct CscWindow {
   int32_t* row = nullptr;   // nnz, window-relative row, ascending per column
   double* val = nullptr;    // nnz
 };
+// sorted: rows ascending within each column (a deterministic summation
+// order for the kernels that walk a column in order); callers that only
+// accumulate with atomics pass false.
 CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val,
-                            int64_t rows, int64_t cols, DevBuf& scratch);
+                            int64_t rows, int64_t cols, DevBuf& scratch, bool sorted = true);
 
 // X[p,j] = nu*X[p,j] + eta * sum_{(i,a) in column j of H} a * M[p, row0+i]
 // with M a dense k x * logical map (ldm).
