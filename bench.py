@@ -199,7 +199,8 @@ def make_points(cfg):
 
 
 def bench_sparse(args, S, torch):
-    """CSR window update with sparse-sign maps: bandwidth-bound gathers."""
+    """CSR linear_update with sparse-sign maps (one pass over the whole CSR):
+    bandwidth-bound gathers and scatters."""
     cfg = WORKLOADS[args.workload]
     A = make_sparse(cfg)
     Ad = S.CrsMatrix.from_scipy(A)
@@ -221,14 +222,18 @@ def bench_sparse(args, S, torch):
     ctx.set_profiling(False)
     stats = {kname: ctx.kernel_stats(kname) for kname in ("window", "sketch_left", "sketch_right", "sketch_core")}
     m, n, k, s, w, z = cfg["m"], cfg["n"], cfg["k"], cfg["s"], cfg["window"], cfg["zeta"]
-    nnz_w = A.nnz / (m / w)
-    # algorithmic bytes per launch (per window)
+    nnz = A.nnz
+    # Algorithmic HBM bytes of each stage of the whole-matrix pass (one
+    # group of launches per step): CSR / CSC entries 12 B (int32 index +
+    # double), row pointers 8 B, map-transpose entries 12 B, dense outputs 8 B
+    # per element (Y read and written: nu != 0).
     bytes_per = {
-        "sketch_right": 12 * nnz_w + 8 * (w + 1) + 24 * z * n + 16 * w * k + 8 * w * s,
-        "sketch_left": 12 * nnz_w + 8 * (n + 1) + 12 * z * w + 16 * k * n,
-        "sketch_core": 12 * z * w + 8 * w * s + 16 * s * s,
+        "window": 24 * nnz + 8 * (m + 1) + 8 * (n + 1),
+        "sketch_left": 12 * nnz + 8 * (n + 1) + 2 * (12 * z * m + 8 * (m + 1)) + 8 * (k + s) * n,
+        "sketch_right": 12 * nnz + 8 * (m + 1) + 12 * z * n + 8 * (n + 1) + 16 * m * k,
+        "sketch_core": 8 * s * n + 12 * z * n + 8 * (s + 1) + 8 * s * s,
     }
-    dom = max(bytes_per, key=lambda kname: stats[kname][1])
+    dom = max((kn for kn in bytes_per if stats[kn][0]), key=lambda kname: stats[kname][1])
     avg_ms = stats[dom][1] / stats[dom][0]
     achieved = bytes_per[dom] / (avg_ms * 1e-3) / 1e9
     out = {
@@ -240,6 +245,8 @@ def bench_sparse(args, S, torch):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": None, "bytes_per_launch": bytes_per[dom],
                      "avg_launch_ms": avg_ms},
+        "step_bytes": sum(bytes_per.values()),
+        "step_hbm_gbs": sum(bytes_per.values()) / dt / 1e9,
         "kernel_ms": {kname: (v[1] / v[0] if v[0] else None) for kname, v in stats.items()},
         "cpu_baseline": None,
     }

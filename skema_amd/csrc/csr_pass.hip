@@ -48,10 +48,35 @@ __global__ __launch_bounds__(256) void k_left_all_sparse(const int64_t* cptr, co
   __syncthreads();
   for (int64_t e = cptr[j] + tid; e < cptr[j + 1]; e += 256) {
     const int64_t i = crow[e];
-    const double a = cval[e] * cw[i / window];
+    const double a = cval[e] * cw[(uint32_t)i / (uint32_t)window];
     for (int64_t f = mrp[i]; f < mrp[i + 1]; ++f) {
       const int64_t p = (int64_t)mci[f] - p0;
       if (p >= 0 && p < P) atomicAdd(&acc[p], a * mv[f]);
+    }
+  }
+  __syncthreads();
+  for (int64_t p = tid; p < P; p += 256) out[(p0 + p) + j * ldo] = acc[p];
+}
+
+// The same with the map's transpose in ELL form (z entries per row,
+// u = row | sign << 31, magnitude mag): one contiguous 4z-byte read per
+// nonzero.
+__global__ __launch_bounds__(256) void k_left_all_ell(const int64_t* cptr, const int32_t* crow, const double* cval,
+                                                      const uint32_t* ell, int z, double mag, const double* cw,
+                                                      int64_t window, int64_t p0, int64_t P, double* out, int64_t ldo) {
+  extern __shared__ double acc[];  // P doubles
+  const int tid = threadIdx.x;
+  const int64_t j = blockIdx.x;
+  for (int64_t p = tid; p < P; p += 256) acc[p] = 0.0;
+  __syncthreads();
+  for (int64_t e = cptr[j] + tid; e < cptr[j + 1]; e += 256) {
+    const int64_t i = crow[e];
+    const double a = cval[e] * cw[(uint32_t)i / (uint32_t)window] * mag;
+    const uint32_t* me = ell + i * z;
+    for (int f = 0; f < z; ++f) {
+      const uint32_t u = me[f];
+      const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
+      if (p >= 0 && p < P) atomicAdd(&acc[p], (u >> 31) ? -a : a);
     }
   }
   __syncthreads();
@@ -94,6 +119,39 @@ __global__ __launch_bounds__(256) void k_right_all_sparse(const int64_t* rowptr,
   }
 }
 
+template <int PCH>
+__global__ __launch_bounds__(256) void k_right_all_ell(const int64_t* rowptr, const int32_t* col, const double* val,
+                                                       int64_t rows, const uint32_t* ell, int z, double mag,
+                                                       int64_t p0, int64_t P, double eta, double nu, double* Y,
+                                                       int64_t ldy) {
+  __shared__ double acc[PCH * kPad];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int64_t r1 = (r0 + 64 < rows) ? r0 + 64 : rows;
+  for (int e = tid; e < PCH * kPad; e += 256) acc[e] = 0.0;
+  __syncthreads();
+  const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
+  int64_t r = r0;
+  for (int64_t e = e0 + tid; e < e1; e += 256) {
+    while (rowptr[r + 1] <= e) ++r;
+    const double a = val[e] * mag;
+    const uint32_t* me = ell + (int64_t)col[e] * z;
+    for (int f = 0; f < z; ++f) {
+      const uint32_t u = me[f];
+      const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
+      if (p >= 0 && p < P) atomicAdd(&acc[p * kPad + (r - r0)], (u >> 31) ? -a : a);
+    }
+  }
+  __syncthreads();
+  const int64_t i = r0 + lane;
+  if (tid < 64 && i < rows) {
+    for (int64_t p = 0; p < P; ++p) {
+      double* y = Y + i + (p0 + p) * ldy;
+      *y = (nu != 0.0) ? nu * *y + eta * acc[p * kPad + lane] : eta * acc[p * kPad + lane];
+    }
+  }
+}
+
 // out(p, j) = sum over column j's nonzeros (i, a) of a cw[i / window] M(p, i),
 // M dense (P x m logical, column-major, ld ldm); one wave per column, lanes
 // own rows p (PR per lane).
@@ -109,7 +167,7 @@ __global__ __launch_bounds__(256) void k_left_all_dense(const int64_t* cptr, con
   for (int r = 0; r < PR; ++r) acc[r] = 0.0;
   for (int64_t e = cptr[j]; e < cptr[j + 1]; ++e) {
     const int64_t i = crow[e];
-    const double a = cval[e] * cw[i / window];
+    const double a = cval[e] * cw[(uint32_t)i / (uint32_t)window];
     const double* mcol = M + i * ldm;
 #pragma unroll
     for (int r = 0; r < PR; ++r) {
@@ -157,14 +215,20 @@ __global__ __launch_bounds__(256) void k_right_all_dense(const int64_t* rowptr, 
 
 unsigned nblocks(int64_t work, int64_t per) { return (unsigned)std::max<int64_t>(1, cdiv(work, per)); }
 
-// Sparse left product over map rows [0, P) in LDS chunks of up to 4096 rows.
-void left_sparse(hipStream_t st, const CscWindow& H, const Csr& Mt, int64_t P, const double* cw, int64_t window,
+// Sparse left product over map rows [0, P) in LDS chunks of up to 4096 rows
+// (ELL form of the map's transpose when it has one).
+void left_sparse(hipStream_t st, const CscWindow& H, const Map& M, int64_t P, const double* cw, int64_t window,
                  double* out, int64_t ldo) {
   if (H.cols == 0) return;
+  const Csr& Mt = M.logical_csr_t();
   for (int64_t p0 = 0; p0 < P; p0 += 4096) {
     const int64_t pc = std::min<int64_t>(4096, P - p0);
-    hipLaunchKernelGGL(k_left_all_sparse, dim3((unsigned)H.cols), dim3(256), (size_t)pc * sizeof(double), st, H.ptr,
-                       H.row, H.val, Mt.rp(), Mt.ci(), Mt.v(), cw, window, p0, pc, out, ldo);
+    if (M.ell_zeta > 0)
+      hipLaunchKernelGGL(k_left_all_ell, dim3((unsigned)H.cols), dim3(256), (size_t)pc * sizeof(double), st, H.ptr,
+                         H.row, H.val, M.ell.as<uint32_t>(), M.ell_zeta, M.ell_mag, cw, window, p0, pc, out, ldo);
+    else
+      hipLaunchKernelGGL(k_left_all_sparse, dim3((unsigned)H.cols), dim3(256), (size_t)pc * sizeof(double), st,
+                         H.ptr, H.row, H.val, Mt.rp(), Mt.ci(), Mt.v(), cw, window, p0, pc, out, ldo);
     check_launch("k_left_all_sparse");
   }
 }
@@ -199,6 +263,9 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
   const double eta = sk.p.eta, nu = sk.p.nu;
   // window partition of for_each_window: windows of p.window rows (the last
   // one shorter), or one window when window == m
+  // rows and windows fit 32 bits (CSR column indices are int32 and the
+  // window lookups below divide in 32 bits)
+  SKEMA_REQUIRE(m < ((int64_t)1 << 31), kErrShape, "linear_update_csr: more than 2^31 rows");
   const int64_t window = (sk.p.window >= m) ? m : sk.p.window;
   const int64_t W = cdiv(m, window);
   std::vector<double> hw((size_t)W);
@@ -226,16 +293,23 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
   if (sk.sparse_maps) {
     {
       Timed t(c, "sketch_left");
-      left_sparse(st, H, sk.maps[0]->logical_csr_t(), k, cw, window, X, k);
-      left_sparse(st, H, sk.maps[2]->logical_csr_t(), s, cw, window, Wm, s);
+      left_sparse(st, H, *sk.maps[0], k, cw, window, X, k);
+      left_sparse(st, H, *sk.maps[2], s, cw, window, Wm, s);
     }
     {
       Timed t(c, "sketch_right");
-      const Csr& Ot = sk.maps[1]->logical_csr_t();
+      const Map& Om = *sk.maps[1];
+      const Csr& Ot = Om.logical_csr_t();
       for (int64_t p0 = 0; p0 < k; p0 += 128) {
         const int64_t pc = std::min<int64_t>(128, k - p0);
         const dim3 grid(nblocks(m, 64));
-        if (pc <= 64)
+        if (Om.ell_zeta > 0 && pc <= 64)
+          hipLaunchKernelGGL(k_right_all_ell<64>, grid, dim3(256), 0, st, rowptr, col, val, m, Om.ell.as<uint32_t>(),
+                             Om.ell_zeta, Om.ell_mag, p0, pc, eta, nu, Y, m);
+        else if (Om.ell_zeta > 0)
+          hipLaunchKernelGGL(k_right_all_ell<128>, grid, dim3(256), 0, st, rowptr, col, val, m, Om.ell.as<uint32_t>(),
+                             Om.ell_zeta, Om.ell_mag, p0, pc, eta, nu, Y, m);
+        else if (pc <= 64)
           hipLaunchKernelGGL(k_right_all_sparse<64>, grid, dim3(256), 0, st, rowptr, col, val, m, Ot.rp(), Ot.ci(),
                              Ot.v(), p0, pc, eta, nu, Y, m);
         else

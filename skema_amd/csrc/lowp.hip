@@ -323,201 +323,19 @@ __device__ __forceinline__ void store_nt4(const NtArgs& g, int64_t i0, int64_t j
   }
 }
 
-// ---- the NT kernel -------------------------------------------------------------
-// Operand forms.  Every form stages BK elements of K per pipeline step as
-// 16-byte chunks into a K-contiguous LDS row of RS bytes (BK * E + 16 pad), and
-// feeds the matrix cores in sub-steps that read 16-byte fragments:
-//   bf16   : 16x16x32 bf16, lane l reads row l%16, K 8*(l/16) .. +7 of a 32-slice
-//   bf16x3 : same on hi and lo planes, three products per sub-step
-//   fp32   : 16x16x4 f32 x 4, lane l reads K 4*(l/16) .. +3 of a 16-slice and
-//            issues element q of A against element q of B (every K index of
-//            the slice is met once, so the sum is the same contraction)
-//   fp8    : 16x16x128 f8f6f4 with unit block scales, lane l reads 32 bytes at
-//            K 32*(l/16); the row scales multiply the sum in the epilogue.
-struct FormBf16 {
-  static constexpr int E = 2, BK = 64, NP = 1, KS = 32;
-  using Frag = bf16x8;
-  __device__ static Frag load(const uint8_t* const* p, int lq) { return *reinterpret_cast<const bf16x8*>(p[0] + 16 * lq); }
-  __device__ static void mma(f4v& acc, const Frag& a, const Frag& b) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-  }
-};
-struct SplitFrag {
-  bf16x8 h, l;
-};
-struct FormBf16x3 {
-  static constexpr int E = 2, BK = 32, NP = 2, KS = 32;
-  using Frag = SplitFrag;
-  __device__ static Frag load(const uint8_t* const* p, int lq) {
-    return SplitFrag{*reinterpret_cast<const bf16x8*>(p[0] + 16 * lq), *reinterpret_cast<const bf16x8*>(p[1] + 16 * lq)};
-  }
-  __device__ static void mma(f4v& acc, const Frag& a, const Frag& b) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.h, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.h, b.l, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.l, b.h, acc, 0, 0, 0);
-  }
-};
-struct FormF32 {
-  static constexpr int E = 4, BK = 32, NP = 1, KS = 16;
-  using Frag = f4v;
-  __device__ static Frag load(const uint8_t* const* p, int lq) { return *reinterpret_cast<const f4v*>(p[0] + 16 * lq); }
-  __device__ static void mma(f4v& acc, const Frag& a, const Frag& b) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b[q], acc, 0, 0, 0);
-  }
-};
-struct FormFp8 {
-  static constexpr int E = 1, BK = 128, NP = 1, KS = 128;
-  using Frag = i8v;
-  __device__ static Frag load(const uint8_t* const* p, int lq) {
-    const i4v* q = reinterpret_cast<const i4v*>(p[0] + 32 * lq);
-    const i4v x0 = q[0], x1 = q[1];
-    return i8v{x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-  }
-  __device__ static void mma(f4v& acc, const Frag& a, const Frag& b) {
-    acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, acc, 0, 0, 0, 127, 0, 127);
-  }
-};
-
-template <class F>
-__device__ __forceinline__ double row_scale(const float* amax, int64_t r) {
-  if constexpr (std::is_same<F, FormFp8>::value) return ldexp(1.0, block_exponent(amax[r]));
-  else return 1.0;
-}
-
-// C(i, j) = sum_k A(i, k) B(j, k) over this block's K range (blockIdx.z of
-// g.splits), BM x BN per 256-thread block as 2 x 2 waves.  LDS is double
-// buffered: the chunks of step t+1 sit in registers while step t computes,
-// are stored into the other buffer, and the loads of step t+2 are issued;
-// one barrier per step.  With one split the epilogue writes the output;
-// otherwise the FP32 partial goes to g.part[z] and k_nt_reduce finishes.
-template <class F, int BM, int BN, bool DB>
-__global__ __launch_bounds__(kThreads) void k_nt(NtArgs g) {
-  constexpr int BK = F::BK, E = F::E, NP = F::NP, NB = DB ? 2 : 1;
-  constexpr int RS = BK * E + 16;                 // LDS row stride (bytes)
-  constexpr int CPR = BK * E / 16;                // 16-byte chunks per row and step
-  constexpr int CA = BM * CPR / kThreads, CB = BN * CPR / kThreads;
-  constexpr int TM = BM / 32, TN = BN / 32;
-  static_assert(CA * kThreads == BM * CPR && CB * kThreads == BN * CPR, "tile / thread mismatch");
-  __shared__ __attribute__((aligned(16))) uint8_t sA[NB][NP][BM * RS];
-  __shared__ __attribute__((aligned(16))) uint8_t sB[NB][NP][BN * RS];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
-  const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * g.k_per_split;
-  const int64_t kend = (g.K < kbeg + g.k_per_split) ? g.K : kbeg + g.k_per_split;
-  const int steps = (int)((kend - kbeg + BK - 1) / BK);
-  const uint8_t* Ap[2] = {static_cast<const uint8_t*>(g.ah), static_cast<const uint8_t*>(g.al)};
-  const uint8_t* Bp[2] = {static_cast<const uint8_t*>(g.bh), static_cast<const uint8_t*>(g.bl)};
-
-  f4v acc[TM][TN];
-#pragma unroll
-  for (int x = 0; x < TM; ++x)
-#pragma unroll
-    for (int y = 0; y < TN; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
-  i4v ra[NP][CA], rb[NP][CB];
-
-  // 16 bytes of row `row` at element k (absolute), zero past kend / nrows.
-  auto load16 = [&](const uint8_t* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
-    const uint8_t* src = base + (row * ld + koff + k) * E;
-    i4v v;
-    if (row < nrows && k + 16 / E <= kend && (((uintptr_t)src) & 15) == 0) {
-      v = *reinterpret_cast<const i4v*>(src);
-    } else {
-      uint8_t bytes[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) bytes[q] = (row < nrows && k + q / E < kend) ? src[q] : (uint8_t)0;
-      v = *reinterpret_cast<const i4v*>(bytes);
-    }
-    return v;
-  };
-  auto gload = [&](int64_t k0) {
-#pragma unroll
-    for (int e = 0; e < CA; ++e) {
-      const int c = tid + e * kThreads, row = c / CPR, kc = (c % CPR) * (16 / E);
-#pragma unroll
-      for (int h = 0; h < NP; ++h) ra[h][e] = load16(Ap[h], m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
-    }
-#pragma unroll
-    for (int e = 0; e < CB; ++e) {
-      const int c = tid + e * kThreads, row = c / CPR, kc = (c % CPR) * (16 / E);
-#pragma unroll
-      for (int h = 0; h < NP; ++h) rb[h][e] = load16(Bp[h], n0 + row, g.N, g.ldb, 0, k0 + kc);
-    }
-  };
-  auto sstore = [&](int b) {
-#pragma unroll
-    for (int e = 0; e < CA; ++e) {
-      const int c = tid + e * kThreads, row = c / CPR, cb = (c % CPR) * 16;
-#pragma unroll
-      for (int h = 0; h < NP; ++h) *reinterpret_cast<i4v*>(&sA[b][h][row * RS + cb]) = ra[h][e];
-    }
-#pragma unroll
-    for (int e = 0; e < CB; ++e) {
-      const int c = tid + e * kThreads, row = c / CPR, cb = (c % CPR) * 16;
-#pragma unroll
-      for (int h = 0; h < NP; ++h) *reinterpret_cast<i4v*>(&sB[b][h][row * RS + cb]) = rb[h][e];
-    }
-  };
-  auto compute = [&](int b) {
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += F::KS) {
-      typename F::Frag fa[TM], fb[TN];
-#pragma unroll
-      for (int x = 0; x < TM; ++x) {
-        const uint8_t* pa[NP];
-#pragma unroll
-        for (int h = 0; h < NP; ++h) pa[h] = &sA[b][h][(wm * (BM / 2) + x * 16 + l16) * RS + ks * E];
-        fa[x] = F::load(pa, lq);
-      }
-#pragma unroll
-      for (int y = 0; y < TN; ++y) {
-        const uint8_t* pb[NP];
-#pragma unroll
-        for (int h = 0; h < NP; ++h) pb[h] = &sB[b][h][(wn * (BN / 2) + y * 16 + l16) * RS + ks * E];
-        fb[y] = F::load(pb, lq);
-      }
-#pragma unroll
-      for (int x = 0; x < TM; ++x)
-#pragma unroll
-        for (int y = 0; y < TN; ++y) F::mma(acc[x][y], fa[x], fb[y]);
-    }
-  };
-
-  if constexpr (DB) {
-    gload(kbeg);
-    sstore(0);
-    if (steps > 1) gload(kbeg + BK);
-    __syncthreads();
-    for (int st = 0; st < steps; ++st) {
-      const int b = st & 1;
-      compute(b);
-      if (st + 1 < steps) {
-        sstore(b ^ 1);
-        if (st + 2 < steps) gload(kbeg + (int64_t)(st + 2) * BK);
-      }
-      __syncthreads();
-    }
-  } else {
-    // single buffer: more blocks fit per CU, whose waves hide each other's loads
-    gload(kbeg);
-    for (int st = 0; st < steps; ++st) {
-      __syncthreads();
-      sstore(0);
-      __syncthreads();
-      if (st + 1 < steps) gload(kbeg + (int64_t)(st + 1) * BK);
-      compute(0);
-    }
-  }
-
-  // 16x16 f32 MFMA result: lane l holds rows 4*(l/16)+v of column l%16.
+// Epilogue of the NT kernels: lane l holds rows 4*(l/16)+v of column l%16 of
+// each 16x16 tile, i.e. four consecutive rows of a column-major output.  fp8
+// (SCALED) multiplies by the two operands' row scales.  With a K split the
+// FP32 partial goes to g.part[z] for k_nt_reduce.
+template <int TM, int TN, int BM, int BN, bool SCALED>
+__device__ __forceinline__ void nt_epilogue(const NtArgs& g, const f4v (&acc)[TM][TN], int64_t m0, int64_t n0, int wm,
+                                            int wn, int lane) {
+  const int l16 = lane & 15, lq = lane >> 4;
   double sb[TN];
 #pragma unroll
   for (int y = 0; y < TN; ++y) {
     const int64_t j = n0 + wn * (BN / 2) + y * 16 + l16;
-    sb[y] = (j < g.N) ? row_scale<F>(g.amax_b, j) : 0.0;
+    sb[y] = (SCALED && j < g.N) ? ldexp(1.0, block_exponent(g.amax_b[j])) : 1.0;
   }
 #pragma unroll
   for (int x = 0; x < TM; ++x) {
@@ -525,7 +343,7 @@ __global__ __launch_bounds__(kThreads) void k_nt(NtArgs g) {
     if (i0 >= g.M) continue;
     double sa[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) sa[v] = (i0 + v < g.M) ? row_scale<F>(g.amax_a, i0 + v) : 0.0;
+    for (int v = 0; v < 4; ++v) sa[v] = (SCALED && i0 + v < g.M) ? ldexp(1.0, block_exponent(g.amax_a[i0 + v])) : 1.0;
 #pragma unroll
     for (int y = 0; y < TN; ++y) {
       const int64_t j = n0 + wn * (BN / 2) + y * 16 + l16;
@@ -545,6 +363,292 @@ __global__ __launch_bounds__(kThreads) void k_nt(NtArgs g) {
   }
 }
 
+// ---- NT kernels: C(i, j) = sum over this block's K range (blockIdx.z of the
+// split) of A(i, k) B(j, k), A and B K-contiguous, 2 x 2 waves per block ----
+template <int BM, int BN, bool SPLIT>
+__global__ __launch_bounds__(kThreads) void k_nt_bf16(NtArgs g) {
+  constexpr int BK = 64, LDK = BK + 8;
+  constexpr int TM = BM / 32, TN = BN / 32;
+  constexpr int NP = SPLIT ? 2 : 1;
+  constexpr int CA = BM * BK / 8 / kThreads, CB = BN * BK / 8 / kThreads;
+  __shared__ __attribute__((aligned(16))) __bf16 sA[NP][BM][LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[NP][BN][LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
+  const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.k_per_split;
+  const int64_t kend = (g.K < kbeg + g.k_per_split) ? g.K : kbeg + g.k_per_split;
+  const __bf16* A[2] = {static_cast<const __bf16*>(g.ah), static_cast<const __bf16*>(g.al)};
+  const __bf16* B[2] = {static_cast<const __bf16*>(g.bh), static_cast<const __bf16*>(g.bl)};
+
+  f4v acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+  bf16x8 ra[NP][CA], rb[NP][CB];
+
+  auto load_chunk = [&](const __bf16* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
+    const __bf16* src = base + row * ld + koff + k;
+    bf16x8 v;
+    if (row < nrows && k + 8 <= kend && (((uintptr_t)src) & 15) == 0) {
+      v = *reinterpret_cast<const bf16x8*>(src);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = (row < nrows && k + q < kend) ? src[q] : (__bf16)0.0f;
+    }
+    return v;
+  };
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+#pragma unroll
+      for (int h = 0; h < NP; ++h) ra[h][e] = load_chunk(A[h], m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+#pragma unroll
+      for (int h = 0; h < NP; ++h) rb[h][e] = load_chunk(B[h], n0 + row, g.N, g.ldb, 0, k0 + kc);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+#pragma unroll
+      for (int h = 0; h < NP; ++h) *reinterpret_cast<bf16x8*>(&sA[h][row][kc]) = ra[h][e];
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
+#pragma unroll
+      for (int h = 0; h < NP; ++h) *reinterpret_cast<bf16x8*>(&sB[h][row][kc]) = rb[h][e];
+    }
+  };
+
+  gload(kbeg);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + BK < kend) gload(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 32) {
+      const int kk = ks + 8 * (lane >> 4);
+      bf16x8 a[NP][TM], b[NP][TN];
+#pragma unroll
+      for (int h = 0; h < NP; ++h) {
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+          a[h][x] = *reinterpret_cast<const bf16x8*>(&sA[h][wm * (BM / 2) + x * 16 + (lane & 15)][kk]);
+#pragma unroll
+        for (int y = 0; y < TN; ++y)
+          b[h][y] = *reinterpret_cast<const bf16x8*>(&sB[h][wn * (BN / 2) + y * 16 + (lane & 15)][kk]);
+      }
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < TN; ++y) {
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[0][y], acc[x][y], 0, 0, 0);
+          if (SPLIT) {
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][x], b[NP - 1][y], acc[x][y], 0, 0, 0);
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[NP - 1][x], b[0][y], acc[x][y], 0, 0, 0);
+          }
+        }
+    }
+  }
+  // 16x16 f32 MFMA result: lane l holds rows 4*(l/16)+v of column l%16.
+  nt_epilogue<TM, TN, BM, BN, false>(g, acc, m0, n0, wm, wn, lane);
+}
+
+// FP32 NT GEMM on v_mfma_f32_16x16x4_f32.  Each lane reads 4 consecutive K
+// elements of its row (K quarter l/16 of a 16-wide chunk) with one 16-byte LDS
+// read and issues 4 MFMAs, element q of A with element q of B: every K index
+// of the chunk is met exactly once, so the sum is the same contraction.
+template <int BM, int BN>
+__global__ __launch_bounds__(kThreads) void k_nt_f32(NtArgs g) {
+  constexpr int BK = 32, LDK = BK + 4;
+  constexpr int TM = BM / 32, TN = BN / 32;
+  constexpr int CA = BM * BK / 4 / kThreads, CB = BN * BK / 4 / kThreads;
+  __shared__ __attribute__((aligned(16))) float sA[BM][LDK];
+  __shared__ __attribute__((aligned(16))) float sB[BN][LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
+  const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.k_per_split;
+  const int64_t kend = (g.K < kbeg + g.k_per_split) ? g.K : kbeg + g.k_per_split;
+  const float* A = static_cast<const float*>(g.ah);
+  const float* B = static_cast<const float*>(g.bh);
+
+  f4v acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+  f4v ra[CA], rb[CB];
+
+  auto load4 = [&](const float* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
+    const float* src = base + row * ld + koff + k;
+    f4v v;
+    if (row < nrows && k + 4 <= kend && (((uintptr_t)src) & 15) == 0) {
+      v = *reinterpret_cast<const f4v*>(src);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = (row < nrows && k + q < kend) ? src[q] : 0.f;
+    }
+    return v;
+  };
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 4), kc = (c % (BK / 4)) * 4;
+      ra[e] = load4(A, m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 4), kc = (c % (BK / 4)) * 4;
+      rb[e] = load4(B, n0 + row, g.N, g.ldb, 0, k0 + kc);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 4), kc = (c % (BK / 4)) * 4;
+      *reinterpret_cast<f4v*>(&sA[row][kc]) = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 4), kc = (c % (BK / 4)) * 4;
+      *reinterpret_cast<f4v*>(&sB[row][kc]) = rb[e];
+    }
+  };
+
+  gload(kbeg);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + BK < kend) gload(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 16) {
+      const int kk = ks + 4 * (lane >> 4);
+      f4v a[TM], b[TN];
+#pragma unroll
+      for (int x = 0; x < TM; ++x) a[x] = *reinterpret_cast<const f4v*>(&sA[wm * (BM / 2) + x * 16 + (lane & 15)][kk]);
+#pragma unroll
+      for (int y = 0; y < TN; ++y) b[y] = *reinterpret_cast<const f4v*>(&sB[wn * (BN / 2) + y * 16 + (lane & 15)][kk]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+          for (int y = 0; y < TN; ++y)
+            acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[x][q], b[y][q], acc[x][y], 0, 0, 0);
+    }
+  }
+  nt_epilogue<TM, TN, BM, BN, false>(g, acc, m0, n0, wm, wn, lane);
+}
+
+// FP8 NT GEMM: BK = 128 = one MFMA; lane l feeds 32 bytes (row l%16, K
+// indices 32*(l/16) .. +31).  Block scales are all 2^0 (E8M0 127); the row
+// scales of A and B multiply the FP32 sum in the epilogue.
+template <int BM, int BN>
+__global__ __launch_bounds__(kThreads) void k_nt_fp8(NtArgs g) {
+  constexpr int BK = 128, LDK = BK + 16;
+  constexpr int TM = BM / 32, TN = BN / 32;
+  constexpr int CA = BM * BK / 16 / kThreads, CB = BN * BK / 16 / kThreads;
+  __shared__ __attribute__((aligned(16))) uint8_t sA[BM][LDK];
+  __shared__ __attribute__((aligned(16))) uint8_t sB[BN][LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
+  const int64_t n0 = (int64_t)(t / g.tiles_m) * BN;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.k_per_split;
+  const int64_t kend = (g.K < kbeg + g.k_per_split) ? g.K : kbeg + g.k_per_split;
+  const uint8_t* A = static_cast<const uint8_t*>(g.ah);
+  const uint8_t* B = static_cast<const uint8_t*>(g.bh);
+
+  f4v acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = f4v{0.f, 0.f, 0.f, 0.f};
+  i4v ra[CA], rb[CB];
+
+  auto load16 = [&](const uint8_t* base, int64_t row, int64_t nrows, int64_t ld, int64_t koff, int64_t k) {
+    const uint8_t* src = base + row * ld + koff + k;
+    i4v v;
+    if (row < nrows && k + 16 <= kend && (((uintptr_t)src) & 15) == 0) {
+      v = *reinterpret_cast<const i4v*>(src);
+    } else {
+      uint8_t b[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) b[q] = (row < nrows && k + q < kend) ? src[q] : (uint8_t)0;
+      v = *reinterpret_cast<const i4v*>(b);
+    }
+    return v;
+  };
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
+      ra[e] = load16(A, m0 + row, g.M, g.lda, g.koff_a, k0 + kc);
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
+      rb[e] = load16(B, n0 + row, g.N, g.ldb, 0, k0 + kc);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int e = 0; e < CA; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
+      *reinterpret_cast<i4v*>(&sA[row][kc]) = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) {
+      const int c = tid + e * kThreads, row = c / (BK / 16), kc = (c % (BK / 16)) * 16;
+      *reinterpret_cast<i4v*>(&sB[row][kc]) = rb[e];
+    }
+  };
+
+  gload(kbeg);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + BK < kend) gload(k0 + BK);
+    const int kb = lane >> 4;
+    i8v a[TM], b[TN];
+#pragma unroll
+    for (int x = 0; x < TM; ++x) {
+      const i4v* p = reinterpret_cast<const i4v*>(&sA[wm * (BM / 2) + x * 16 + (lane & 15)][32 * kb]);
+      const i4v lo = p[0], hi = p[1];
+      a[x] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    }
+#pragma unroll
+    for (int y = 0; y < TN; ++y) {
+      const i4v* p = reinterpret_cast<const i4v*>(&sB[wn * (BN / 2) + y * 16 + (lane & 15)][32 * kb]);
+      const i4v lo = p[0], hi = p[1];
+      b[y] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    }
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+#pragma unroll
+      for (int y = 0; y < TN; ++y)
+        acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[x], b[y], acc[x][y], 0, 0, 0, 127, 0, 127);
+  }
+  nt_epilogue<TM, TN, BM, BN, true>(g, acc, m0, n0, wm, wn, lane);
+}
+
 // Sum of the FP32 partials of the K splits (in FP64, fixed order), then the
 // epilogue.
 __global__ void k_nt_reduce(NtArgs g, int splits) {
@@ -556,15 +660,35 @@ __global__ void k_nt_reduce(NtArgs g, int splits) {
   }
 }
 
-// Tile and K-split of one NT product.  Candidate tiles are 64/128 x 64/128;
-// a launch is modelled as rounds of `res` co-resident blocks per CU over 256
-// CUs, each round costing BM * BN * K/splits, plus FP32 partial traffic.
+// Tile and K-split of one NT product: measured choices for the three window
+// products, otherwise a model over 64/128 x 64/128 tiles (a launch as rounds
+// of `res` co-resident blocks per CU over 256 CUs, each round costing
+// BM * BN * K/splits, plus FP32 partial traffic).
 struct NtPlan {
   int bm, bn, splits;
-  bool db;
 };
 NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
-  const NtPlan cands[] = {{128, 128, 1, true}, {128, 64, 1, true}, {64, 128, 1, true}, {64, 64, 1, true}};
+  const int ks = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
+  // Shapes of the window products, with the tile and split measured fastest
+  // for each form at w = 4096, n = 16384, k = 401, s = 803 (one MI355X):
+  //   right  [y | G] = A_w [Omega; Psi]^T : M = w, N = k+s, K = n (long)
+  //   left   x = Upsilon_w A_w            : M = k, N = n, K = w
+  //   core   z = Phi_w G                  : M = N = s, K = w
+  if (M >= 2048 && N <= 2048 && K >= 8192) {
+    if (prec == SKEMA_PREC_FP8) return {128, 64, 1};
+    if (prec == SKEMA_PREC_FP32) return {128, 128, 4};
+    return {64, 64, 2};
+  }
+  if (M <= 512 && N >= 4096 && K >= 16 * ks) {
+    if (prec == SKEMA_PREC_BF16) return {128, 64, 1};
+    if (prec == SKEMA_PREC_FP8) return {64, 64, 1};
+    return {64, 64, 2};
+  }
+  if (M <= 1024 && N <= 1024 && K >= 2048) {
+    if (prec == SKEMA_PREC_BF16X3) return {64, 64, 2};
+    return {128, 128, 4};
+  }
+  const NtPlan cands[] = {{128, 128, 1}, {128, 64, 1}, {64, 128, 1}, {64, 64, 1}};
   NtPlan best = cands[3];
   double best_cost = 1e300;
   const int kstep = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
@@ -582,17 +706,18 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   return best;
 }
 
-template <class F, bool DB>
-void launch_nt_db(hipStream_t st, const NtArgs& a, int bm, int bn, dim3 grid) {
-  if (bm == 128 && bn == 128) hipLaunchKernelGGL((k_nt<F, 128, 128, DB>), grid, dim3(kThreads), 0, st, a);
-  else if (bm == 128) hipLaunchKernelGGL((k_nt<F, 128, 64, DB>), grid, dim3(kThreads), 0, st, a);
-  else if (bn == 128) hipLaunchKernelGGL((k_nt<F, 64, 128, DB>), grid, dim3(kThreads), 0, st, a);
-  else hipLaunchKernelGGL((k_nt<F, 64, 64, DB>), grid, dim3(kThreads), 0, st, a);
+template <int BM, int BN>
+void launch_nt_tile(hipStream_t st, int prec, const NtArgs& a, dim3 grid) {
+  if (prec == SKEMA_PREC_FP8) hipLaunchKernelGGL((k_nt_fp8<BM, BN>), grid, dim3(kThreads), 0, st, a);
+  else if (prec == SKEMA_PREC_FP32) hipLaunchKernelGGL((k_nt_f32<BM, BN>), grid, dim3(kThreads), 0, st, a);
+  else if (prec == SKEMA_PREC_BF16X3) hipLaunchKernelGGL((k_nt_bf16<BM, BN, true>), grid, dim3(kThreads), 0, st, a);
+  else hipLaunchKernelGGL((k_nt_bf16<BM, BN, false>), grid, dim3(kThreads), 0, st, a);
 }
-template <class F>
-void launch_nt(hipStream_t st, const NtArgs& a, const NtPlan& pl, dim3 grid) {
-  if (pl.db) launch_nt_db<F, true>(st, a, pl.bm, pl.bn, grid);
-  else launch_nt_db<F, false>(st, a, pl.bm, pl.bn, grid);
+void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid) {
+  if (pl.bm == 128 && pl.bn == 128) launch_nt_tile<128, 128>(st, prec, a, grid);
+  else if (pl.bm == 128) launch_nt_tile<128, 64>(st, prec, a, grid);
+  else if (pl.bn == 128) launch_nt_tile<64, 128>(st, prec, a, grid);
+  else launch_nt_tile<64, 64>(st, prec, a, grid);
 }
 
 void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
@@ -608,10 +733,7 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
   }
   a.tiles_m = (int)cdiv(a.M, pl.bm);
   const dim3 grid((unsigned)(a.tiles_m * cdiv(a.N, pl.bn)), 1, (unsigned)splits);
-  if (prec == SKEMA_PREC_FP8) launch_nt<FormFp8>(st, a, pl, grid);
-  else if (prec == SKEMA_PREC_FP32) launch_nt<FormF32>(st, a, pl, grid);
-  else if (prec == SKEMA_PREC_BF16X3) launch_nt<FormBf16x3>(st, a, pl, grid);
-  else launch_nt<FormBf16>(st, a, pl, grid);
+  launch_nt(st, prec, a, pl, grid);
   check_launch("k_nt");
   if (splits > 1) {
     const int blocks = (int)std::min<int64_t>(cdiv(a.M * a.N, 256), 8192);

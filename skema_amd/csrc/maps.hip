@@ -126,6 +126,31 @@ Map* make_sparse_map(Ctx* ctx, int64_t nrow, int64_t ncol, int64_t nnz, const in
   std::vector<double> tv;
   build_csr_transpose_host(nrow, ncol, nnz, rp, ci.data(), v.data(), trp, tci, tv);
   upload_csr(m->csr_t, ncol, nrow, nnz, trp.data(), tci.data(), tv.data(), ctx->stream);
+  // ELL view of the logical transpose: rows of the stored CSR when the map
+  // is stored transposed, rows of its transpose otherwise
+  {
+    const bool st = transposed;
+    const int64_t lc = st ? nrow : ncol;  // logical columns = rows of the logical transpose
+    const int64_t* p = st ? rp : trp.data();
+    const int32_t* c = st ? ci.data() : tci.data();
+    const double* val = st ? v.data() : tv.data();
+    const int64_t z = lc > 0 ? p[1] - p[0] : 0;
+    bool ok = z > 0 && z <= 16 && nnz == z * lc && val[0] != 0.0;
+    const double mag = ok ? std::fabs(val[0]) : 0.0;
+    for (int64_t r = 0; ok && r < lc; ++r) ok = (p[r + 1] - p[r] == z);
+    for (int64_t e = 0; ok && e < nnz; ++e) ok = (std::fabs(val[e]) == mag);
+    if (ok) {
+      std::vector<uint32_t> h((size_t)nnz);
+      for (int64_t e = 0; e < nnz; ++e)  // entry e is (row e / z, slot e % z): rows hold z entries each
+        h[(size_t)e] = (uint32_t)c[e] | (val[e] < 0.0 ? 0x80000000u : 0u);
+      m->ell.alloc((size_t)nnz * sizeof(uint32_t));
+      SKEMA_HIP(hipMemcpyAsync(m->ell.p, h.data(), (size_t)nnz * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               ctx->stream));
+      SKEMA_HIP(hipStreamSynchronize(ctx->stream));
+      m->ell_zeta = (int)z;
+      m->ell_mag = mag;
+    }
+  }
   return m.release();
 }
 

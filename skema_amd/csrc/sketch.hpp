@@ -82,6 +82,14 @@ struct Map {
   int64_t lcols() const { return transposed ? nrow : ncol; }
   const Csr& logical_csr() const { return transposed ? csr_t : csr; }
   const Csr& logical_csr_t() const { return transposed ? csr : csr_t; }
+  // ELL view of the logical transpose when every logical column holds the
+  // same number ell_zeta (<= 16) of entries of one magnitude (sparse-sign
+  // maps): ell[c * ell_zeta + f] = row | sign << 31, value +-ell_mag.  One
+  // 4*zeta-byte load per column instead of a row-pointer pair and two
+  // scattered index/value reads.  ell_zeta == 0 when absent.
+  DevBuf ell;
+  int ell_zeta = 0;
+  double ell_mag = 0.0;
 };
 
 struct Sketch {
