@@ -113,7 +113,7 @@ def flops_per_row(cfg):
     return 2.0 * n * (2 * k + s) + 2.0 * s * s
 
 
-def run_form(S, ctx, cfg, A, maps, form, steps, warmup, torch):
+def run_form(S, ctx, cfg, A, maps, form, steps, warmup, torch, error=True):
     p = S.AlgParams(matrix_m=cfg["m"], matrix_n=cfg["n"], rank=cfg["rank"], sketch_range=cfg["k"],
                     sketch_core=cfg["s"], window=cfg["window"], precision=form)
     sk = S.SketchySVD(p, maps=maps, ctx=ctx)
@@ -134,9 +134,11 @@ def run_form(S, ctx, cfg, A, maps, form, steps, warmup, torch):
     stats = {k: ctx.kernel_stats(k) for k in ("sketch_left", "sketch_right", "sketch_core")}
     ms_per_step = (t1 - t0) * 1e3 / steps
     # Rank-r quality: ||A - U S V^T||_F / ||A||_F by the streamed residual pass.
-    U, Sv, V = sk.low_rank_approx(device=True)
-    e2, a2 = S.residual_fro(A, U, Sv, V, window=cfg["window"], ctx=ctx)
-    err = math.sqrt(e2 / a2)
+    err = None
+    if error:
+        U, Sv, V = sk.low_rank_approx(device=True)
+        e2, a2 = S.residual_fro(A, U, Sv, V, window=cfg["window"], ctx=ctx)
+        err = math.sqrt(e2 / a2)
     sk.close()
     return dict(form=form, ms_per_step=ms_per_step, rows_per_s=cfg["m"] / (ms_per_step * 1e-3),
                 kernel_avg_ms=(tot_ms / cnt if cnt else None), kernel_launches=cnt, rel_fro_err=err,
@@ -340,7 +342,9 @@ def bench_rbf(args, S, torch):
 
 
 def bench_adjoint(args, S, torch):
-    """Forward loss and the hand-written adjoint (dL/dX, dL/dgamma), FP64."""
+    """Forward loss and the hand-written adjoint (dL/dX, dL/dgamma) in the FP64
+    form and the BF16 form (BF16 Gram, sketch and tile products, FP32
+    accumulation); the BF16 gradient's error is stated against FP64's."""
     cfg = WORKLOADS[args.workload]
     X = make_points(cfg)
     Xd = S.colmajor(torch.as_tensor(X, device="cuda"))
@@ -350,43 +354,68 @@ def bench_adjoint(args, S, torch):
     om = S.GaussDimRedux(k, n, seed=0, ctx=ctx)
     ph = S.GaussDimRedux(s, n, seed=1, ctx=ctx)
     ps = S.GaussDimRedux(s, n, seed=2, ctx=ctx)
-
-    def run(grad, steps):
-        out = None
-        for _ in range(steps):
-            out = S.rbf_sketch_loss_grad(Xd, gamma, om, ph, ps, window=cfg["window"], grad=grad, ctx=ctx)
-        return out
-
-    run(True, args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run(False, args.steps)
-    torch.cuda.synchronize()
-    t_fwd = (time.perf_counter() - t0) / args.steps
-    ctx.reset_stats()
-    ctx.set_profiling(True)
-    t0 = time.perf_counter()
-    loss, dX, dg = run(True, args.steps)
-    torch.cuda.synchronize()
-    t_all = (time.perf_counter() - t0) / args.steps
-    ctx.set_profiling(False)
-    cnt, tot = ctx.kernel_stats("rbf_adjoint")
-    adj_ms = tot / cnt
+    forms = [f for f in args.forms.split(",") if f in ("fp64", "bf16")] or ["fp64", "bf16"]
     h = k + s
     # tile pass: (F_I R_J^T) with inner 2h, the Gram over d, and H X_J over d
     flops = float(n) * n * (4.0 * h + 4.0 * d)
-    achieved = flops / (adj_ms * 1e-3) / 1e12
+    res = {}
+    for form in forms:
+        def run(grad, steps):
+            out = None
+            for _ in range(steps):
+                out = S.rbf_sketch_loss_grad(Xd, gamma, om, ph, ps, window=cfg["window"], grad=grad, ctx=ctx,
+                                             precision=form)
+            return out
+
+        print(f"[bench] adjoint form {form}", file=sys.stderr, flush=True)
+        run(True, args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(False, args.steps)
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / args.steps
+        ctx.reset_stats()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        loss, dX, dg = run(True, args.steps)
+        torch.cuda.synchronize()
+        t_all = (time.perf_counter() - t0) / args.steps
+        ctx.set_profiling(False)
+        cnt, tot = ctx.kernel_stats("rbf_adjoint")
+        fc, ft = ctx.kernel_stats("rbf_forward")
+        adj_ms = tot / cnt
+        peak = PEAKS_TFLOPS[form]
+        achieved = flops / (adj_ms * 1e-3) / 1e12
+        res[form] = dict(ms=t_all * 1e3, forward_ms=t_fwd * 1e3, forward_kernel_ms=(ft / fc if fc else None),
+                         adjoint_ms=adj_ms, achieved=achieved, peak=peak, loss=loss, dgamma=dg,
+                         dX=dX.clone())
+        print(f"[bench] {form}: {t_all * 1e3:.2f} ms/step (tile pass {adj_ms:.2f} ms, {achieved:.1f} TFLOP/s)",
+              file=sys.stderr, flush=True)
+    best = min(res, key=lambda f: res[f]["ms"])
+    b = res[best]
+    forms_out = {}
+    for f, r in res.items():
+        e = {"ms_per_step": r["ms"], "forward_ms": r["forward_ms"], "forward_kernels_ms": r["forward_kernel_ms"],
+             "adjoint_tile_pass_ms": r["adjoint_ms"], "achieved_tflops": r["achieved"], "loss": r["loss"],
+             "dgamma": r["dgamma"]}
+        if "fp64" in res and f != "fp64":
+            ref = res["fp64"]
+            e["speedup_vs_fp64"] = ref["ms"] / r["ms"]
+            e["loss_rel_err_vs_fp64"] = abs(r["loss"] - ref["loss"]) / abs(ref["loss"])
+            e["dX_rel_err_vs_fp64"] = float(torch.linalg.norm(r["dX"] - ref["dX"]) / torch.linalg.norm(ref["dX"]))
+            e["dgamma_rel_err_vs_fp64"] = abs(r["dgamma"] - ref["dgamma"]) / abs(ref["dgamma"])
+        forms_out[f] = e
     out = {
-        "metric": METRIC, "value": n / t_all, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": t_all * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generators; no dataset)",
+        "metric": METRIC, "value": n / (b["ms"] * 1e-3), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": b["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": DTYPE[best], "data": "synthetic (seeded generators; no dataset)",
         "config": {"workload": f"{args.workload}: loss + adjoint of the fused RBF sketch, {n} points in d={d}, "
                                f"gamma=0.5/d, k={k} s={s}, forward windows {cfg['window']}",
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": "rbf_adjoint", "achieved": achieved, "peak": 78.6,
-                     "unit": "TFLOP/s", "frac": achieved / 78.6, "traffic": None,
-                     "flops_per_launch": flops, "avg_launch_ms": adj_ms},
-        "forward_ms": t_fwd * 1e3, "loss": loss, "dgamma": dg, "cpu_baseline": None,
+                   "form": best, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": "rbf_adjoint", "achieved": b["achieved"], "peak": b["peak"],
+                     "unit": "TFLOP/s", "frac": b["achieved"] / b["peak"], "traffic": None,
+                     "flops_per_launch": flops, "avg_launch_ms": b["adjoint_ms"]},
+        "forms": forms_out, "cpu_baseline": None,
     }
     print(json.dumps(out))
 
@@ -403,6 +432,9 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
+    ap.add_argument("--no-error", action="store_true",
+                    help="time the update only: no finalize, residual or optimal-tail passes (counter runs); "
+                         "the forms are then ranked by speed alone")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the sketch path is built and measured on a single MI355X; "
@@ -440,14 +472,15 @@ def main():
     for form in args.forms.split(","):
         try:
             print(f"[bench] timing form {form}", file=sys.stderr, flush=True)
-            results[form] = run_form(S, ctx, cfg, A, maps, form, args.steps, args.warmup, torch)
-            print(f"[bench] {form}: {results[form]['ms_per_step']:.2f} ms/step, err {results[form]['rel_fro_err']:.6e}",
+            results[form] = run_form(S, ctx, cfg, A, maps, form, args.steps, args.warmup, torch,
+                                     error=not args.no_error)
+            print(f"[bench] {form}: {results[form]['ms_per_step']:.2f} ms/step, err {results[form]['rel_fro_err']}",
                   file=sys.stderr, flush=True)
         except S.SkemaError as e:
             if e.code != 7:
                 raise
             print(f"[bench] form {form} not available: {e}", file=sys.stderr, flush=True)
-    if "fp64" in results:
+    if "fp64" in results and not args.no_error:
         base = results["fp64"]
         best = base
         for form, r in results.items():
@@ -456,8 +489,10 @@ def main():
     else:  # no FP64 reference run (profiling a single form): report the fastest form
         best = max(results.values(), key=lambda r: r["rows_per_s"])
         base = best
-    print("[bench] optimal rank-r tail (subspace iteration)", file=sys.stderr, flush=True)
-    tail = optimal_tail(A, cfg["rank"], torch)
+    tail = None
+    if not args.no_error:
+        print("[bench] optimal rank-r tail (subspace iteration)", file=sys.stderr, flush=True)
+        tail = optimal_tail(A, cfg["rank"], torch)
     name, flops = dominant_kernel(cfg)
     form = best["form"]
     mfma_flops = flops * MFMA_PER_FLOP[form]
@@ -494,8 +529,9 @@ def main():
         },
         "cpu_baseline": cpu,
         "forms": {f: {"rows_per_s": r["rows_per_s"], "ms_per_step": r["ms_per_step"],
-                      "rel_fro_err": r["rel_fro_err"], "err_vs_fp64": r["rel_fro_err"] / base["rel_fro_err"],
-                      "err_vs_optimal": r["rel_fro_err"] / tail if tail > 0 else None,
+                      "rel_fro_err": r["rel_fro_err"],
+                      "err_vs_fp64": (r["rel_fro_err"] / base["rel_fro_err"]) if r["rel_fro_err"] else None,
+                      "err_vs_optimal": (r["rel_fro_err"] / tail) if (tail and r["rel_fro_err"]) else None,
                       "speedup_vs_fp64": r["rows_per_s"] / base["rows_per_s"], "kernel_ms": r["kernel_ms"]}
                   for f, r in results.items()},
         "flops_per_row": flops_per_row(cfg),

@@ -20,12 +20,19 @@
 //   fp32   : v_mfma_f32_16x16x4_f32 on FP32-rounded operands
 //   bf16   : v_mfma_f32_16x16x32_bf16, one product
 //   bf16x3 : same instruction, x = hi + lo (both bf16), a*b ~ hi*hi + hi*lo + lo*hi
-//   fp8    : v_mfma_scale_f32_16x16x128_f8f6f4 with unit block scales on e4m3
-//            operands.  Every row of each operand carries one power-of-two
-//            scale over its whole reduction range (per row of a map and of
-//            the window, per column of the window), chosen from that row's
-//            max |x| so it lands in [128, 256) before rounding; the FP32
-//            accumulator is multiplied by the two scales in the FP64 epilogue.
+//   fp8    : v_mfma_scale_f32_16x16x128_f8f6f4 on e4m3 operands.  The maps
+//            carry one power-of-two scale per row over the whole reduction
+//            (applied in the FP64 epilogue).  The window is quantised in one
+//            read with an E8M0 scale per 32-element block of each column (L)
+//            and each row (R), fed to the instruction's own block-scale
+//            operands; windows that start off a 16-row boundary use one
+//            scale per row and per column instead (two reads, epilogue).
+//            Each scale is chosen from the block's (row's) max |x| so it
+//            lands in [128, 256) before rounding.
+//
+// Products on 16-byte-aligned operands run on the wide NT kernels (k_ntx:
+// A streamed to registers, B through LDS); the others on the 2 x 2-wave
+// kernels (k_nt_*).
 #include <cmath>
 #include <type_traits>
 
@@ -180,6 +187,87 @@ __global__ __launch_bounds__(kThreads) void k_quant_fp8(const double* A, int64_t
   }
 }
 
+// Window -> e4m3 copies with E8M0 block scales in one read of A_w: L
+// (column-major) with column j cut into 32-row blocks and R (row-major) with
+// row i cut into 32-column blocks; each block is scaled by 2^-e of its own
+// max |x| (max * 2^-e in [128, 256)) and its scale byte 127 + e goes to row
+// j of Ls (i of Rs) at position (b % 4) ld/4 + b / 4 for block b (the order
+// the NT kernels read: four consecutive K steps of one lane group in one
+// dword).  Tile 128 rows x 64 columns; each thread then owns one (column,
+// row block) of L and one (row, column block) of R, so no maximum crosses a
+// block.
+constexpr int kMxR = 128, kMxC = 64;
+__device__ __forceinline__ void quant32_store(const float (&v)[32], int valid, uint8_t* dst, uint8_t* sdst) {
+  float m = 0.f;
+#pragma unroll
+  for (int q = 0; q < 32; ++q) m = fmaxf(m, fabsf(v[q]));
+  const int e = block_exponent(m);
+  const float inv = ldexpf(1.f, -e);
+  uint32_t w[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) w[c] = pack4_e4m3(v[4 * c] * inv, v[4 * c + 1] * inv, v[4 * c + 2] * inv, v[4 * c + 3] * inv);
+  if (valid == 32) {
+    reinterpret_cast<i4v*>(dst)[0] = i4v{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+    reinterpret_cast<i4v*>(dst)[1] = i4v{(int)w[4], (int)w[5], (int)w[6], (int)w[7]};
+  } else {
+    for (int q = 0; q < valid; ++q) dst[q] = (uint8_t)(w[q >> 2] >> (8 * (q & 3)));
+  }
+  *sdst = (uint8_t)(127 + e);
+}
+__global__ __launch_bounds__(kThreads) void k_quant_fp8_mx(const double* A, int64_t lda, int64_t rows, int64_t n,
+                                                           uint8_t* Lq, int64_t ldl, uint8_t* Ls, int64_t ldls,
+                                                           uint8_t* Rq, int64_t ldr, uint8_t* Rs, int64_t ldrs) {
+  __shared__ __attribute__((aligned(16))) float t[kMxC][kMxR + 4];  // t[j][i]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t i0 = (int64_t)blockIdx.x * kMxR, j0 = (int64_t)blockIdx.y * kMxC;
+  // a wave reads one column's 128 rows per instruction (two rows per lane)
+  const int64_t gi = i0 + 2 * lane;
+  const bool vec = ((lda & 1) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && gi + 1 < rows;
+#pragma unroll 4
+  for (int e = 0; e < kMxC / 4; ++e) {
+    const int j = wave + 4 * e;
+    const int64_t gj = j0 + j;
+    double2 v = {0.0, 0.0};
+    if (gj < n) {
+      const double* src = A + gi + gj * lda;
+      if (vec) v = *reinterpret_cast<const double2*>(src);
+      else {
+        if (gi < rows) v.x = src[0];
+        if (gi + 1 < rows) v.y = src[1];
+      }
+    }
+    *reinterpret_cast<float2*>(&t[j][2 * lane]) = float2{(float)v.x, (float)v.y};
+  }
+  __syncthreads();
+  {  // L: column j, rows 32 rb .. 32 rb + 31 of the tile (4 lanes = one 128-byte run)
+    const int j = tid >> 2, rb = tid & 3;
+    const int64_t gj = j0 + j, gr = i0 + 32 * rb;
+    if (gj < n && gr < rows) {
+      float v[32];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float4 f = *reinterpret_cast<const float4*>(&t[j][32 * rb + 4 * c]);
+        v[4 * c] = f.x; v[4 * c + 1] = f.y; v[4 * c + 2] = f.z; v[4 * c + 3] = f.w;
+      }
+      const int valid = (int)(rows - gr < 32 ? rows - gr : 32);
+      const int64_t b = gr / 32;
+      quant32_store(v, valid, Lq + gj * ldl + gr, Ls + gj * ldls + (b & 3) * (ldls / 4) + (b >> 2));
+    }
+  }
+  {  // R: row i, columns 32 cb .. 32 cb + 31 of the tile
+    const int i = tid >> 1, cb = tid & 1;
+    const int64_t gr = i0 + i, gc = j0 + 32 * cb;
+    if (gr < rows && gc < n) {
+      float v[32];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) v[q] = t[32 * cb + q][i];
+      const int valid = (int)(n - gc < 32 ? n - gc : 32);
+      const int64_t b = gc / 32;
+      quant32_store(v, valid, Rq + gr * ldr + gc, Rs + gr * ldrs + (b & 3) * (ldrs / 4) + (b >> 2));
+    }
+  }
+}
+
 // Map -> row-major reduced copy (rows x cols logical, column-major source).
 template <class T>
 __global__ void k_convert_map(const double* src, int64_t ld, int64_t rows, int64_t cols, T* hi, T* lo, int64_t ldo) {
@@ -241,8 +329,16 @@ struct NtArgs {
   void* gl;
   int64_t ldg;
   int tiles_m;
+  int n_fast;  // wide kernels: tile order (see k_ntx)
   int64_t k_per_split;
   float* part;  // FP32 partials [split][M x N] when K is split
+  // fp8 with E8M0 block scales (wide kernels only): byte (row r, 32-element
+  // K block b) at asc[r * ldas + b] for A, bsc[...] likewise for B.  An
+  // operand with block scales has no row scale in the epilogue.
+  const uint8_t* asc;
+  int64_t ldas;
+  const uint8_t* bsc;
+  int64_t ldbs;
 };
 
 // Rows i0 .. i0+3 of column j (the four accumulator values of one lane in
@@ -327,26 +423,26 @@ __device__ __forceinline__ void store_nt4(const NtArgs& g, int64_t i0, int64_t j
 // each 16x16 tile, i.e. four consecutive rows of a column-major output.  fp8
 // (SCALED) multiplies by the two operands' row scales.  With a K split the
 // FP32 partial goes to g.part[z] for k_nt_reduce.
-template <int TM, int TN, int BM, int BN, bool SCALED>
-__device__ __forceinline__ void nt_epilogue(const NtArgs& g, const f4v (&acc)[TM][TN], int64_t m0, int64_t n0, int wm,
-                                            int wn, int lane) {
+template <int TM, int TN, bool SA, bool SB>
+__device__ __forceinline__ void nt_epilogue_at(const NtArgs& g, const f4v (&acc)[TM][TN], int64_t ibase,
+                                               int64_t jbase, int lane) {
   const int l16 = lane & 15, lq = lane >> 4;
   double sb[TN];
 #pragma unroll
   for (int y = 0; y < TN; ++y) {
-    const int64_t j = n0 + wn * (BN / 2) + y * 16 + l16;
-    sb[y] = (SCALED && j < g.N) ? ldexp(1.0, block_exponent(g.amax_b[j])) : 1.0;
+    const int64_t j = jbase + y * 16 + l16;
+    sb[y] = (SB && j < g.N) ? ldexp(1.0, block_exponent(g.amax_b[j])) : 1.0;
   }
 #pragma unroll
   for (int x = 0; x < TM; ++x) {
-    const int64_t i0 = m0 + wm * (BM / 2) + x * 16 + 4 * lq;
+    const int64_t i0 = ibase + x * 16 + 4 * lq;
     if (i0 >= g.M) continue;
     double sa[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) sa[v] = (SCALED && i0 + v < g.M) ? ldexp(1.0, block_exponent(g.amax_a[i0 + v])) : 1.0;
+    for (int v = 0; v < 4; ++v) sa[v] = (SA && i0 + v < g.M) ? ldexp(1.0, block_exponent(g.amax_a[i0 + v])) : 1.0;
 #pragma unroll
     for (int y = 0; y < TN; ++y) {
-      const int64_t j = n0 + wn * (BN / 2) + y * 16 + l16;
+      const int64_t j = jbase + y * 16 + l16;
       if (j >= g.N) continue;
       double val[4];
 #pragma unroll
@@ -361,6 +457,11 @@ __device__ __forceinline__ void nt_epilogue(const NtArgs& g, const f4v (&acc)[TM
       }
     }
   }
+}
+template <int TM, int TN, int BM, int BN, bool SCALED>
+__device__ __forceinline__ void nt_epilogue(const NtArgs& g, const f4v (&acc)[TM][TN], int64_t m0, int64_t n0, int wm,
+                                            int wn, int lane) {
+  nt_epilogue_at<TM, TN, SCALED, SCALED>(g, acc, m0 + wm * (BM / 2), n0 + wn * (BN / 2), lane);
 }
 
 // ---- NT kernels: C(i, j) = sum over this block's K range (blockIdx.z of the
@@ -649,6 +750,212 @@ __global__ __launch_bounds__(kThreads) void k_nt_fp8(NtArgs g) {
   nt_epilogue<TM, TN, BM, BN, true>(g, acc, m0, n0, wm, wn, lane);
 }
 
+// ---- Wide NT kernels (fp8, bf16): the four waves are stacked along M and
+// each streams its own rows of A from global memory straight into MFMA
+// registers, one K step ahead; only the B tile goes through LDS (double
+// buffered, one barrier per step), so LDS carries B once per wave instead of
+// A and B.  A K step is 128 bytes of every row (128 e4m3 or 64 bf16).  Lane
+// l (group q = l/16) holds bytes [16q, 16q+16) and [64+16q, 64+16q+16) of
+// row l%16 of a tile; B is read from LDS with the same assignment, so every
+// MFMA contracts matching K indices (the order of a sum over K is free).
+// Requires 16-byte aligned rows (pointers, leading dimensions, K offset).
+// This K assignment is the f8f6f4 instruction's own: VGPRs 0-3 of lane l
+// carry K [16q, 16q+16) and VGPRs 4-7 K [64+16q, 64+16q+16) of the 128, and
+// the E8M0 scale of (row l%16, K block [32b, 32b+32)) is read from lane
+// 16b + l%16 (both checked with single-lane probes on gfx950), so fp8
+// operands may carry one scale per row and 32-element K block (ASC, BSC).
+enum { kNxFp8 = 0, kNxBf16 = 1 };
+
+__device__ __forceinline__ i4v ld16_masked(const uint8_t* p, int64_t rem) {
+  if (rem >= 16) return *reinterpret_cast<const i4v*>(p);
+  uint8_t b[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) b[q] = (q < rem) ? p[q] : (uint8_t)0;
+  return *reinterpret_cast<const i4v*>(b);
+}
+
+template <int FORM, int TM, int TN, bool ASC, bool BSC>
+__global__ __launch_bounds__(kThreads) void k_ntx(NtArgs g) {
+  constexpr int BM = 64 * TM, BN = 16 * TN, SB = 128, LDB = SB + 16;
+  constexpr int CB = BN * (SB / 16) / kThreads;  // 16-byte chunks of B per thread and step
+  static_assert(CB >= 1 && BN * (SB / 16) % kThreads == 0, "B tile must split evenly over the block");
+  constexpr int64_t ES = FORM == kNxFp8 ? 1 : 2;
+  __shared__ __attribute__((aligned(16))) uint8_t sB[2][BN][LDB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, q = lane >> 4;
+  // Tiles of one XCD are consecutive in t; n_fast walks all N tiles of an M
+  // tile before the next, so an XCD shares its A rows (the larger operand)
+  // in its own L2 and only B is read by every XCD.
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int tiles_n = (int)((g.N + BN - 1) / BN);
+  const int64_t m0 = (int64_t)(g.n_fast ? t / tiles_n : t % g.tiles_m) * BM;
+  const int64_t n0 = (int64_t)(g.n_fast ? t % tiles_n : t / g.tiles_m) * BN;
+  // byte range of K for this split
+  const int64_t kb0 = (int64_t)blockIdx.z * g.k_per_split * ES;
+  const int64_t kb1 = (g.K < ((int64_t)blockIdx.z + 1) * g.k_per_split ? g.K : ((int64_t)blockIdx.z + 1) * g.k_per_split) * ES;
+  const int nsteps = (int)((kb1 - kb0 + SB - 1) / SB);
+
+  // A rows of this lane (clamped into the matrix: rows past M are computed
+  // and never stored)
+  const uint8_t* pa[TM];
+#pragma unroll
+  for (int x = 0; x < TM; ++x) {
+    int64_t row = m0 + wave * (16 * TM) + x * 16 + l16;
+    row = row < g.M ? row : g.M - 1;
+    pa[x] = static_cast<const uint8_t*>(g.ah) + row * g.lda * ES + g.koff_a * ES + kb0 + 16 * q;
+  }
+  const uint8_t* pb[CB];
+  int brow[CB], bch[CB];
+#pragma unroll
+  for (int e = 0; e < CB; ++e) {
+    const int c = tid + e * kThreads;
+    brow[e] = c / (SB / 16);
+    bch[e] = (c % (SB / 16)) * 16;
+    int64_t row = n0 + brow[e];
+    row = row < g.N ? row : g.N - 1;
+    pb[e] = static_cast<const uint8_t*>(g.bh) + row * g.ldb * ES + kb0 + bch[e];
+  }
+  // Block scales (E8M0).  Lane group q needs K block 4 st + q of step st;
+  // the scale rows are stored grouped by q (block b at (b % 4) * ldq + b / 4,
+  // ldq = ld / 4), so four consecutive steps of a lane are one dword.
+  const uint32_t* psa[TM];
+  const uint32_t* psb[TN];
+  const int64_t st0 = kb0 / SB;  // first step of this split (a multiple of 4)
+  if (ASC) {
+#pragma unroll
+    for (int x = 0; x < TM; ++x) {
+      int64_t row = m0 + wave * (16 * TM) + x * 16 + l16;
+      row = row < g.M ? row : g.M - 1;
+      psa[x] = reinterpret_cast<const uint32_t*>(g.asc + row * g.ldas + q * (g.ldas / 4) + st0);
+    }
+  }
+  if (BSC) {
+#pragma unroll
+    for (int y = 0; y < TN; ++y) {
+      int64_t row = n0 + y * 16 + l16;
+      row = row < g.N ? row : g.N - 1;
+      psb[y] = reinterpret_cast<const uint32_t*>(g.bsc + row * g.ldbs + q * (g.ldbs / 4) + st0);
+    }
+  }
+  uint32_t swa[TM], swb[TN];
+
+  i4v an[TM][2], rb[CB];
+  // the scale dword of step st (re-read on every step of its group of four:
+  // a cached load is cheaper than a branch in the pipelined loop)
+  auto load_scales = [&](int st) {
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+      if (ASC) swa[x] = psa[x][st >> 2];
+#pragma unroll
+    for (int y = 0; y < TN; ++y)
+      if (BSC) swb[y] = psb[y][st >> 2];
+  };
+  auto load_full = [&](int st) {
+    load_scales(st);
+    const int64_t off = (int64_t)st * SB;
+#pragma unroll
+    for (int x = 0; x < TM; ++x) {
+      an[x][0] = *reinterpret_cast<const i4v*>(pa[x] + off);
+      an[x][1] = *reinterpret_cast<const i4v*>(pa[x] + off + 64);
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) rb[e] = *reinterpret_cast<const i4v*>(pb[e] + off);
+  };
+  auto load_any = [&](int st) {
+    const int64_t off = (int64_t)st * SB;
+    const int64_t rem = kb1 - kb0 - off;  // valid bytes from the step start
+    if (rem >= SB) {
+      load_full(st);
+      return;
+    }
+    load_scales(st);
+#pragma unroll
+    for (int x = 0; x < TM; ++x) {
+      an[x][0] = ld16_masked(pa[x] + off, rem - 16 * q);
+      an[x][1] = ld16_masked(pa[x] + off + 64, rem - 64 - 16 * q);
+    }
+#pragma unroll
+    for (int e = 0; e < CB; ++e) rb[e] = ld16_masked(pb[e] + off, rem - bch[e]);
+  };
+  auto store_b = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < CB; ++e) *reinterpret_cast<i4v*>(&sB[buf][brow[e]][bch[e]]) = rb[e];
+  };
+
+  f4v acc[TM][TN];
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TN; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  // this step's operands leave the prefetch registers before the next
+  // step's loads are issued into them
+  i4v ac[TM][2];
+  int sac[TM], sbc[TN];
+  auto take = [&](int st) {
+    const int sh = 8 * (st & 3);
+#pragma unroll
+    for (int x = 0; x < TM; ++x) {
+      ac[x][0] = an[x][0];
+      ac[x][1] = an[x][1];
+      sac[x] = ASC ? (int)((swa[x] >> sh) & 0xffu) : 127;
+    }
+#pragma unroll
+    for (int y = 0; y < TN; ++y) sbc[y] = BSC ? (int)((swb[y] >> sh) & 0xffu) : 127;
+  };
+  auto compute = [&](int buf) {
+#pragma unroll
+    for (int y = 0; y < TN; ++y) {
+      const i4v b0 = *reinterpret_cast<const i4v*>(&sB[buf][y * 16 + l16][16 * q]);
+      const i4v b1 = *reinterpret_cast<const i4v*>(&sB[buf][y * 16 + l16][64 + 16 * q]);
+#pragma unroll
+      for (int x = 0; x < TM; ++x) {
+        const i4v a0 = ac[x][0], a1 = ac[x][1];
+        if (FORM == kNxFp8) {
+          const i8v av = i8v{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+          const i8v bv = i8v{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av, bv, acc[x][y], 0, 0, 0, sac[x], 0, sbc[y]);
+        } else {
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a0),
+                                                              __builtin_bit_cast(bf16x8, b0), acc[x][y], 0, 0, 0);
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a1),
+                                                              __builtin_bit_cast(bf16x8, b1), acc[x][y], 0, 0, 0);
+        }
+      }
+    }
+  };
+
+  const int nfull = (int)((kb1 - kb0) / SB);
+  if (nsteps > 0) {
+    load_any(0);
+    store_b(0);
+  }
+  __syncthreads();
+  int st = 0;
+  // steady state: the next step is a full one, so the body has no branch and
+  // the order below is the order issued: next loads, this step's LDS reads
+  // and MFMAs, then (sched_barrier: the compiler would otherwise hoist it)
+  // the wait for the loads and the LDS store of the next B tile.
+  for (; st + 1 < nfull; ++st) {
+    take(st);
+    load_full(st + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(st & 1);
+    __builtin_amdgcn_sched_barrier(0);
+    store_b((st + 1) & 1);
+    __syncthreads();
+  }
+  // last full step and the partial tail
+  for (; st < nsteps; ++st) {
+    take(st);
+    const bool more = st + 1 < nsteps;
+    if (more) load_any(st + 1);
+    compute(st & 1);
+    if (more) store_b((st + 1) & 1);
+    __syncthreads();
+  }
+  nt_epilogue_at<TM, TN, FORM == kNxFp8 && !ASC, FORM == kNxFp8 && !BSC>(g, acc, m0 + wave * (16 * TM), n0, lane);
+}
+
 // Sum of the FP32 partials of the K splits (in FP64, fixed order), then the
 // epilogue.
 __global__ void k_nt_reduce(NtArgs g, int splits) {
@@ -664,11 +971,45 @@ __global__ void k_nt_reduce(NtArgs g, int splits) {
 // products, otherwise a model over 64/128 x 64/128 tiles (a launch as rounds
 // of `res` co-resident blocks per CU over 256 CUs, each round costing
 // BM * BN * K/splits, plus FP32 partial traffic).
+// tm > 0 selects the wide kernel k_ntx<form, tm, tn, ...> (BM = 64 tm,
+// BN = 16 tn).
 struct NtPlan {
   int bm, bn, splits;
+  int tm = 0, tn = 0;
 };
+NtPlan plan_ntx(int prec, int64_t M, int64_t N, int64_t K) {
+  // wide-kernel tiles with the blocks per CU their registers admit
+  struct C { int tm, tn, res; };
+  const C cands[] = {{4, 8, 1}, {4, 4, 2}, {2, 8, 2}, {2, 4, 3}};
+  const int kstep = prec == SKEMA_PREC_FP8 ? 128 : 64;
+  NtPlan best{0, 0, 1, 2, 4};
+  double best_cost = 1e300;
+  for (C c : cands) {
+    const int64_t bm = 64 * c.tm, bn = 16 * c.tn;
+    const int64_t tiles = cdiv(M, bm) * cdiv(N, bn);
+    for (int sp = 1; sp <= 8; sp *= 2) {
+      if (sp > 1 && K / sp < 8 * kstep) break;
+      const double rounds = (double)cdiv(tiles * sp, 256 * c.res);
+      const double cost = rounds * (double)(bm * bn) * (double)cdiv(K, sp) * c.res +
+                          (sp > 1 ? 8.0 * sp * (double)M * N : 0.0);
+      if (cost < best_cost) { best_cost = cost; best = NtPlan{0, 0, sp, c.tm, c.tn}; }
+    }
+  }
+  return best;
+}
+
 NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   const int ks = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
+  if (prec == SKEMA_PREC_FP8 || prec == SKEMA_PREC_BF16) {
+    // wide kernels, measured per window product (w = 4096, n = 16384,
+    // k = 401, s = 803; right 0.25 / 0.32 ms, left 0.155 / 0.153 ms, core
+    // 0.047 ms for fp8 / bf16), otherwise the round model
+    if (M >= 2048 && N <= 2048 && K >= 8192) return NtPlan{0, 0, 2, 2, 8};
+    if (M <= 512 && N >= 4096 && K >= 2048)
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 4, 4} : NtPlan{0, 0, 1, 2, 8};
+    if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4};
+    return plan_ntx(prec, M, N, K);
+  }
   // Shapes of the window products, with the tile and split measured fastest
   // for each form at w = 4096, n = 16384, k = 401, s = 803 (one MI355X):
   //   right  [y | G] = A_w [Omega; Psi]^T : M = w, N = k+s, K = n (long)
@@ -713,7 +1054,32 @@ void launch_nt_tile(hipStream_t st, int prec, const NtArgs& a, dim3 grid) {
   else if (prec == SKEMA_PREC_BF16X3) hipLaunchKernelGGL((k_nt_bf16<BM, BN, true>), grid, dim3(kThreads), 0, st, a);
   else hipLaunchKernelGGL((k_nt_bf16<BM, BN, false>), grid, dim3(kThreads), 0, st, a);
 }
+template <int TM, int TN>
+void launch_ntx_tile(hipStream_t st, int prec, const NtArgs& a, dim3 grid) {
+  if (prec == SKEMA_PREC_FP8) {
+    if (a.asc) hipLaunchKernelGGL((k_ntx<kNxFp8, TM, TN, true, false>), grid, dim3(kThreads), 0, st, a);
+    else if (a.bsc) hipLaunchKernelGGL((k_ntx<kNxFp8, TM, TN, false, true>), grid, dim3(kThreads), 0, st, a);
+    else hipLaunchKernelGGL((k_ntx<kNxFp8, TM, TN, false, false>), grid, dim3(kThreads), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((k_ntx<kNxBf16, TM, TN, false, false>), grid, dim3(kThreads), 0, st, a);
+  }
+}
+// The wide kernels need 16-byte aligned rows of both operands.
+bool ntx_eligible(int prec, const NtArgs& a) {
+  if (prec != SKEMA_PREC_FP8 && prec != SKEMA_PREC_BF16) return false;
+  const int64_t es = prec == SKEMA_PREC_FP8 ? 1 : 2;
+  return ((uintptr_t)a.ah % 16) == 0 && ((uintptr_t)a.bh % 16) == 0 && (a.lda * es) % 16 == 0 &&
+         (a.ldb * es) % 16 == 0 && (a.koff_a * es) % 16 == 0;
+}
 void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid) {
+  if (pl.tm > 0) {
+    if (pl.tm == 4 && pl.tn == 8) launch_ntx_tile<4, 8>(st, prec, a, grid);
+    else if (pl.tm == 4 && pl.tn == 4) launch_ntx_tile<4, 4>(st, prec, a, grid);
+    else if (pl.tm == 2 && pl.tn == 8) launch_ntx_tile<2, 8>(st, prec, a, grid);
+    else if (pl.tm == 2 && pl.tn == 4) launch_ntx_tile<2, 4>(st, prec, a, grid);
+    else SKEMA_REQUIRE(false, kErrUnsupported, "no wide NT kernel for this tile");
+    return;
+  }
   if (pl.bm == 128 && pl.bn == 128) launch_nt_tile<128, 128>(st, prec, a, grid);
   else if (pl.bm == 128) launch_nt_tile<128, 64>(st, prec, a, grid);
   else if (pl.bn == 128) launch_nt_tile<64, 128>(st, prec, a, grid);
@@ -721,10 +1087,18 @@ void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
 }
 
 void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
-  const NtPlan pl = plan_nt(prec, a.M, a.N, a.K);
+  NtPlan pl = plan_nt(prec, a.M, a.N, a.K);
+  SKEMA_REQUIRE(!(a.asc && a.bsc), kErrUnsupported, "block scales on both NT operands");
+  if ((a.asc || a.bsc) && pl.tm == 0) pl = plan_ntx(prec, a.M, a.N, a.K);
+  SKEMA_REQUIRE(!(a.asc || a.bsc) || (ntx_eligible(prec, a) && (!a.asc || a.koff_a == 0)), kErrUnsupported,
+                "block-scaled fp8 operands need 16-byte aligned rows (and no K offset on A)");
+  if (pl.tm > 0 && !ntx_eligible(prec, a)) pl = NtPlan{64, 64, pl.splits};
+  if (pl.tm > 0) { pl.bm = 64 * pl.tm; pl.bn = 16 * pl.tn; }
   const int kstep = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
   int splits = ws ? pl.splits : 1;
-  a.k_per_split = cdiv(cdiv(a.K, splits), kstep) * kstep;
+  // block-scaled operands: splits start on a 4-step boundary (one scale dword)
+  const int64_t kgrain = (a.asc || a.bsc) ? 4 * kstep : kstep;
+  a.k_per_split = cdiv(cdiv(a.K, splits), kgrain) * kgrain;
   splits = (int)cdiv(a.K, a.k_per_split);
   a.part = nullptr;
   if (splits > 1) {
@@ -732,6 +1106,8 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
     a.part = ws->as<float>();
   }
   a.tiles_m = (int)cdiv(a.M, pl.bm);
+  // wide kernels: keep the larger operand's rows within an XCD
+  a.n_fast = (a.M >= a.N) ? 1 : 0;
   const dim3 grid((unsigned)(a.tiles_m * cdiv(a.N, pl.bn)), 1, (unsigned)splits);
   launch_nt(st, prec, a, pl, grid);
   check_launch("k_nt");
@@ -820,11 +1196,20 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
   const int core_prec = f32 ? SKEMA_PREC_FP32 : (split ? SKEMA_PREC_BF16X3 : SKEMA_PREC_BF16);
   const int gtype = f32 ? 3 : (split ? 2 : 1);
   const size_t gsz = f32 ? 4 : 2;
+  // fp8 windows that start on a 16-row boundary (every window of a multiple-
+  // of-16 window size) take the block-scaled form: one read of A_w, E8M0
+  // scales per 32-element block of L's columns and R's rows; others take the
+  // two-pass form with one scale per row and per column.
+  const bool mx = fp8 && (row0 % 16) == 0;
+  // scale rows: 4 groups of ceil(steps / 4) * 4 bytes (steps of 128 elements)
+  const int64_t ldls = 4 * align_up((size_t)cdiv(rows, 128), 4), ldrs = 4 * align_up((size_t)cdiv(n, 128), 4);
   const size_t le = (size_t)n * ldl, re = (size_t)rows * ldr, ge = (size_t)s * ldl;
   const size_t o_l = 0, o_r = align_up(le * esz * planes, 256), o_rmax = o_r + align_up(re * esz * planes, 256),
                o_cmax = o_rmax + (fp8 ? align_up((size_t)rows * 4, 256) : 0),
                o_g = o_cmax + (fp8 ? align_up((size_t)n * 4, 256) : 0),
-               total = o_g + align_up(ge * gsz * planes, 256);
+               o_ls = o_g + align_up(ge * gsz * planes, 256),
+               o_rs = o_ls + (mx ? align_up((size_t)n * ldls, 256) : 0),
+               total = o_rs + (mx ? align_up((size_t)rows * ldrs, 256) : 0);
   sk.win.reserve(total);
   char* base = sk.win.as<char>();
   float* rmax = (float*)(base + o_rmax);
@@ -832,7 +1217,14 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
   {
     Timed t(c, "window");
     dim3 grid((unsigned)cdiv(rows, kQ), (unsigned)cdiv(n, kQ));
-    if (fp8) {
+    if (mx) {
+      // unit scales (0x7f) wherever no block writes one: K padding past the window
+      fill_i32(st, (int*)(base + o_ls), (int64_t)(align_up((size_t)n * ldls, 256) / 4), 0x7f7f7f7f);
+      fill_i32(st, (int*)(base + o_rs), (int64_t)(align_up((size_t)rows * ldrs, 256) / 4), 0x7f7f7f7f);
+      dim3 gmx((unsigned)cdiv(rows, kMxR), (unsigned)cdiv(n, kMxC));
+      hipLaunchKernelGGL(k_quant_fp8_mx, gmx, dim3(kThreads), 0, st, H, ldh, rows, n, (uint8_t*)(base + o_l), ldl,
+                         (uint8_t*)(base + o_ls), ldls, (uint8_t*)(base + o_r), ldr, (uint8_t*)(base + o_rs), ldrs);
+    } else if (fp8) {
       fill_i32(st, (int*)rmax, rows, 0);
       fill_i32(st, (int*)cmax, n, 0);
       hipLaunchKernelGGL(k_absmax_fp8, grid, dim3(kThreads), 0, st, H, ldh, rows, n, (unsigned*)rmax,
@@ -864,7 +1256,8 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
     a.lda = sk.lp_ldups; a.koff_a = row0;
     a.bh = base + o_l;
     a.bl = split ? (const void*)((const __bf16*)(base + o_l) + le) : nullptr;
-    a.amax_b = fp8 ? cmax : nullptr;
+    a.amax_b = (fp8 && !mx) ? cmax : nullptr;
+    if (mx) { a.bsc = (const uint8_t*)(base + o_ls); a.ldbs = ldls; }
     a.ldb = ldl;
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
@@ -876,7 +1269,8 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
     a.M = rows; a.N = k + s; a.K = n;
     a.ah = base + o_r;
     a.al = split ? (const void*)((const __bf16*)(base + o_r) + re) : nullptr;
-    a.amax_a = fp8 ? rmax : nullptr;
+    a.amax_a = (fp8 && !mx) ? rmax : nullptr;
+    if (mx) { a.asc = (const uint8_t*)(base + o_rs); a.ldas = ldrs; }
     a.lda = ldr; a.koff_a = 0;
     a.bh = sk.lp_omps.p;
     a.bl = split ? (const void*)(sk.lp_omps.as<__bf16>() + oe) : nullptr;
@@ -901,6 +1295,17 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
     a.out.C2 = a.out.C1; a.out.ldc2 = s;
     gemm_nt(st, core_prec, a, &c->gws.buf);
   }
+}
+
+void gemm_nt_bf16(hipStream_t st, int64_t M, int64_t N, int64_t K, const __bf16* A, int64_t lda, const __bf16* B,
+                  int64_t ldb, double alpha, double* C, int64_t ldc, DevBuf* ws) {
+  NtArgs a{};
+  a.M = M; a.N = N; a.K = K;
+  a.ah = A; a.lda = lda; a.koff_a = 0;
+  a.bh = B; a.ldb = ldb;
+  a.out.C1 = C; a.out.ldc1 = ldc; a.out.alpha1 = alpha; a.out.beta1 = 0.0; a.out.nsplit = N;
+  a.out.C2 = C; a.out.ldc2 = ldc;
+  gemm_nt(st, SKEMA_PREC_BF16, a, ws);
 }
 
 void lowp_core_bf16(Sketch& sk, const __bf16* G, int64_t ldg, int64_t row0, int64_t rows) {

@@ -10,5 +10,9 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
 // Z <- nu Z + eta Phi[:, row0:row0+rows] G for a BF16 G (rows x s,
 // column-major, ld ldg) on the BF16 NT GEMM (bf16 form maps).
 void lowp_core_bf16(Sketch& sk, const __bf16* G, int64_t ldg, int64_t row0, int64_t rows);
+// C (M x N, column-major, FP64) = alpha * A B^T with A (M x K) and B (N x K)
+// BF16 row-major (K contiguous), FP32 accumulation.
+void gemm_nt_bf16(hipStream_t st, int64_t M, int64_t N, int64_t K, const __bf16* A, int64_t lda, const __bf16* B,
+                  int64_t ldb, double alpha, double* C, int64_t ldc, DevBuf* ws);
 
 }  // namespace skema

@@ -409,12 +409,60 @@ bool rbf_bf16_supported(const Sketch& sk) {
   return !sk.sparse_maps && sk.p.precision == SKEMA_PREC_BF16 && sk.p.nfeat >= 1 && sk.p.nfeat <= kRbfMaxFeat;
 }
 
+void rbf_points_bf16(hipStream_t st, const double* points, int64_t ldp, int64_t n, int64_t d, int dp, __bf16* P,
+                     float* nrm) {
+  hipLaunchKernelGGL(k_points_bf16, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, points, ldp, n, d, (int64_t)dp,
+                     P, nrm);
+  check_launch("k_points_bf16");
+}
+
+void rbf_right_bf16(Ctx* c, const __bf16* P, int dp, const float* nrm, float gamma, int64_t npts, int64_t row0,
+                    int64_t rows, const __bf16* OP, int64_t ldop, int64_t k, int64_t s, const GemmOut& yout,
+                    __bf16* G, int64_t ldg) {
+  hipStream_t st = c->stream;
+  RbfLpArgs a;
+  a.P = P; a.ldpb = dp; a.nrm = nrm; a.gamma = gamma; a.row0 = row0;
+  a.M = rows; a.N = k + s; a.K = npts;
+  a.OP = OP; a.ldop = ldop;
+  a.out = yout; a.out.nsplit = k;
+  a.G = G; a.ldg = ldg;
+  const int tn = pick(kTnChoices, cdiv(cdiv(a.N, 16), 8));
+  const int64_t col_blocks = cdiv(a.N, 128 * tn);
+  a.tiles_m = (int)cdiv(a.M, 64);
+  // one 512-thread block per CU: choose the split of the points that
+  // minimises rounds x points per split plus the partial traffic
+  const int64_t per_split = a.tiles_m * col_blocks;
+  int64_t splits = 1, best = INT64_MAX;
+  const int64_t max_splits = std::max<int64_t>(1, std::min<int64_t>(64, a.K / 1024));
+  for (int64_t sp = 1; sp <= max_splits; ++sp) {
+    const int64_t cost = cdiv(per_split * sp, 256) * cdiv(a.K, sp) + (sp > 1 ? sp * (a.M * a.N / 65536 + 1) : 0);
+    if (cost < best) { best = cost; splits = sp; }
+  }
+  a.k_per_split = cdiv(cdiv(a.K, splits), 32) * 32;
+  splits = cdiv(a.K, a.k_per_split);
+  a.part = nullptr;
+  if (splits > 1) {
+    c->gws.buf.reserve((size_t)splits * a.M * a.N * sizeof(float));
+    a.part = c->gws.buf.as<float>();
+  }
+  const dim3 grid((unsigned)(a.tiles_m * splits), (unsigned)col_blocks);
+  if (dp == 32) launch_right<32>(st, a, grid, tn);
+  else if (dp == 64) launch_right<64>(st, a, grid, tn);
+  else launch_right<128>(st, a, grid, tn);
+  check_launch("k_rbf_right_bf16");
+  if (splits > 1) {
+    const int blocks = (int)std::min<int64_t>(cdiv(a.M * a.N, 256), 8192);
+    hipLaunchKernelGGL(k_rbf_right_bf16_reduce, dim3(blocks), dim3(256), 0, st, a, (int)splits);
+    check_launch("k_rbf_right_bf16_reduce");
+  }
+}
+
 void update_rbf_window_bf16(Sketch& sk, const double* points, int64_t ldp, int64_t row0, int64_t rows) {
   Ctx* c = sk.ctx;
   hipStream_t st = c->stream;
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n, d = sk.p.nfeat;
   const double eta = sk.p.eta, nu = sk.p.nu;
-  const int dp = d <= 32 ? 32 : (d <= 64 ? 64 : 128);
+  const int dp = rbf_bf16_padded_features(d);
   const int64_t ldg = cdiv(rows, 128) * 128;
   const size_t o_p = 0, o_n = align_up((size_t)n * dp * 2, 256), o_g = o_n + align_up((size_t)n * 4, 256),
                total = o_g + align_up((size_t)s * ldg * 2, 256);
@@ -425,16 +473,12 @@ void update_rbf_window_bf16(Sketch& sk, const double* points, int64_t ldp, int64
   __bf16* G = (__bf16*)(base + o_g);
   {
     Timed t(c, "window");
-    hipLaunchKernelGGL(k_points_bf16, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, points, ldp, n, d,
-                       (int64_t)dp, P, nrm);
-    check_launch("k_points_bf16");
+    rbf_points_bf16(st, points, ldp, n, d, dp, P, nrm);
   }
-  RbfLpArgs base_args;
-  base_args.P = P; base_args.ldpb = dp; base_args.nrm = nrm; base_args.gamma = (float)sk.p.gamma;
-  base_args.row0 = row0;
   {
     Timed t(c, "rbf_left");
-    RbfLpArgs a = base_args;
+    RbfLpArgs a;
+    a.P = P; a.ldpb = dp; a.nrm = nrm; a.gamma = (float)sk.p.gamma; a.row0 = row0;
     a.M = k; a.N = n; a.K = rows;
     a.U = sk.lp_ups.as<__bf16>(); a.ldu = sk.lp_ldups;
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
@@ -448,40 +492,10 @@ void update_rbf_window_bf16(Sketch& sk, const double* points, int64_t ldp, int64
   }
   {
     Timed t(c, "rbf_right");
-    RbfLpArgs a = base_args;
-    a.M = rows; a.N = k + s; a.K = n;
-    a.OP = sk.lp_omps.as<__bf16>(); a.ldop = sk.lp_ldomps;
-    a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
-    a.G = G; a.ldg = ldg;
-    const int tn = pick(kTnChoices, cdiv(cdiv(a.N, 16), 8));
-    const int64_t col_blocks = cdiv(a.N, 128 * tn);
-    a.tiles_m = (int)cdiv(a.M, 64);
-    // one 512-thread block per CU: choose the split of the points that
-    // minimises rounds x points per split plus the partial traffic
-    const int64_t per_split = a.tiles_m * col_blocks;
-    int64_t splits = 1, best = INT64_MAX;
-    const int64_t max_splits = std::max<int64_t>(1, std::min<int64_t>(64, a.K / 1024));
-    for (int64_t sp = 1; sp <= max_splits; ++sp) {
-      const int64_t cost = cdiv(per_split * sp, 256) * cdiv(a.K, sp) + (sp > 1 ? sp * (a.M * a.N / 65536 + 1) : 0);
-      if (cost < best) { best = cost; splits = sp; }
-    }
-    a.k_per_split = cdiv(cdiv(a.K, splits), 32) * 32;
-    splits = cdiv(a.K, a.k_per_split);
-    a.part = nullptr;
-    if (splits > 1) {
-      c->gws.buf.reserve((size_t)splits * a.M * a.N * sizeof(float));
-      a.part = c->gws.buf.as<float>();
-    }
-    const dim3 grid((unsigned)(a.tiles_m * splits), (unsigned)col_blocks);
-    if (dp == 32) launch_right<32>(st, a, grid, tn);
-    else if (dp == 64) launch_right<64>(st, a, grid, tn);
-    else launch_right<128>(st, a, grid, tn);
-    check_launch("k_rbf_right_bf16");
-    if (splits > 1) {
-      const int blocks = (int)std::min<int64_t>(cdiv(a.M * a.N, 256), 8192);
-      hipLaunchKernelGGL(k_rbf_right_bf16_reduce, dim3(blocks), dim3(256), 0, st, a, (int)splits);
-      check_launch("k_rbf_right_bf16_reduce");
-    }
+    GemmOut y;
+    y.C1 = sk.Y.as<double>() + row0; y.ldc1 = m; y.alpha1 = eta; y.beta1 = nu;
+    rbf_right_bf16(c, P, dp, nrm, (float)sk.p.gamma, n, row0, rows, sk.lp_omps.as<__bf16>(), sk.lp_ldomps, k, s, y,
+                   G, ldg);
   }
   {
     Timed t(c, "sketch_core");

@@ -50,13 +50,16 @@ struct Timed {
       SKEMA_HIP(hipEventRecord(a, c->stream));
     }
   }
-  ~Timed() {
+  // Closes the interval now (else at scope exit).
+  void stop() {
     if (a) {
       hipEvent_t b = c->stats.get();
       (void)hipEventRecord(b, c->stream);
       c->stats.open[name].push_back({a, b});
+      a = nullptr;
     }
   }
+  ~Timed() { stop(); }
 };
 
 // CSR on the device.

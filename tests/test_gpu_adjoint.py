@@ -91,3 +91,28 @@ def test_adjoint_dense_reference_n4096(S):
     assert abs(loss - L_ref) <= 1e-10 * abs(L_ref)
     assert O.rel_err(dX.cpu().numpy(), dX_ref) <= 1e-9
     assert abs(dg - dg_ref) <= 1e-9 * abs(dg_ref)
+
+
+# BF16 form: the Gram, S = F R^T and (H X) on BF16 MFMAs with FP32
+# accumulation, K and H in FP32.  K carries ~0.3% relative error (BF16
+# Gram of the rounded points), S a BF16 rounding of F; dX = -2 gamma
+# (rowsum(H) x - H X) subtracts two nearly equal terms inside a cluster, so
+# its normwise error is several times that of K.
+BF16_LOSS_TOL, BF16_DX_TOL, BF16_DG_TOL = 2e-2, 1e-1, 5e-2
+
+
+@pytest.mark.parametrize("n,d,k,s,window", [(300, 7, 21, 43, 64), (4096, 64, 201, 403, 4096)])
+def test_adjoint_bf16_form_against_dense_reference(S, n, d, k, s, window):
+    print(f"case adjoint bf16 n={n} d={d}", flush=True)
+    X = _points(n, d, 9)
+    gamma = 0.5 / d
+    (Om, Ph, Ps), (om, ph, ps) = _maps(S, n, k, s, seed=11)
+    loss, dX, dg = S.sketchy.rbf_sketch_loss_grad(dev(X), gamma, om, ph, ps, window=window, precision="bf16")
+    L_ref, dX_ref, dg_ref = _dense_reference(X, gamma, Om, Ph, Ps)
+    e_l = abs(loss - L_ref) / abs(L_ref)
+    e_x = O.rel_err(dX.cpu().numpy(), dX_ref)
+    e_g = abs(dg - dg_ref) / abs(dg_ref)
+    print(f"bf16 adjoint n={n}: loss {e_l:.3e} dX {e_x:.3e} dgamma {e_g:.3e}", flush=True)
+    assert e_l <= BF16_LOSS_TOL, e_l
+    assert e_x <= BF16_DX_TOL, e_x
+    assert e_g <= BF16_DG_TOL, e_g

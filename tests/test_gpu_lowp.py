@@ -84,3 +84,23 @@ def test_lowp_window_offsets_unaligned(S):
         X, Y, Z = g.state()
         assert O.rel_err(X, osk.X) <= TOLS[form]
         assert O.rel_err(Y, osk.Y) <= TOLS[form]
+
+
+@pytest.mark.parametrize("form", ["fp8", "bf16"])
+def test_lowp_window_products_at_sketch_scale(S, form):
+    """k = 401, s = 803 over 4096-row windows (the sizes of the 65536 x 16384
+    workload on a narrower matrix): the large-tile NT kernels, their K loop
+    over many steps and the split-K partials all take part."""
+    print(f"case lowp sketch-scale {form}", flush=True)
+    m, n = 8192, 2048
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((m, 64)) @ rng.standard_normal((64, n)) + 0.1 * rng.standard_normal((m, n))
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=100, window=4096, sketch_nu=0.9, sketch_eta=1.1)
+    osk = O.SketchySVD(p)
+    osk.linear_update(A)
+    g = _run(S, A, p, osk, form)
+    X, Y, Z = g.state()
+    tol = TOLS[form]
+    assert O.rel_err(X, osk.X) <= tol, O.rel_err(X, osk.X)
+    assert O.rel_err(Y, osk.Y) <= tol, O.rel_err(Y, osk.Y)
+    assert O.rel_err(Z, osk.Z) <= 2 * tol, O.rel_err(Z, osk.Z)
