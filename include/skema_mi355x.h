@@ -238,6 +238,34 @@ int skema_residual_fro_dense(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, co
                              const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
                              int64_t window, double* err2, double* norm2);
 
+/* ---- residual norms of the final factors ---------------------------------- */
+/* rnorms[c] = ||[A^T u_c - s_c v_c ; A v_c - s_c u_c]||_2 for c < r: the
+ * reference's residuals(A, U, S, V, rank, algParams, window)
+ * (src/Skema_Residuals.hpp:160-217, windowed A V / A^T U at :60-121), called
+ * by SketchySVD::compute_residuals (src/Skema_SketchySVD.cpp:718-727).
+ * U (m x r, ldu), S (r), V (n x r, ldv) on the device; rnorms (r) on the host. */
+int skema_residuals_dense(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, const double* A, int64_t lda,
+                          const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
+                          double* rnorms);
+/* Same for a device CSR A (m x n; rowptr int64[m+1], col int32, val). */
+int skema_residuals_csr(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, const int64_t* rowptr, const int32_t* col,
+                        const double* val, const double* U, int64_t ldu, const double* S, const double* V,
+                        int64_t ldv, double* rnorms);
+/* Same for A = the n x n Gaussian RBF matrix of the device point set
+ * (n x d, ldp), generated in row windows by the fused generator (the
+ * GaussRBFWindow path of map_A_by_window, src/Skema_Residuals.hpp:60-121). */
+int skema_residuals_rbf(skema_ctx* ctx, const double* points, int64_t n, int64_t d, int64_t ldp, double gamma,
+                        int64_t r, const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
+                        int64_t window, double* rnorms);
+/* Eigenpair residuals rnorms[c] = ||A v_c - s_c v_c||_2 of a symmetric A
+ * (residuals(A, V, S, rank, ...), src/Skema_Residuals.hpp:123-158, used by
+ * SketchySPD). */
+int skema_residuals_sym_dense(skema_ctx* ctx, int64_t n, int64_t r, const double* A, int64_t lda, const double* V,
+                              int64_t ldv, const double* S, double* rnorms);
+int skema_residuals_sym_rbf(skema_ctx* ctx, const double* points, int64_t n, int64_t d, int64_t ldp, double gamma,
+                            int64_t r, const double* V, int64_t ldv, const double* S, int64_t window,
+                            double* rnorms);
+
 /* ---- adjoint of the fused RBF window + Gauss sketch ----------------------- */
 /* Loss L = ||K Omega^T||_F^2 + ||Phi K Psi^T||_F^2 with K = exp(-gamma D) over
  * the n points (n x d, ldp, device).  Omega (k x n), Phi (s x n), Psi (s x n)

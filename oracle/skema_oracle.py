@@ -16,6 +16,8 @@ What it restates (reference = jeremy-myers/skema, paths relative to its root):
   src/Skema_DimRedux_SparseSign.cpp:9-155, src/Skema_DimRedux.hpp:95-308).
 * ``Window::get`` and ``GaussRBFWindow::get`` / ``GaussRBF::compute``
   (src/Skema_Window.hpp:36-107, src/Skema_Kernel.cpp:5-81).
+* ``residuals`` / ``map_A_by_window`` (src/Skema_Residuals.hpp:17-217): the
+  per-triplet residual norms of the final factors.
 * ``linalg::qr`` / ``ls`` / ``svd`` (src/Skema_BlasLapack.hpp:71-97,168-243):
   the SAME LAPACK routines (dgeqrf+dorgqr with lwork=n, dgels, dgesvd 'S','S')
   called through ``scipy.linalg.lapack`` with the reference's arguments.
@@ -501,6 +503,60 @@ class SketchySVD:
         r = self.rank
         self.uvecs, self.svals, self.vvecs = QU[:, :r].copy(), S[:r].copy(), PV[:, :r].copy()
         return self.uvecs, self.svals, self.vvecs
+
+
+# --------------------------------------------------------------------------
+# Residual norms of the final factors (src/Skema_Residuals.hpp)
+# --------------------------------------------------------------------------
+def _row_windows(m: int, window: Optional[int]):
+    w = m if not window else window
+    for r0 in range(0, m, w):
+        yield r0, min(r0 + w, m)
+
+
+def map_A_by_window(A, U, V, window: Optional[int] = None, rbf_gamma: Optional[float] = None):
+    """(A V, A^T U) window by window (src/Skema_Residuals.hpp:60-121): each
+    window A_w = window->get(A, idx) contributes av = A_w V to the rows idx of
+    AV and atu = A_w^T U[idx, :] added into AtU.  With rbf_gamma, A is the
+    point set and A_w the Gaussian RBF rows of the window
+    (GaussRBFWindow::get, src/Skema_Window.hpp:90-107)."""
+    if rbf_gamma is not None:
+        X = np.asarray(A, dtype=np.float64)
+        m = n = X.shape[0]
+        win = GaussRBFWindow(rbf_gamma)
+    else:
+        m, n = A.shape
+        win = Window()
+    r = V.shape[1]
+    AV = np.zeros((m, r))
+    AtU = np.zeros((n, r))
+    for r0, r1 in _row_windows(m, window):
+        Aw = win.get(A if rbf_gamma is None else X, (r0, r1))
+        AtU += _dense_or_sparse_T_mul(Aw, U[r0:r1, :])
+        AV[r0:r1, :] = _dense(Aw @ V) if sp.issparse(Aw) else Aw @ V
+    return AV, AtU
+
+
+def _dense_or_sparse_T_mul(Aw, B):
+    out = Aw.T @ B
+    return np.asarray(out.todense()) if sp.issparse(out) else np.asarray(out)
+
+
+def residual_norms(A, U, S, V, window: Optional[int] = None, rbf_gamma: Optional[float] = None) -> np.ndarray:
+    """SketchySVD triplet residuals (src/Skema_Residuals.hpp:160-217):
+    rnorms[i] = || [A^T u_i - s_i v_i ; A v_i - s_i u_i] ||_2 (the stacked
+    vector of length n + m, ``rvec`` in the reference)."""
+    AV, AtU = map_A_by_window(A, U, V, window, rbf_gamma)
+    dAV = AV - U * S
+    dAtU = AtU - V * S
+    return np.sqrt((dAtU ** 2).sum(axis=0) + (dAV ** 2).sum(axis=0))
+
+
+def residual_norms_sym(A, V, S, window: Optional[int] = None, rbf_gamma: Optional[float] = None) -> np.ndarray:
+    """SketchySPD eigen-residuals (src/Skema_Residuals.hpp:123-158):
+    rnorms[i] = || A v_i - s_i v_i ||_2 with A V by windows (:17-56)."""
+    AV, _ = map_A_by_window(A, V, V, window, rbf_gamma)
+    return np.sqrt(((AV - V * S) ** 2).sum(axis=0))
 
 
 # --------------------------------------------------------------------------

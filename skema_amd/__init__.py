@@ -19,6 +19,9 @@ from .sketchy import (  # noqa: F401
     map_shapes,
     rbf_sketch_loss_grad,
     residual_fro,
+    residual_norms,
+    residual_norms_rbf,
+    residual_norms_sym,
 )
 
 __version__ = "1.0.0"

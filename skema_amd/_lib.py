@@ -107,6 +107,11 @@ SIGNATURES = {
     "skema_sketch_timings_json": (_i32, [_vp, C.c_char_p, C.c_size_t]),
     "skema_residual_fro_dense": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
                                         C.POINTER(_d), C.POINTER(_d)]),
+    "skema_residuals_dense": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "skema_residuals_csr": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "skema_residuals_rbf": (_i32, [_vp, _vp, _i64, _i64, _i64, _d, _i64, _vp, _i64, _vp, _vp, _i64, _i64, _vp]),
+    "skema_residuals_sym_dense": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "skema_residuals_sym_rbf": (_i32, [_vp, _vp, _i64, _i64, _i64, _d, _i64, _vp, _i64, _vp, _i64, _vp]),
     "skema_rbf_sketch_loss_grad": (_i32, [_vp, _vp, _i64, _i64, _i64, _d, _vp, _vp, _vp, _i64, _i32,
                                           C.POINTER(_d), _vp, _i64, C.POINTER(_d)]),
 }

@@ -9,7 +9,9 @@
 #include "adjoint.hpp"
 #include "linalg.hpp"
 #include "rbf.hpp"
+#include "rbf_fused.hpp"
 #include "csr_pass.hpp"
+#include "residuals.hpp"
 #include "sketch.hpp"
 #include "sparse.hpp"
 #include "util.hpp"
@@ -773,6 +775,85 @@ int skema_residual_fro_dense(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, co
     }
     *err2 = e2;
     *norm2 = a2;
+  });
+}
+
+namespace {
+void check_factors(int64_t m, int64_t n, int64_t r, const double* U, int64_t ldu, const double* S, const double* V,
+                   int64_t ldv, const double* rnorms) {
+  SKEMA_REQUIRE(m > 0 && n > 0 && r >= 0 && r <= std::min(m, n), kErrShape, "residuals: bad sizes");
+  SKEMA_REQUIRE(rnorms || r == 0, kErrArg, "residuals: null rnorms");
+  if (U) check_mat(U, m, r, ldu, "residuals U");
+  check_mat(V, n, r, ldv, "residuals V");
+  if (r) check_dev(S, r * sizeof(double), "residuals S");
+}
+}  // namespace
+
+int skema_residuals_dense(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, const double* A, int64_t lda,
+                          const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
+                          double* rnorms) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx && U, kErrArg, "residuals: null argument");
+    Ctx* c = &ctx->c;
+    set_device(c);
+    check_factors(m, n, r, U, ldu, S, V, ldv, rnorms);
+    check_mat(A, m, n, lda, "residuals A");
+    residuals_dense(c, m, n, r, A, lda, U, ldu, S, V, ldv, rnorms);
+  });
+}
+
+int skema_residuals_csr(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, const int64_t* rowptr, const int32_t* col,
+                        const double* val, const double* U, int64_t ldu, const double* S, const double* V,
+                        int64_t ldv, double* rnorms) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx && U, kErrArg, "residuals: null argument");
+    Ctx* c = &ctx->c;
+    set_device(c);
+    check_factors(m, n, r, U, ldu, S, V, ldv, rnorms);
+    check_csr_window(rowptr, col, val, m, c->stream);
+    residuals_csr(c, m, n, r, rowptr, col, val, U, ldu, S, V, ldv, rnorms);
+  });
+}
+
+int skema_residuals_rbf(skema_ctx* ctx, const double* points, int64_t n, int64_t d, int64_t ldp, double gamma,
+                        int64_t r, const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
+                        int64_t window, double* rnorms) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx && U, kErrArg, "residuals: null argument");
+    SKEMA_REQUIRE(window > 0, kErrShape, "residuals: window must be positive");
+    SKEMA_REQUIRE(d >= 1 && d <= kRbfMaxFeat, kErrUnsupported, "residuals: RBF features outside [1, 128]");
+    Ctx* c = &ctx->c;
+    set_device(c);
+    check_factors(n, n, r, U, ldu, S, V, ldv, rnorms);
+    check_mat(points, n, d, ldp, "residuals points");
+    residuals_rbf(c, points, n, d, ldp, gamma, r, U, ldu, S, V, ldv, window, rnorms);
+  });
+}
+
+int skema_residuals_sym_dense(skema_ctx* ctx, int64_t n, int64_t r, const double* A, int64_t lda, const double* V,
+                              int64_t ldv, const double* S, double* rnorms) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx, kErrArg, "residuals: null context");
+    Ctx* c = &ctx->c;
+    set_device(c);
+    check_factors(n, n, r, nullptr, 0, S, V, ldv, rnorms);
+    check_mat(A, n, n, lda, "residuals A");
+    residuals_sym_dense(c, n, r, A, lda, V, ldv, S, rnorms);
+  });
+}
+
+int skema_residuals_sym_rbf(skema_ctx* ctx, const double* points, int64_t n, int64_t d, int64_t ldp, double gamma,
+                            int64_t r, const double* V, int64_t ldv, const double* S, int64_t window,
+                            double* rnorms) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx, kErrArg, "residuals: null context");
+    SKEMA_REQUIRE(window > 0, kErrShape, "residuals: window must be positive");
+    SKEMA_REQUIRE(d >= 1 && d <= kRbfMaxFeat, kErrUnsupported, "residuals: RBF features outside [1, 128]");
+    Ctx* c = &ctx->c;
+    set_device(c);
+    check_factors(n, n, r, nullptr, 0, S, V, ldv, rnorms);
+    check_mat(points, n, d, ldp, "residuals points");
+    residuals_sym_rbf(c, points, n, d, ldp, gamma, r, V, ldv, S, window, rnorms);
   });
 }
 

@@ -392,6 +392,20 @@ class SketchySVD:
         L.call("skema_low_rank_approx", self.h, U.ctypes.data, S.ctypes.data, V.ctypes.data)
         return U.T.copy(), S, V.T.copy()
 
+    def compute_residuals(self, A, U=None, S=None, V=None) -> np.ndarray:
+        """SketchySVD::compute_residuals (src/Skema_SketchySVD.cpp:718-727):
+        per-triplet residual norms of (U, S, V), by default the factors of a
+        fresh low_rank_approx, against the input A (dense, CrsMatrix, or the
+        point set of an RBF sketch)."""
+        if U is None:
+            U, S, V = self.low_rank_approx(device=True)
+        if self.params.kernel_func == "gaussrbf":
+            self.rnrms = residual_norms_rbf(A, self.params.kernel_gamma, U, S, V, window=self.params.window,
+                                            ctx=self.ctx)
+        else:
+            self.rnrms = residual_norms(A, U, S, V, ctx=self.ctx)
+        return self.rnrms
+
     def checkpoint(self, name: str) -> np.ndarray:
         rows, cols = C.c_int64(), C.c_int64()
         L.call("skema_get_checkpoint", self.h, name.encode(), None, C.byref(rows), C.byref(cols))
@@ -426,6 +440,56 @@ def residual_fro(A, U, S, V, window: int = 4096, ctx: Optional[Context] = None) 
     L.call("skema_residual_fro_dense", ctx.h, m, n, r, _ptr(A), _ld(A), _ptr(U), _ld(U), _ptr(S), _ptr(V), _ld(V),
            window, C.byref(e2), C.byref(a2))
     return e2.value, a2.value
+
+
+def residual_norms(A, U, S, V, ctx: Optional[Context] = None) -> np.ndarray:
+    """Per-triplet residual norms ||[A^T u_i - s_i v_i ; A v_i - s_i u_i]||_2
+    (residuals, src/Skema_Residuals.hpp:160-217) for a dense column-major
+    device A or a CrsMatrix; U, S, V device tensors (m x r, r, n x r)."""
+    ctx = ctx or default_context()
+    r = U.shape[1]
+    out = np.empty(r)
+    if isinstance(A, CrsMatrix):
+        m, n = A.m, A.n
+        L.call("skema_residuals_csr", ctx.h, m, n, r, _ptr(A.rowptr), _ptr(A.col), _ptr(A.val), _ptr(U), _ld(U),
+               _ptr(S), _ptr(V), _ld(V), out.ctypes.data)
+    else:
+        m, n = A.shape
+        L.call("skema_residuals_dense", ctx.h, m, n, r, _ptr(A), _ld(A), _ptr(U), _ld(U), _ptr(S), _ptr(V), _ld(V),
+               out.ctypes.data)
+    return out
+
+
+def residual_norms_rbf(points, gamma: float, U, S, V, window: int = 4096,
+                       ctx: Optional[Context] = None) -> np.ndarray:
+    """The same for A = the Gaussian RBF matrix of the device point set,
+    generated window by window (the GaussRBFWindow path of map_A_by_window,
+    src/Skema_Residuals.hpp:60-121)."""
+    ctx = ctx or default_context()
+    n, d = points.shape
+    r = U.shape[1]
+    out = np.empty(r)
+    L.call("skema_residuals_rbf", ctx.h, _ptr(points), n, d, _ld(points), float(gamma), r, _ptr(U), _ld(U), _ptr(S),
+           _ptr(V), _ld(V), window, out.ctypes.data)
+    return out
+
+
+def residual_norms_sym(A, V, S, window: int = 4096, gamma: Optional[float] = None,
+                       ctx: Optional[Context] = None) -> np.ndarray:
+    """Eigenpair residuals ||A v_i - s_i v_i||_2 (src/Skema_Residuals.hpp:123-158):
+    A a dense symmetric device matrix, or (gamma given) the RBF matrix of the
+    device point set A."""
+    ctx = ctx or default_context()
+    r = V.shape[1]
+    out = np.empty(r)
+    if gamma is None:
+        n = A.shape[0]
+        L.call("skema_residuals_sym_dense", ctx.h, n, r, _ptr(A), _ld(A), _ptr(V), _ld(V), _ptr(S), out.ctypes.data)
+    else:
+        n, d = A.shape
+        L.call("skema_residuals_sym_rbf", ctx.h, _ptr(A), n, d, _ld(A), float(gamma), r, _ptr(V), _ld(V), _ptr(S),
+               window, out.ctypes.data)
+    return out
 
 
 def rbf_sketch_loss_grad(points, gamma: float, omega: _DimRedux, phi: _DimRedux, psi: _DimRedux,

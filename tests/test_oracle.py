@@ -357,3 +357,83 @@ def test_oracle_reproduces_committed_golden(golden_dir):
     assert O.rel_err(S, g["S"]) <= 1e-14
     Ua, Va = O.align_signs(U, V, g["U"], g["V"])
     assert O.rel_err(Ua, g["U"]) <= 1e-13 and O.rel_err(Va, g["V"]) <= 1e-13
+
+
+# --------------------------------------------------------------------------
+# Residual norms (src/Skema_Residuals.hpp:17-217), pinned by what the
+# singular value decomposition fixes: exact triplets have zero residual, a
+# shifted singular value by delta gives sqrt(2)|delta|, a flipped u_i gives
+# 2 sqrt(2) s_i, plus a loop restatement, window invariance and the RBF path
+# equal to the dense path on the materialised kernel.
+# --------------------------------------------------------------------------
+def _exact_triplets(m, n, r, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, n))
+    U, S, Vt = np.linalg.svd(A, full_matrices=False)
+    return A, U[:, :r].copy(), S[:r].copy(), Vt[:r, :].T.copy()
+
+
+@pytest.mark.parametrize("window", [None, 7, 64])
+def test_residuals_zero_for_exact_triplets(window):
+    A, U, S, V = _exact_triplets(60, 40, 6, 1)
+    rn = O.residual_norms(A, U, S, V, window=window)
+    assert rn.shape == (6,)
+    assert rn.max() <= 1e-12 * S[0]
+
+
+def test_residuals_closed_forms_of_planted_errors():
+    A, U, S, V = _exact_triplets(50, 30, 5, 2)
+    delta = np.array([1e-3, -2e-2, 0.5, 0.0, 3.0])
+    rn = O.residual_norms(A, U, S + delta, V)
+    assert np.allclose(rn, math.sqrt(2.0) * np.abs(delta), rtol=1e-10, atol=1e-12 * S[0])
+    Uf = U.copy()
+    Uf[:, 2] *= -1.0  # a flipped left vector: A v - s u' = 2 s u, A^T u' - s v = -2 s v
+    rn = O.residual_norms(A, Uf, S, V)
+    assert abs(rn[2] - 2.0 * math.sqrt(2.0) * S[2]) <= 1e-10 * S[2]
+    assert np.delete(rn, 2).max() <= 1e-12 * S[0]
+
+
+def test_residuals_loop_restatement_and_sparse():
+    rng = np.random.default_rng(3)
+    m, n, r = 9, 7, 3
+    A = rng.standard_normal((m, n))
+    U = rng.standard_normal((m, r))
+    V = rng.standard_normal((n, r))
+    S = rng.standard_normal(r)
+    ref = np.zeros(r)
+    for c in range(r):
+        acc = 0.0
+        for j in range(n):
+            t = sum(A[i, j] * U[i, c] for i in range(m)) - S[c] * V[j, c]
+            acc += t * t
+        for i in range(m):
+            t = sum(A[i, j] * V[j, c] for j in range(n)) - S[c] * U[i, c]
+            acc += t * t
+        ref[c] = math.sqrt(acc)
+    assert np.allclose(O.residual_norms(A, U, S, V, window=4), ref, rtol=1e-13)
+    As = sp.csr_matrix(np.where(rng.random((m, n)) < 0.4, A, 0.0))
+    assert np.allclose(O.residual_norms(As, U, S, V, window=2),
+                       O.residual_norms(As.toarray(), U, S, V), rtol=1e-13)
+
+
+def test_residuals_rbf_path_equals_materialised_kernel():
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((40, 3))
+    gamma = 0.3
+    K = O.gauss_rbf_compute(X, X, gamma, (0, 40))
+    U = rng.standard_normal((40, 4))
+    V = rng.standard_normal((40, 4))
+    S = rng.random(4)
+    rn = O.residual_norms(X, U, S, V, window=9, rbf_gamma=gamma)
+    assert np.allclose(rn, O.residual_norms(K, U, S, V), rtol=1e-12)
+
+
+def test_residuals_sym_eigenpairs():
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((30, 30))
+    A = B @ B.T
+    w, Q = np.linalg.eigh(A)
+    V, S = Q[:, ::-1][:, :4].copy(), w[::-1][:4].copy()
+    assert O.residual_norms_sym(A, V, S, window=8).max() <= 1e-11 * S[0]
+    d = np.array([0.0, 1e-2, -3.0, 0.25])
+    assert np.allclose(O.residual_norms_sym(A, V, S + d), np.abs(d), rtol=1e-9, atol=1e-11 * S[0])
