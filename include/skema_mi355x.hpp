@@ -260,6 +260,17 @@ class SketchySVD {
     return {U, S.to_host(), V};
   }
 
+  // compute_residuals (src/Skema_SketchySVD.cpp:718-727): the per-triplet
+  // residual norms rnrms of the factors (U, S, V) returned by low_rank_approx.
+  const std::vector<double>& compute_residuals(const MatrixType& A, const DeviceMatrix& U,
+                                               const std::vector<double>& S, const DeviceMatrix& V) {
+    DeviceMatrix Sd = DeviceMatrix::from_host(S.data(), (int64_t)S.size(), 1, (int64_t)S.size());
+    rnrms_.assign(S.size(), 0.0);
+    residuals(A, U, Sd, V);
+    return rnrms_;
+  }
+  const std::vector<double>& rnrms() const { return rnrms_; }
+
   std::string timings_json() const {
     std::vector<char> buf(16384);
     check(skema_sketch_timings_json(h_.get(), buf.data(), buf.size()));
@@ -274,11 +285,25 @@ class SketchySVD {
     check(skema_linear_update_csr(h_.get(), A.rowptr(), A.col(), A.val()));
   }
   void update_all(const PointSet& P) { check(skema_linear_update_rbf(h_.get(), P.X.data(), P.X.ld())); }
+  void residuals(const DeviceMatrix& A, const DeviceMatrix& U, const DeviceMatrix& S, const DeviceMatrix& V) {
+    check(skema_residuals_dense(ctx_.get(), p_.matrix_m, p_.matrix_n, S.extent(0), A.data(), A.ld(), U.data(),
+                                U.ld(), S.data(), V.data(), V.ld(), rnrms_.data()));
+  }
+  void residuals(const CrsMatrix& A, const DeviceMatrix& U, const DeviceMatrix& S, const DeviceMatrix& V) {
+    check(skema_residuals_csr(ctx_.get(), p_.matrix_m, p_.matrix_n, S.extent(0), A.rowptr(), A.col(), A.val(),
+                              U.data(), U.ld(), S.data(), V.data(), V.ld(), rnrms_.data()));
+  }
+  void residuals(const PointSet& P, const DeviceMatrix& U, const DeviceMatrix& S, const DeviceMatrix& V) {
+    check(skema_residuals_rbf(ctx_.get(), P.X.data(), P.X.extent(0), P.X.extent(1), P.X.ld(), p_.kernel_gamma,
+                              S.extent(0), U.data(), U.ld(), S.data(), V.data(), V.ld(), p_.window,
+                              rnrms_.data()));
+  }
 
   AlgParams p_;
   Context ctx_;
   std::vector<DimReduxT> maps_;
   std::shared_ptr<skema_sketch> h_;
+  std::vector<double> rnrms_;
 };
 
 }  // namespace skema_mi355x

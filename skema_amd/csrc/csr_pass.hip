@@ -60,27 +60,33 @@ __global__ __launch_bounds__(256) void k_left_all_sparse(const int64_t* cptr, co
 
 // The same with the map's transpose in ELL form (z entries per row,
 // u = row | sign << 31, magnitude mag): one contiguous 4z-byte read per
-// nonzero.
+// nonzero.  One wave per column (four per block, each with its own LDS
+// accumulator of P doubles): a column holds a few hundred nonzeros, so a
+// block per column would leave most of its lanes idle.
 __global__ __launch_bounds__(256) void k_left_all_ell(const int64_t* cptr, const int32_t* crow, const double* cval,
-                                                      const uint32_t* ell, int z, double mag, const double* cw,
-                                                      int64_t window, int64_t p0, int64_t P, double* out, int64_t ldo) {
-  extern __shared__ double acc[];  // P doubles
-  const int tid = threadIdx.x;
-  const int64_t j = blockIdx.x;
-  for (int64_t p = tid; p < P; p += 256) acc[p] = 0.0;
-  __syncthreads();
-  for (int64_t e = cptr[j] + tid; e < cptr[j + 1]; e += 256) {
-    const int64_t i = crow[e];
-    const double a = cval[e] * cw[(uint32_t)i / (uint32_t)window] * mag;
-    const uint32_t* me = ell + i * z;
-    for (int f = 0; f < z; ++f) {
-      const uint32_t u = me[f];
-      const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
-      if (p >= 0 && p < P) atomicAdd(&acc[p], (u >> 31) ? -a : a);
+                                                      int64_t ncols, const uint32_t* ell, int z, double mag,
+                                                      const double* cw, int64_t window, int64_t p0, int64_t P,
+                                                      double* out, int64_t ldo) {
+  extern __shared__ double acc_all[];  // 4 x P doubles
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t j = (int64_t)blockIdx.x * 4 + wave;
+  double* acc = acc_all + wave * P;
+  for (int64_t p = lane; p < P; p += 64) acc[p] = 0.0;
+  if (j < ncols) {
+    for (int64_t e = cptr[j] + lane; e < cptr[j + 1]; e += 64) {
+      const int64_t i = crow[e];
+      const double a = cval[e] * cw[(uint32_t)i / (uint32_t)window] * mag;
+      const uint32_t* me = ell + i * z;
+      for (int f = 0; f < z; ++f) {
+        const uint32_t u = me[f];
+        const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
+        if (p >= 0 && p < P) atomicAdd(&acc[p], (u >> 31) ? -a : a);
+      }
     }
   }
-  __syncthreads();
-  for (int64_t p = tid; p < P; p += 256) out[(p0 + p) + j * ldo] = acc[p];
+  __syncthreads();  // the wave's LDS atomics are complete before it reads back
+  if (j < ncols)
+    for (int64_t p = lane; p < P; p += 64) out[(p0 + p) + j * ldo] = acc[p];
 }
 
 // Y(i, p) = nu Y(i, p) + eta * sum over row i's nonzeros (j, a) of a * Mt(j, p)
@@ -125,27 +131,30 @@ __global__ __launch_bounds__(256) void k_right_all_ell(const int64_t* rowptr, co
                                                        int64_t p0, int64_t P, double eta, double nu, double* Y,
                                                        int64_t ldy) {
   __shared__ double acc[PCH * kPad];
-  const int tid = threadIdx.x, lane = tid & 63;
+  __shared__ int64_t rp[65];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = (int64_t)blockIdx.x * 64;
   const int64_t r1 = (r0 + 64 < rows) ? r0 + 64 : rows;
   for (int e = tid; e < PCH * kPad; e += 256) acc[e] = 0.0;
+  if (tid <= r1 - r0) rp[tid] = rowptr[r0 + tid];
   __syncthreads();
-  const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
-  int64_t r = r0;
+  const int64_t e0 = rp[0], e1 = rp[r1 - r0];
+  int r = 0;  // block-relative row of the thread's current nonzero (advances monotonically, LDS reads)
   for (int64_t e = e0 + tid; e < e1; e += 256) {
-    while (rowptr[r + 1] <= e) ++r;
+    while (rp[r + 1] <= e) ++r;
     const double a = val[e] * mag;
     const uint32_t* me = ell + (int64_t)col[e] * z;
     for (int f = 0; f < z; ++f) {
       const uint32_t u = me[f];
       const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
-      if (p >= 0 && p < P) atomicAdd(&acc[p * kPad + (r - r0)], (u >> 31) ? -a : a);
+      if (p >= 0 && p < P) atomicAdd(&acc[p * kPad + r], (u >> 31) ? -a : a);
     }
   }
   __syncthreads();
+  // wave w writes columns p = w, w + 4, ...: 64 consecutive rows per store
   const int64_t i = r0 + lane;
-  if (tid < 64 && i < rows) {
-    for (int64_t p = 0; p < P; ++p) {
+  if (i < rows) {
+    for (int64_t p = wave; p < P; p += 4) {
       double* y = Y + i + (p0 + p) * ldy;
       *y = (nu != 0.0) ? nu * *y + eta * acc[p * kPad + lane] : eta * acc[p * kPad + lane];
     }
@@ -221,11 +230,13 @@ void left_sparse(hipStream_t st, const CscWindow& H, const Map& M, int64_t P, co
                  double* out, int64_t ldo) {
   if (H.cols == 0) return;
   const Csr& Mt = M.logical_csr_t();
-  for (int64_t p0 = 0; p0 < P; p0 += 4096) {
-    const int64_t pc = std::min<int64_t>(4096, P - p0);
+  const int64_t chunk = M.ell_zeta > 0 ? 1024 : 4096;
+  for (int64_t p0 = 0; p0 < P; p0 += chunk) {
+    const int64_t pc = std::min<int64_t>(chunk, P - p0);
     if (M.ell_zeta > 0)
-      hipLaunchKernelGGL(k_left_all_ell, dim3((unsigned)H.cols), dim3(256), (size_t)pc * sizeof(double), st, H.ptr,
-                         H.row, H.val, M.ell.as<uint32_t>(), M.ell_zeta, M.ell_mag, cw, window, p0, pc, out, ldo);
+      hipLaunchKernelGGL(k_left_all_ell, dim3(nblocks(H.cols, 4)), dim3(256), (size_t)4 * pc * sizeof(double), st,
+                         H.ptr, H.row, H.val, H.cols, M.ell.as<uint32_t>(), M.ell_zeta, M.ell_mag, cw, window, p0, pc,
+                         out, ldo);
     else
       hipLaunchKernelGGL(k_left_all_sparse, dim3((unsigned)H.cols), dim3(256), (size_t)pc * sizeof(double), st,
                          H.ptr, H.row, H.val, Mt.rp(), Mt.ci(), Mt.v(), cw, window, p0, pc, out, ldo);
@@ -305,16 +316,16 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
         const dim3 grid(nblocks(m, 64));
         if (Om.ell_zeta > 0 && pc <= 64)
           hipLaunchKernelGGL(k_right_all_ell<64>, grid, dim3(256), 0, st, rowptr, col, val, m, Om.ell.as<uint32_t>(),
-                             Om.ell_zeta, Om.ell_mag, p0, pc, eta, nu, Y, m);
+                             Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
         else if (Om.ell_zeta > 0)
           hipLaunchKernelGGL(k_right_all_ell<128>, grid, dim3(256), 0, st, rowptr, col, val, m, Om.ell.as<uint32_t>(),
-                             Om.ell_zeta, Om.ell_mag, p0, pc, eta, nu, Y, m);
+                             Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
         else if (pc <= 64)
           hipLaunchKernelGGL(k_right_all_sparse<64>, grid, dim3(256), 0, st, rowptr, col, val, m, Ot.rp(), Ot.ci(),
-                             Ot.v(), p0, pc, eta, nu, Y, m);
+                             Ot.v(), p0, pc, eta, 0.0, Y, m);
         else
           hipLaunchKernelGGL(k_right_all_sparse<128>, grid, dim3(256), 0, st, rowptr, col, val, m, Ot.rp(), Ot.ci(),
-                             Ot.v(), p0, pc, eta, nu, Y, m);
+                             Ot.v(), p0, pc, eta, 0.0, Y, m);
         check_launch("k_right_all_sparse");
       }
     }
@@ -323,7 +334,7 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
       GemmOut o;
       o.C1 = Z; o.ldc1 = s; o.alpha1 = 1.0; o.beta1 = 0.0; o.nsplit = s;
       o.C2 = Z; o.ldc2 = s;
-      spmm_dense_csrT(st, s, Wm, s, sk.maps[3]->logical_csr(), o);
+      spmm_dense_csrT(st, s, Wm, s, sk.maps[3]->logical_csr(), o, &c->gws.buf);
     }
     return;
   }
@@ -341,13 +352,13 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
       const int pr = (int)cdiv(pc, 64);
       if (pr <= 2)
         hipLaunchKernelGGL(k_right_all_dense<2>, grid, dim3(256), 0, st, rowptr, col, val, m, Om + p0, k + s, pc, eta,
-                           nu, Y + p0 * m, m);
+                           0.0, Y + p0 * m, m);
       else if (pr <= 4)
         hipLaunchKernelGGL(k_right_all_dense<4>, grid, dim3(256), 0, st, rowptr, col, val, m, Om + p0, k + s, pc, eta,
-                           nu, Y + p0 * m, m);
+                           0.0, Y + p0 * m, m);
       else
         hipLaunchKernelGGL(k_right_all_dense<8>, grid, dim3(256), 0, st, rowptr, col, val, m, Om + p0, k + s, pc, eta,
-                           nu, Y + p0 * m, m);
+                           0.0, Y + p0 * m, m);
       check_launch("k_right_all_dense");
     }
   }

@@ -836,51 +836,49 @@ __global__ __launch_bounds__(kThreads) void k_ntx(NtArgs g) {
       psb[y] = reinterpret_cast<const uint32_t*>(g.bsc + row * g.ldbs + q * (g.ldbs / 4) + st0);
     }
   }
-  // Two register sets: the loads for step st + 2 are issued while step st
-  // computes, so a load has two steps of MFMA work to arrive.
-  uint32_t swa[2][TM], swb[2][TN];
-  i4v an[2][TM][2], rb[2][CB];
+  uint32_t swa[TM], swb[TN];
 
+  i4v an[TM][2], rb[CB];
   // the scale dword of step st (re-read on every step of its group of four:
   // a cached load is cheaper than a branch in the pipelined loop)
-  auto load_scales = [&](int st, int set) {
+  auto load_scales = [&](int st) {
 #pragma unroll
     for (int x = 0; x < TM; ++x)
-      if (ASC) swa[set][x] = psa[x][st >> 2];
+      if (ASC) swa[x] = psa[x][st >> 2];
 #pragma unroll
     for (int y = 0; y < TN; ++y)
-      if (BSC) swb[set][y] = psb[y][st >> 2];
+      if (BSC) swb[y] = psb[y][st >> 2];
   };
-  auto load_full = [&](int st, int set) {
-    load_scales(st, set);
+  auto load_full = [&](int st) {
+    load_scales(st);
     const int64_t off = (int64_t)st * SB;
 #pragma unroll
     for (int x = 0; x < TM; ++x) {
-      an[set][x][0] = *reinterpret_cast<const i4v*>(pa[x] + off);
-      an[set][x][1] = *reinterpret_cast<const i4v*>(pa[x] + off + 64);
+      an[x][0] = *reinterpret_cast<const i4v*>(pa[x] + off);
+      an[x][1] = *reinterpret_cast<const i4v*>(pa[x] + off + 64);
     }
 #pragma unroll
-    for (int e = 0; e < CB; ++e) rb[set][e] = *reinterpret_cast<const i4v*>(pb[e] + off);
+    for (int e = 0; e < CB; ++e) rb[e] = *reinterpret_cast<const i4v*>(pb[e] + off);
   };
-  auto load_any = [&](int st, int set) {
+  auto load_any = [&](int st) {
     const int64_t off = (int64_t)st * SB;
     const int64_t rem = kb1 - kb0 - off;  // valid bytes from the step start
     if (rem >= SB) {
-      load_full(st, set);
+      load_full(st);
       return;
     }
-    load_scales(st, set);
+    load_scales(st);
 #pragma unroll
     for (int x = 0; x < TM; ++x) {
-      an[set][x][0] = ld16_masked(pa[x] + off, rem - 16 * q);
-      an[set][x][1] = ld16_masked(pa[x] + off + 64, rem - 64 - 16 * q);
+      an[x][0] = ld16_masked(pa[x] + off, rem - 16 * q);
+      an[x][1] = ld16_masked(pa[x] + off + 64, rem - 64 - 16 * q);
     }
 #pragma unroll
-    for (int e = 0; e < CB; ++e) rb[set][e] = ld16_masked(pb[e] + off, rem - bch[e]);
+    for (int e = 0; e < CB; ++e) rb[e] = ld16_masked(pb[e] + off, rem - bch[e]);
   };
-  auto store_b = [&](int buf, int set) {
+  auto store_b = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < CB; ++e) *reinterpret_cast<i4v*>(&sB[buf][brow[e]][bch[e]]) = rb[set][e];
+    for (int e = 0; e < CB; ++e) *reinterpret_cast<i4v*>(&sB[buf][brow[e]][bch[e]]) = rb[e];
   };
 
   f4v acc[TM][TN];
@@ -889,45 +887,38 @@ __global__ __launch_bounds__(kThreads) void k_ntx(NtArgs g) {
 #pragma unroll
     for (int y = 0; y < TN; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
 
-  // this step's operands leave the prefetch registers before the loads of
-  // step st + 2 are issued into them
+  // this step's operands leave the prefetch registers before the next
+  // step's loads are issued into them
   i4v ac[TM][2];
   int sac[TM], sbc[TN];
-  auto take = [&](int st, int set) {
+  auto take = [&](int st) {
     const int sh = 8 * (st & 3);
 #pragma unroll
     for (int x = 0; x < TM; ++x) {
-      ac[x][0] = an[set][x][0];
-      ac[x][1] = an[set][x][1];
-      sac[x] = ASC ? (int)((swa[set][x] >> sh) & 0xffu) : 127;
+      ac[x][0] = an[x][0];
+      ac[x][1] = an[x][1];
+      sac[x] = ASC ? (int)((swa[x] >> sh) & 0xffu) : 127;
     }
 #pragma unroll
-    for (int y = 0; y < TN; ++y) sbc[y] = BSC ? (int)((swb[set][y] >> sh) & 0xffu) : 127;
+    for (int y = 0; y < TN; ++y) sbc[y] = BSC ? (int)((swb[y] >> sh) & 0xffu) : 127;
   };
   auto compute = [&](int buf) {
-    // all B fragments of the step first: the LDS reads overlap each other
-    // instead of one read latency per column tile
-    i4v b0[TN], b1[TN];
 #pragma unroll
     for (int y = 0; y < TN; ++y) {
-      b0[y] = *reinterpret_cast<const i4v*>(&sB[buf][y * 16 + l16][16 * q]);
-      b1[y] = *reinterpret_cast<const i4v*>(&sB[buf][y * 16 + l16][64 + 16 * q]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int y = 0; y < TN; ++y) {
+      const i4v b0 = *reinterpret_cast<const i4v*>(&sB[buf][y * 16 + l16][16 * q]);
+      const i4v b1 = *reinterpret_cast<const i4v*>(&sB[buf][y * 16 + l16][64 + 16 * q]);
 #pragma unroll
       for (int x = 0; x < TM; ++x) {
         const i4v a0 = ac[x][0], a1 = ac[x][1];
         if (FORM == kNxFp8) {
           const i8v av = i8v{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
-          const i8v bv = i8v{b0[y][0], b0[y][1], b0[y][2], b0[y][3], b1[y][0], b1[y][1], b1[y][2], b1[y][3]};
+          const i8v bv = i8v{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
           acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(av, bv, acc[x][y], 0, 0, 0, sac[x], 0, sbc[y]);
         } else {
           acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a0),
-                                                              __builtin_bit_cast(bf16x8, b0[y]), acc[x][y], 0, 0, 0);
+                                                              __builtin_bit_cast(bf16x8, b0), acc[x][y], 0, 0, 0);
           acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a1),
-                                                              __builtin_bit_cast(bf16x8, b1[y]), acc[x][y], 0, 0, 0);
+                                                              __builtin_bit_cast(bf16x8, b1), acc[x][y], 0, 0, 0);
         }
       }
     }
@@ -935,42 +926,32 @@ __global__ __launch_bounds__(kThreads) void k_ntx(NtArgs g) {
 
   const int nfull = (int)((kb1 - kb0) / SB);
   if (nsteps > 0) {
-    load_any(0, 0);
-    store_b(0, 0);
+    load_any(0);
+    store_b(0);
   }
-  if (nsteps > 1) load_any(1, 1);
   __syncthreads();
-  // One pipelined step; `set` (the register set of step st) is a literal at
-  // every call so the register arrays are never indexed at run time.
-  auto steady = [&](int st, int set) {
-    take(st, set);
-    load_full(st + 2, set);
-    __builtin_amdgcn_sched_barrier(0);
-    compute(st & 1);
-    __builtin_amdgcn_sched_barrier(0);
-    store_b((st + 1) & 1, set ^ 1);
-    __syncthreads();
-  };
-  auto last = [&](int st, int set) {
-    take(st, set);
-    if (st + 2 < nsteps) load_any(st + 2, set);
-    compute(st & 1);
-    if (st + 1 < nsteps) store_b((st + 1) & 1, set ^ 1);
-    __syncthreads();
-  };
   int st = 0;
-  // steady state, two steps per trip: the loads of step st + 2 are issued
-  // first, then this step's LDS reads and MFMAs, then (sched_barrier: the
-  // compiler would otherwise hoist it) the wait for step st + 1's loads and
-  // the LDS store of its B tile.
-  for (; st + 3 < nfull; st += 2) {
-    steady(st, 0);
-    steady(st + 1, 1);
+  // steady state: the next step is a full one, so the body has no branch and
+  // the order below is the order issued: next loads, this step's LDS reads
+  // and MFMAs, then (sched_barrier: the compiler would otherwise hoist it)
+  // the wait for the loads and the LDS store of the next B tile.
+  for (; st + 1 < nfull; ++st) {
+    take(st);
+    load_full(st + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(st & 1);
+    __builtin_amdgcn_sched_barrier(0);
+    store_b((st + 1) & 1);
+    __syncthreads();
   }
-  // the last steps (including a partial tail)
+  // last full step and the partial tail
   for (; st < nsteps; ++st) {
-    if ((st & 1) == 0) last(st, 0);
-    else last(st, 1);
+    take(st);
+    const bool more = st + 1 < nsteps;
+    if (more) load_any(st + 1);
+    compute(st & 1);
+    if (more) store_b((st + 1) & 1);
+    __syncthreads();
   }
   nt_epilogue_at<TM, TN, FORM == kNxFp8 && !ASC, FORM == kNxFp8 && !BSC>(g, acc, m0 + wave * (16 * TM), n0, lane);
 }

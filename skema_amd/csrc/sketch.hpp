@@ -66,6 +66,7 @@ struct Timed {
 struct Csr {
   int64_t rows = 0, cols = 0, nnz = 0;
   DevBuf rowptr, col, val;  // int64[rows+1], int32[nnz], double[nnz]
+  mutable int64_t max_row = -1;  // longest row, filled on first use
   const int64_t* rp() const { return rowptr.as<int64_t>(); }
   const int32_t* ci() const { return col.as<int32_t>(); }
   const double* v() const { return val.as<double>(); }

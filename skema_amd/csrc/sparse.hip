@@ -2,6 +2,8 @@
 // Bandwidth-bound gathers: consecutive lanes own consecutive output rows of a
 // column-major result, so stores and the dense-operand loads coalesce, and
 // no two threads ever accumulate into the same element.
+#include <vector>
+
 #include "sparse.hpp"
 #include "util.hpp"
 
@@ -51,6 +53,30 @@ __global__ void k_spmm_csr_rows(const int64_t* rp, const int32_t* ci, const doub
   *c = (beta != 0.0) ? alpha * acc + beta * *c : alpha * acc;
 }
 
+// Split form for short M and long rows of S: block (p, z) sums the z-th
+// chunk of row p's nonzeros for all M rows of B (thread = row i), partials
+// part[z][i + p M] reduced in a fixed order by k_spmm_split_reduce.
+__global__ __launch_bounds__(256) void k_spmm_dense_csrT_split(int64_t M, const double* B, int64_t ldb,
+                                                               const int64_t* rp, const int32_t* ci, const double* cv,
+                                                               int64_t R, int64_t chunk, double* part) {
+  const int64_t p = blockIdx.x, z = blockIdx.y;
+  const int64_t lo = rp[p] + z * chunk, hi0 = rp[p + 1];
+  const int64_t hi = (lo + chunk < hi0) ? lo + chunk : hi0;
+  for (int64_t i = threadIdx.x; i < M; i += 256) {
+    double acc = 0.0;
+    for (int64_t e = lo; e < hi; ++e) acc += cv[e] * B[i + (int64_t)ci[e] * ldb];
+    part[z * M * R + i + p * M] = acc;
+  }
+}
+__global__ void k_spmm_split_reduce(int64_t M, int64_t R, int splits, const double* part, GemmOut o) {
+  const int64_t total = M * R;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int z = 0; z < splits; ++z) v += part[(int64_t)z * total + e];
+    put(o, e % M, e / M, v);
+  }
+}
+
 __global__ void k_spmm_dense_csrT(int64_t M, const double* B, int64_t ldb, const int64_t* rp, const int32_t* ci,
                                   const double* cv, int64_t R, GemmOut o) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -79,6 +105,55 @@ __global__ void k_count_cols(const int64_t* rowptr, int64_t rows, const int32_t*
   const int64_t nnz = rowptr[rows] - base;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
     atomicAdd(&counts[col[base + e]], 1);
+}
+
+// Column counts per row block in LDS (cols <= kHistCols): block b takes rows
+// [b R, (b+1) R) (thread per row), counts its nonzeros per column in an LDS
+// histogram and writes it to T[b][0..cols).  No global atomics.
+constexpr int kHistCols = 32768;
+__global__ __launch_bounds__(1024) void k_count_cols_blocked(const int64_t* rowptr, int64_t rows, const int32_t* col,
+                                                            int64_t cols, int64_t R, int* T) {
+  __shared__ int hist[kHistCols];
+  const int64_t b = blockIdx.x;
+  for (int64_t c = threadIdx.x; c < cols; c += 1024) hist[c] = 0;
+  __syncthreads();
+  const int64_t r0 = b * R, r1 = (r0 + R < rows) ? r0 + R : rows;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 1024)
+    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) atomicAdd(&hist[col[e]], 1);
+  __syncthreads();
+  for (int64_t c = threadIdx.x; c < cols; c += 1024) T[b * cols + c] = hist[c];
+}
+// Per column: T[b][c] <- sum_{b' < b} T[b'][c] (exclusive, in block order),
+// counts[c] <- the column total.
+__global__ void k_scan_blocks(int64_t nb, int64_t cols, int* T, int* counts) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  int run = 0;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int t = T[b * cols + c];
+    T[b * cols + c] = run;
+    run += t;
+  }
+  counts[c] = run;
+}
+// Scatter of block b's rows into its own sub-range of every column:
+// cursor[c] = ptr[c] + T[b][c] in LDS, one LDS atomic per nonzero.
+__global__ __launch_bounds__(1024) void k_fill_csc_blocked(const int64_t* rowptr, int64_t rows, const int32_t* col,
+                                                          const double* val, int64_t cols, int64_t R, const int* T,
+                                                          const int64_t* ptr, int32_t* orow, double* oval) {
+  __shared__ int cur[kHistCols];
+  const int64_t b = blockIdx.x;
+  for (int64_t c = threadIdx.x; c < cols; c += 1024) cur[c] = (int)ptr[c] + T[b * cols + c];
+  __syncthreads();
+  const int64_t r0 = b * R, r1 = (r0 + R < rows) ? r0 + R : rows;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 1024) {
+    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+      const int32_t c = col[e];
+      const int pos = atomicAdd(&cur[c], 1);
+      orow[pos] = (int32_t)(i);
+      oval[pos] = val[e];
+    }
+  }
 }
 
 // Exclusive scan of counts[0..n) into ptr[0..n] by one block of 1024 threads.
@@ -254,8 +329,36 @@ void spmm_csr_rows(hipStream_t st, const Csr& S, int64_t r0, int64_t r1, int64_t
   check_launch("k_spmm_csr_rows");
 }
 
-void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, const Csr& S, const GemmOut& out) {
+void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, const Csr& S, const GemmOut& out,
+                     DevBuf* ws) {
   if (M == 0 || S.rows == 0) return;
+  // few output elements and long rows of S: split each row's nonzeros so the
+  // grid covers the chip (partials then a fixed-order sum)
+  const int64_t avg = S.nnz / S.rows;
+  if (ws && M <= 1024 && M * S.rows < 256 * 1024 && avg >= 64) {
+    const int64_t splits = std::min<int64_t>(64, std::max<int64_t>(1, avg / 32));
+    // the longest row bounds the chunk (read once per CSR and kept)
+    if (S.max_row < 0) {
+      std::vector<int64_t> h((size_t)S.rows + 1);
+      SKEMA_HIP(hipMemcpyAsync(h.data(), S.rp(), h.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      SKEMA_HIP(hipStreamSynchronize(st));
+      int64_t mr = 0;
+      for (int64_t r = 0; r < S.rows; ++r) mr = std::max(mr, h[(size_t)r + 1] - h[(size_t)r]);
+      S.max_row = mr;
+    }
+    const int64_t maxrow = S.max_row;
+    const int64_t chunk = std::max<int64_t>(1, cdiv(maxrow, splits));
+    const int64_t zs = cdiv(maxrow, chunk);
+    ws->reserve((size_t)zs * M * S.rows * sizeof(double));
+    double* part = ws->as<double>();
+    hipLaunchKernelGGL(k_spmm_dense_csrT_split, dim3((unsigned)S.rows, (unsigned)zs), dim3(256), 0, st, M, B, ldb,
+                       S.rp(), S.ci(), S.v(), S.rows, chunk, part);
+    check_launch("k_spmm_dense_csrT_split");
+    const unsigned nb = (unsigned)std::min<int64_t>(cdiv(M * S.rows, 256), 4096);
+    hipLaunchKernelGGL(k_spmm_split_reduce, dim3(nb), dim3(256), 0, st, M, S.rows, (int)zs, part, out);
+    check_launch("k_spmm_split_reduce");
+    return;
+  }
   hipLaunchKernelGGL(k_spmm_dense_csrT, dim3(blocks_for(M * S.rows)), dim3(256), 0, st, M, B, ldb, S.rp(), S.ci(),
                      S.v(), S.rows, out);
   check_launch("k_spmm_dense_csrT");
@@ -287,7 +390,12 @@ CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t
   const size_t o_ptr = align_up((size_t)cols * sizeof(int) * 2, 256);
   const size_t o_row = o_ptr + align_up((size_t)(cols + 1) * sizeof(int64_t), 256);
   const size_t o_val = o_row + align_up((size_t)nnz * sizeof(int32_t), 256);
-  const size_t total = o_val + align_up((size_t)nnz * sizeof(double), 256);
+  // Row-blocked counts and placement in LDS when the columns fit (128 row
+  // blocks; no global atomics); global atomic counters otherwise.
+  const int64_t nbk = 128, R = cdiv(rows, nbk);
+  const bool blocked = cols <= kHistCols && rows >= 4096 && nnz > 0;
+  const size_t o_T = o_val + align_up((size_t)nnz * sizeof(double), 256);
+  const size_t total = o_T + (blocked ? align_up((size_t)nbk * cols * sizeof(int), 256) : 0);
   scratch.reserve(total);
   char* base = scratch.as<char>();
   int* counts = reinterpret_cast<int*>(base + o_cnt);
@@ -299,15 +407,25 @@ CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t
   w.ptr = reinterpret_cast<int64_t*>(base + o_ptr);
   w.row = reinterpret_cast<int32_t*>(base + o_row);
   w.val = reinterpret_cast<double*>(base + o_val);
-  fill_i32(st, counts, 2 * cols, 0);
-  if (nnz > 0) {
-    const unsigned nb = blocks_for(nnz) > 65536 ? 65536 : blocks_for(nnz);
-    hipLaunchKernelGGL(k_count_cols, dim3(nb), dim3(256), 0, st, rowptr, rows, col, counts);
+  int* T = blocked ? reinterpret_cast<int*>(base + o_T) : nullptr;
+  if (blocked) {
+    hipLaunchKernelGGL(k_count_cols_blocked, dim3((unsigned)nbk), dim3(1024), 0, st, rowptr, rows, col, cols, R, T);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(blocks_for(cols)), dim3(256), 0, st, nbk, cols, T, counts);
+  } else {
+    fill_i32(st, counts, 2 * cols, 0);
+    if (nnz > 0) {
+      const unsigned nb = blocks_for(nnz) > 65536 ? 65536 : blocks_for(nnz);
+      hipLaunchKernelGGL(k_count_cols, dim3(nb), dim3(256), 0, st, rowptr, rows, col, counts);
+    }
   }
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, cols, w.ptr);
   if (nnz > 0) {
-    hipLaunchKernelGGL(k_fill_csc, dim3(blocks_for(rows)), dim3(256), 0, st, rowptr, rows, col, val, cursor, w.ptr,
-                       w.row, w.val);
+    if (blocked)
+      hipLaunchKernelGGL(k_fill_csc_blocked, dim3((unsigned)nbk), dim3(1024), 0, st, rowptr, rows, col, val, cols, R,
+                         T, w.ptr, w.row, w.val);
+    else
+      hipLaunchKernelGGL(k_fill_csc, dim3(blocks_for(rows)), dim3(256), 0, st, rowptr, rows, col, val, cursor, w.ptr,
+                         w.row, w.val);
     if (sorted) hipLaunchKernelGGL(k_sort_columns, dim3((unsigned)cols), dim3(64), 0, st, w.ptr, w.row, w.val);
   }
   check_launch("csr_window_to_csc");

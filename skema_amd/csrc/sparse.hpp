@@ -13,8 +13,11 @@ void spmm_csr_rows(hipStream_t st, const Csr& S, int64_t r0, int64_t r1, int64_t
                    const double* B, int64_t ldb, int64_t N, double beta, double* C, int64_t ldc);
 
 // out (M x R) = B (M x K) * S^T with S an R x K CSR; written through the
-// split epilogue of GemmOut (column p of the product = row p of S).
-void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, const Csr& S, const GemmOut& out);
+// split epilogue of GemmOut (column p of the product = row p of S).  With a
+// workspace, short M and long rows of S split each row's nonzeros over
+// blocks (FP64 partials, fixed-order sum).
+void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, const Csr& S, const GemmOut& out,
+                     DevBuf* ws = nullptr);
 
 // Column-compressed copy of a CSR window (rowptr may start at an offset).
 struct CscWindow {

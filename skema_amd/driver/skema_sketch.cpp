@@ -135,11 +135,15 @@ int run(const skema_mi355x::AlgParams& p, const M& A, const std::string& history
   sk.linear_update(A);
   auto usv = sk.low_rank_approx();
   const std::vector<double>& S = std::get<1>(usv);
+  // compute_residuals, then the history's svals and rnrms (src/Skema_SketchySVD.cpp:729-755)
+  const std::vector<double>& R = sk.compute_residuals(A, std::get<0>(usv), S, std::get<2>(usv));
   std::ostringstream js;
   js.precision(17);
   std::string t = sk.timings_json();
   js << "{" << t.substr(1, t.size() - 2) << ",\"svals\":[";
   for (size_t i = 0; i < S.size(); ++i) js << (i ? "," : "") << S[i];
+  js << "],\"rnrms\":[";
+  for (size_t i = 0; i < R.size(); ++i) js << (i ? "," : "") << R[i];
   js << "]}";
   if (history.empty()) {
     std::cout << js.str() << std::endl;
