@@ -63,9 +63,12 @@ extern "C" {
 #define SKEMA_PREC_FP64 0   /* plain: FP64 MFMA, the reference's arithmetic   */
 #define SKEMA_PREC_FP32 1   /* FP32 inputs and accumulation                   */
 #define SKEMA_PREC_BF16X3 2 /* split-BF16 (hi*hi + hi*lo + lo*hi), FP32 acc.  */
-#define SKEMA_PREC_FP8 3    /* FP8 e4m3, one power-of-two scale per operand
-                               row (map rows, window rows and columns), FP32
-                               accumulation; core product in BF16           */
+#define SKEMA_PREC_FP8 3    /* FP8 e4m3: maps with one power-of-two scale per
+                               row, the window with an E8M0 scale per
+                               32-element block of each row and column (one
+                               scale per row and column when a window starts
+                               off a 16-row boundary), FP32 accumulation;
+                               core product in BF16                         */
 #define SKEMA_PREC_BF16 4   /* single-term BF16, FP32 acc.                    */
 
 /* Seeded Gauss map distributions. */
@@ -82,6 +85,7 @@ extern "C" {
 typedef struct skema_ctx_s skema_ctx;
 typedef struct skema_map_s skema_map;
 typedef struct skema_sketch_s skema_sketch;
+typedef struct skema_spd_s skema_spd;
 
 /* Sketch parameters: the sketch fields of AlgParams
  * (src/Skema_AlgParams.hpp:12-90; defaults src/Skema_AlgParams.cpp:10-57). */
@@ -230,6 +234,37 @@ int skema_get_checkpoint(const skema_sketch* sk, const char* name, double* host,
 /* Timings with the reference's keys ({"init":{...},"update":{...},
  * "approx":{...}}, src/Skema_SketchySVD.cpp:101-116) as JSON text. */
 int skema_sketch_timings_json(const skema_sketch* sk, char* buf, size_t len);
+
+/* ---- SketchySPD (Nystrom sketch of a symmetric PSD matrix) ---------------- */
+/* SketchySPD<M,D> (src/Skema_SketchySVD.cpp:757-1151): Y = A Omega (m x k)
+ * over row windows, then the fixed-rank Nystrom approximation.  p->m == p->n;
+ * k = range (4r+1 when range < rank, :763-765); omega is the logical n x k
+ * map (the reference's DimReduxT(ncol, range, seeds[0]), :768-773).  The
+ * reference's window loop never resets its pass counter (:795-858) and so
+ * sketches window 0 only; every window is sketched here. */
+int skema_spd_create(skema_ctx* ctx, const skema_params* p, skema_map* omega, skema_spd** out);
+int skema_spd_destroy(skema_spd* sp);
+/* One window of rows [row0, row0+rows): Y[idx] = nu Y[idx] + eta A_w Omega
+ * (update + axpy, :896-931, :1153-1200).  Dense A_w (ldh), a CSR window
+ * (rowptr at row row0 of the whole matrix), or the RBF rows of the point set. */
+int skema_spd_update_dense(skema_spd* sp, const double* H, int64_t ldh, int64_t row0, int64_t rows);
+int skema_spd_update_csr(skema_spd* sp, const int64_t* rowptr, const int32_t* col, const double* val,
+                         int64_t row0, int64_t rows);
+int skema_spd_update_rbf(skema_spd* sp, const double* points, int64_t ldp, int64_t row0, int64_t rows);
+/* linear_update (:790-880): zero Y, then every window. */
+int skema_spd_linear_update_dense(skema_spd* sp, const double* A, int64_t lda);
+int skema_spd_linear_update_csr(skema_spd* sp, const int64_t* rowptr, const int32_t* col, const double* val);
+int skema_spd_linear_update_rbf(skema_spd* sp, const double* points, int64_t ldp);
+/* Y (m x k, column-major) to the host. */
+int skema_spd_get_state(const skema_spd* sp, double* Y);
+/* low_rank_approx (:934-1151): U (n x r) and the eigenvalues lambda (r).
+ * shift_removal SKEMA_SPD_SHIFT_EPS subtracts the shift eps ||Y||_2 (Tropp et
+ * al. 2017, Alg. 3); SKEMA_SPD_SHIFT_REFERENCE subtracts ||Y||_2 as the
+ * reference does (:1141-1145), which clips every eigenvalue below ||Y||_2. */
+#define SKEMA_SPD_SHIFT_EPS 0
+#define SKEMA_SPD_SHIFT_REFERENCE 1
+int skema_spd_low_rank_approx(skema_spd* sp, int shift_removal, double* U, double* S);
+int skema_spd_low_rank_approx_device(skema_spd* sp, int shift_removal, double* U, int64_t ldu, double* S);
 
 /* ---- streamed quality pass ----------------------------------------------- */
 /* ||A - U diag(S) V^T||_F^2 and ||A||_F^2 for dense device A (m x n, lda),

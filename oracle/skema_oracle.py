@@ -18,6 +18,8 @@ What it restates (reference = jeremy-myers/skema, paths relative to its root):
   (src/Skema_Window.hpp:36-107, src/Skema_Kernel.cpp:5-81).
 * ``residuals`` / ``map_A_by_window`` (src/Skema_Residuals.hpp:17-217): the
   per-triplet residual norms of the final factors.
+* ``SketchySPD`` (src/Skema_SketchySVD.cpp:757-1151): the Nystrom sketch and
+  its fixed-rank approximation (two reference defects documented there).
 * ``linalg::qr`` / ``ls`` / ``svd`` (src/Skema_BlasLapack.hpp:71-97,168-243):
   the SAME LAPACK routines (dgeqrf+dorgqr with lwork=n, dgels, dgesvd 'S','S')
   called through ``scipy.linalg.lapack`` with the reference's arguments.
@@ -160,6 +162,25 @@ def ls(A: np.ndarray, B: np.ndarray) -> np.ndarray:
     lw = max(1, m * n + max(m * n, p))
     _, x, info = lapack.dgels(np.asfortranarray(A), np.asfortranarray(B), trans="N", lwork=lw)
     return np.ascontiguousarray(x[:n, :])
+
+
+def ls_trans(A: np.ndarray, B: np.ndarray, trans: str) -> np.ndarray:
+    """linalg::ls(&trans, A, B, m, n, nrhs) with either flag (dgels)."""
+    m, n = A.shape
+    p = B.shape[1]
+    lw = max(1, m * n + max(m * n, p))
+    _, x, info = lapack.dgels(np.asfortranarray(A), np.asfortranarray(B), trans=trans, lwork=lw)
+    rows = n if trans == "N" else m
+    return np.ascontiguousarray(x[:rows, :])
+
+
+def chol_upper(A: np.ndarray) -> np.ndarray:
+    """linalg::chol (src/Skema_BlasLapack.hpp:245-272): dpotrf 'U', the strict
+    lower triangle zeroed; info > 0 raises (not positive definite)."""
+    c, info = lapack.dpotrf(np.asfortranarray(A), lower=0, clean=1)
+    if info > 0:
+        raise np.linalg.LinAlgError(f"dpotrf: the leading minor of order {info} is not positive definite")
+    return np.triu(np.ascontiguousarray(c))
 
 
 def svd(A: np.ndarray) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -503,6 +524,79 @@ class SketchySVD:
         r = self.rank
         self.uvecs, self.svals, self.vvecs = QU[:, :r].copy(), S[:r].copy(), PV[:, :r].copy()
         return self.uvecs, self.svals, self.vvecs
+
+
+# --------------------------------------------------------------------------
+# SketchySPD: the Nystrom sketch of a symmetric PSD matrix
+# --------------------------------------------------------------------------
+class SketchySPD:
+    """SketchySPD<M,D> (src/Skema_SketchySVD.cpp:757-1151).
+
+    Y = A Omega (n x k) by row windows, then the fixed-rank Nystrom
+    approximation of Tropp, Yurtsever, Udell and Cevher (2017, Alg. 3): a
+    shift eps ||Y||_2 Omega, B = Omega^T Y_shift, C = chol((B + B^T) / 2), E = Y
+    C^{-1} and the SVD of E.  Two reference defects are not reproduced:
+    the window loop never resets its ``pass`` counter (:795-858), so only
+    window 0 would be sketched -- here every window is; and the shift removal
+    subtracts ||Y||_2 (the local ``nu``, :1141-1145) rather than the shift
+    eps ||Y||_2 -- ``shift_removal="reference"`` reproduces it.  The
+    sparse-sign ``axpy`` multiplies (``*=``, src/Skema_DimRedux_SparseSign.cpp:
+    114-123); here the shift is added for both map families."""
+
+    def __init__(self, p: AlgParams, omega=None, shift_removal: str = "textbook"):
+        self.p = p
+        self.nrow, self.ncol, self.rank = p.matrix_m, p.matrix_n, p.rank
+        self.range = 4 * p.rank + 1 if p.sketch_range < p.rank else p.sketch_range  # :763-765
+        self.eta, self.nu = p.sketch_eta, p.sketch_nu
+        if omega is None:  # DimReduxT(ncol, range, seeds[0]) (:768-773)
+            if p.dim_redux == "gauss":
+                omega = GaussDimRedux(gauss_map(self.ncol, self.range, p.seeds[0]))
+            else:
+                omega = SparseSignDimRedux(sparse_sign_rows(self.ncol, self.range, p.seeds[0]))
+        self.Omega = omega
+        self.window = GaussRBFWindow(p.kernel_gamma) if p.kernel_func == "gaussrbf" else Window()
+        self.shift_removal = shift_removal
+        self.Y = None
+
+    def update(self, H):
+        """y = H Omega: Gauss rmap (:896-901), sparse-sign lmap of H^T in 'T'
+        mode transposed back (:903-912), CRS rmap (:914-931)."""
+        one, zero = 1.0, 0.0
+        if sp.issparse(H) or not self.Omega.issparse:
+            return self.Omega.rmap(one, H, zero, "N", "N")
+        At = np.ascontiguousarray(H.T)
+        return np.ascontiguousarray(self.Omega.lmap(one, At, zero, "T", "N").T)
+
+    def linear_update(self, A):
+        """src/Skema_SketchySVD.cpp:790-880 with axpy (:1153-1200):
+        Y[idx] = nu Y[idx] + eta y for every window."""
+        m, w = self.nrow, self.p.window
+        self.Y = np.zeros((m, self.range))
+        if w == m:
+            self.Y = self.nu * self.Y + self.eta * self.update(self.window.get(A, (0, m)))
+            return
+        for r0 in range(0, m, w):
+            idx = (r0, min(r0 + w, m))
+            y = self.update(self.window.get(A, idx))
+            self.Y[idx[0]:idx[1], :] = self.nu * self.Y[idx[0]:idx[1], :] + self.eta * y
+
+    def low_rank_approx(self):
+        """src/Skema_SketchySVD.cpp:934-1151 -> (U n x r, lambda r)."""
+        mu = np.finfo(np.float64).eps
+        Y = self.Y
+        nrm = svd(Y.copy())[1][0]  # linalg::nrm2 = largest singular value (BlasLapack.hpp:159-166)
+        shift = mu * nrm
+        Om = _dense(self.Omega.data)
+        Yv = Y + shift * Om  # Omega.axpy (DimRedux_Gauss.cpp:102-130)
+        B = np.asarray(Om.T @ Yv)  # Omega.lmap(1, Yv, 0, 'T', 'N')
+        C = 0.5 * (B + B.T)  # :1004-1022
+        R = chol_upper(C)  # :1048-1058
+        E = ls_trans(R, np.ascontiguousarray(Y.T), "T").T  # E = Y R^{-1} (:1064-1082)
+        U, S, _ = svd(np.ascontiguousarray(E))  # :1094-1107
+        r = self.rank
+        lam = S[:r] ** 2 - (nrm if self.shift_removal == "reference" else shift)  # :1141-1145
+        self.uvecs, self.svals = U[:, :r].copy(), np.maximum(lam, 0.0)
+        return self.uvecs, self.svals
 
 
 # --------------------------------------------------------------------------

@@ -10,6 +10,7 @@ from .sketchy import (  # noqa: F401
     Context,
     CrsMatrix,
     GaussDimRedux,
+    SketchySPD,
     SketchySVD,
     SparseSignDimRedux,
     colmajor,

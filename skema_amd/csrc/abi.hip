@@ -8,10 +8,12 @@
 
 #include "adjoint.hpp"
 #include "linalg.hpp"
+#include "lowp.hpp"
 #include "rbf.hpp"
 #include "rbf_fused.hpp"
 #include "csr_pass.hpp"
 #include "residuals.hpp"
+#include "spd.hpp"
 #include "sketch.hpp"
 #include "sparse.hpp"
 #include "util.hpp"
@@ -24,6 +26,9 @@ struct skema_map_s {
 };
 struct skema_sketch_s {
   skema::Sketch s;
+};
+struct skema_spd_s {
+  skema::Spd s;
 };
 
 namespace skema {
@@ -576,7 +581,10 @@ int skema_linear_update_dense(skema_sketch* h, const double* A, int64_t lda) {
     SKEMA_REQUIRE(sk.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "linear_update_dense on an RBF sketch");
     check_mat(A, sk.p.m, sk.p.n, lda, "linear_update_dense A");
     zero_state(sk);
-    for_each_window(sk, [&](int64_t r0, int64_t w) { update_dense_window(sk, A + r0, lda, r0, w); });
+    if (!sk.sparse_maps && sk.p.precision != SKEMA_PREC_FP64)
+      linear_update_dense_lowp(sk, A, lda);  // quantise pass of window w+1 overlapped with window w
+    else
+      for_each_window(sk, [&](int64_t r0, int64_t w) { update_dense_window(sk, A + r0, lda, r0, w); });
     sk.ctx->sync_if_needed();
   });
 }
@@ -854,6 +862,188 @@ int skema_residuals_sym_rbf(skema_ctx* ctx, const double* points, int64_t n, int
     check_factors(n, n, r, nullptr, 0, S, V, ldv, rnorms);
     check_mat(points, n, d, ldp, "residuals points");
     residuals_sym_rbf(c, points, n, d, ldp, gamma, r, V, ldv, S, window, rnorms);
+  });
+}
+
+int skema_spd_create(skema_ctx* ctx, const skema_params* p, skema_map* omega, skema_spd** out) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx && p && omega && out, kErrArg, "spd_create: null argument");
+    set_device(&ctx->c);
+    SKEMA_REQUIRE(p->m > 0 && p->m == p->n, kErrShape, "spd_create: A must be square (m == n > 0)");
+    SKEMA_REQUIRE(p->rank >= 1 && p->window >= 1, kErrArg, "spd_create: rank and window must be >= 1");
+    SKEMA_REQUIRE(p->eta >= 0.0 && p->nu >= 0.0, kErrArg, "spd_create: eta and nu must be >= 0");
+    SKEMA_REQUIRE(p->precision == SKEMA_PREC_FP64, kErrUnsupported, "spd: only the FP64 form is built");
+    if (p->kernel == SKEMA_KERNEL_GAUSSRBF) {
+      SKEMA_REQUIRE(!p->issparse, kErrUnsupported, "Gauss RBF window on a sparse matrix is not available");
+      SKEMA_REQUIRE(p->nfeat >= 1 && p->nfeat <= kRbfMaxFeat, kErrUnsupported, "spd: RBF features outside [1, 128]");
+      SKEMA_REQUIRE(p->gamma >= 0.0, kErrArg, "RBF window: gamma must be >= 0");
+    }
+    std::unique_ptr<skema_spd_s> h(new skema_spd_s);
+    Spd& sp = h->s;
+    sp.ctx = &ctx->c;
+    sp.p = *p;
+    sp.k = (p->range < p->rank) ? 4 * p->rank + 1 : p->range;
+    SKEMA_REQUIRE(p->rank <= sp.k && sp.k <= p->n, kErrShape, "spd_create: need rank <= range <= n");
+    sp.omega = reinterpret_cast<Map*>(omega);
+    SKEMA_REQUIRE(sp.omega->lrows() == p->n && sp.omega->lcols() == sp.k, kErrShape,
+                  "spd_create: Omega must be the logical n x range map");
+    spd_prepare(sp);
+    sp.Y.alloc((size_t)p->m * sp.k * sizeof(double));
+    fill_f64(ctx->c.stream, sp.Y.as<double>(), p->m * sp.k, 0.0);
+    sp.timings["init"]["omega"] = sp.omega->init_seconds;
+    SKEMA_HIP(hipStreamSynchronize(ctx->c.stream));
+    *out = h.release();
+  });
+}
+
+int skema_spd_destroy(skema_spd* sp) {
+  return guarded([&] {
+    if (!sp) return;
+    set_device(sp->s.ctx);
+    (void)hipStreamSynchronize(sp->s.ctx->stream);
+    delete sp;
+  });
+}
+
+extern "C++" {
+namespace {
+void spd_check_window(const Spd& sp, int64_t row0, int64_t rows) {
+  SKEMA_REQUIRE(rows > 0 && row0 >= 0 && row0 + rows <= sp.p.m, kErrShape, "spd update: window outside [0, m)");
+}
+template <class Fn>
+void spd_windows(const Spd& sp, Fn&& fn) {
+  const int64_t m = sp.p.m, w = std::min(sp.p.window, m);
+  for (int64_t r0 = 0; r0 < m; r0 += w) fn(r0, std::min(w, m - r0));
+}
+void spd_zero(Spd& sp) { fill_f64(sp.ctx->stream, sp.Y.as<double>(), sp.p.m * sp.k, 0.0); }
+}  // namespace
+}  // extern "C++"
+
+int skema_spd_update_dense(skema_spd* h, const double* H, int64_t ldh, int64_t row0, int64_t rows) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h, kErrArg, "null spd");
+    Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_REQUIRE(sp.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "spd_update_dense on an RBF sketch");
+    spd_check_window(sp, row0, rows);
+    check_mat(H, rows, sp.p.n, ldh, "spd window");
+    spd_update_dense(sp, H, ldh, row0, rows);
+    sp.updated = true;
+    sp.ctx->sync_if_needed();
+  });
+}
+
+int skema_spd_update_csr(skema_spd* h, const int64_t* rowptr, const int32_t* col, const double* val, int64_t row0,
+                         int64_t rows) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h, kErrArg, "null spd");
+    Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_REQUIRE(sp.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "spd_update_csr on an RBF sketch");
+    spd_check_window(sp, row0, rows);
+    check_csr_window(rowptr, col, val, rows, sp.ctx->stream);
+    spd_update_csr(sp, rowptr, col, val, row0, rows);
+    sp.updated = true;
+    sp.ctx->sync_if_needed();
+  });
+}
+
+int skema_spd_update_rbf(skema_spd* h, const double* points, int64_t ldp, int64_t row0, int64_t rows) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h, kErrArg, "null spd");
+    Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_REQUIRE(sp.p.kernel == SKEMA_KERNEL_GAUSSRBF, kErrArg, "spd_update_rbf without an RBF window");
+    spd_check_window(sp, row0, rows);
+    check_mat(points, sp.p.n, sp.p.nfeat, ldp, "spd points");
+    spd_update_rbf(sp, points, ldp, row0, rows);
+    sp.updated = true;
+    sp.ctx->sync_if_needed();
+  });
+}
+
+int skema_spd_linear_update_dense(skema_spd* h, const double* A, int64_t lda) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h, kErrArg, "null spd");
+    Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_REQUIRE(sp.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "spd linear_update_dense on an RBF sketch");
+    check_mat(A, sp.p.m, sp.p.n, lda, "spd A");
+    spd_zero(sp);
+    spd_windows(sp, [&](int64_t r0, int64_t w) { spd_update_dense(sp, A + r0, lda, r0, w); });
+    sp.updated = true;
+    sp.ctx->sync_if_needed();
+  });
+}
+
+int skema_spd_linear_update_csr(skema_spd* h, const int64_t* rowptr, const int32_t* col, const double* val) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h, kErrArg, "null spd");
+    Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_REQUIRE(sp.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "spd linear_update_csr on an RBF sketch");
+    check_csr_window(rowptr, col, val, sp.p.m, sp.ctx->stream);
+    spd_zero(sp);
+    spd_windows(sp, [&](int64_t r0, int64_t w) { spd_update_csr(sp, rowptr + r0, col, val, r0, w); });
+    sp.updated = true;
+    sp.ctx->sync_if_needed();
+  });
+}
+
+int skema_spd_linear_update_rbf(skema_spd* h, const double* points, int64_t ldp) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h, kErrArg, "null spd");
+    Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_REQUIRE(sp.p.kernel == SKEMA_KERNEL_GAUSSRBF, kErrArg, "spd linear_update_rbf without an RBF window");
+    check_mat(points, sp.p.n, sp.p.nfeat, ldp, "spd points");
+    spd_zero(sp);
+    spd_windows(sp, [&](int64_t r0, int64_t w) { spd_update_rbf(sp, points, ldp, r0, w); });
+    sp.updated = true;
+    sp.ctx->sync_if_needed();
+  });
+}
+
+int skema_spd_get_state(const skema_spd* h, double* Y) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h && Y, kErrArg, "spd_get_state: null argument");
+    const Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_HIP(hipMemcpyAsync(Y, sp.Y.p, (size_t)sp.p.m * sp.k * sizeof(double), hipMemcpyDeviceToHost,
+                             sp.ctx->stream));
+    SKEMA_HIP(hipStreamSynchronize(sp.ctx->stream));
+  });
+}
+
+int skema_spd_low_rank_approx_device(skema_spd* h, int shift_removal, double* U, int64_t ldu, double* S) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h && U && S, kErrArg, "spd_low_rank_approx: null argument");
+    Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_REQUIRE(shift_removal == SKEMA_SPD_SHIFT_EPS || shift_removal == SKEMA_SPD_SHIFT_REFERENCE, kErrArg,
+                  "spd_low_rank_approx: bad shift_removal");
+    SKEMA_REQUIRE(sp.updated, kErrState, "spd_low_rank_approx before any update");
+    check_mat(U, sp.p.n, sp.p.rank, ldu, "spd U");
+    check_dev(S, (size_t)sp.p.rank * sizeof(double), "spd S");
+    spd_low_rank_approx(sp, shift_removal, U, ldu, S);
+    SKEMA_HIP(hipStreamSynchronize(sp.ctx->stream));
+  });
+}
+
+int skema_spd_low_rank_approx(skema_spd* h, int shift_removal, double* U, double* S) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h && U && S, kErrArg, "spd_low_rank_approx: null argument");
+    Spd& sp = h->s;
+    set_device(sp.ctx);
+    SKEMA_REQUIRE(shift_removal == SKEMA_SPD_SHIFT_EPS || shift_removal == SKEMA_SPD_SHIFT_REFERENCE, kErrArg,
+                  "spd_low_rank_approx: bad shift_removal");
+    SKEMA_REQUIRE(sp.updated, kErrState, "spd_low_rank_approx before any update");
+    const int64_t n = sp.p.n, r = sp.p.rank;
+    DevBuf Ud((size_t)n * r * sizeof(double)), Sd((size_t)r * sizeof(double));
+    spd_low_rank_approx(sp, shift_removal, Ud.as<double>(), n, Sd.as<double>());
+    SKEMA_HIP(hipMemcpyAsync(U, Ud.p, (size_t)n * r * sizeof(double), hipMemcpyDeviceToHost, sp.ctx->stream));
+    SKEMA_HIP(hipMemcpyAsync(S, Sd.p, (size_t)r * sizeof(double), hipMemcpyDeviceToHost, sp.ctx->stream));
+    SKEMA_HIP(hipStreamSynchronize(sp.ctx->stream));
   });
 }
 

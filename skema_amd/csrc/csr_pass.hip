@@ -58,11 +58,26 @@ __global__ __launch_bounds__(256) void k_left_all_sparse(const int64_t* cptr, co
   for (int64_t p = tid; p < P; p += 256) out[(p0 + p) + j * ldo] = acc[p];
 }
 
+// The z entries of ELL row c: with ZC = 8 (the sparse-sign default) two
+// 16-byte loads issued together, otherwise one dword per entry.
+template <int ZC>
+__device__ __forceinline__ void ell_row(const uint32_t* ell, int64_t c, int z, uint32_t (&u)[16]) {
+  if constexpr (ZC == 8) {
+    const uint4 a = *reinterpret_cast<const uint4*>(ell + c * 8);
+    const uint4 b = *reinterpret_cast<const uint4*>(ell + c * 8 + 4);
+    u[0] = a.x; u[1] = a.y; u[2] = a.z; u[3] = a.w;
+    u[4] = b.x; u[5] = b.y; u[6] = b.z; u[7] = b.w;
+  } else {
+    for (int f = 0; f < z; ++f) u[f] = ell[c * z + f];
+  }
+}
+
 // The same with the map's transpose in ELL form (z entries per row,
 // u = row | sign << 31, magnitude mag): one contiguous 4z-byte read per
 // nonzero.  One wave per column (four per block, each with its own LDS
 // accumulator of P doubles): a column holds a few hundred nonzeros, so a
 // block per column would leave most of its lanes idle.
+template <int ZC>
 __global__ __launch_bounds__(256) void k_left_all_ell(const int64_t* cptr, const int32_t* crow, const double* cval,
                                                       int64_t ncols, const uint32_t* ell, int z, double mag,
                                                       const double* cw, int64_t window, int64_t p0, int64_t P,
@@ -76,9 +91,13 @@ __global__ __launch_bounds__(256) void k_left_all_ell(const int64_t* cptr, const
     for (int64_t e = cptr[j] + lane; e < cptr[j + 1]; e += 64) {
       const int64_t i = crow[e];
       const double a = cval[e] * cw[(uint32_t)i / (uint32_t)window] * mag;
-      const uint32_t* me = ell + i * z;
-      for (int f = 0; f < z; ++f) {
-        const uint32_t u = me[f];
+      uint32_t uu[16];
+      ell_row<ZC>(ell, i, z, uu);
+      const int zz = ZC ? ZC : z;
+#pragma unroll
+      for (int f = 0; f < (ZC ? ZC : 16); ++f) {
+        if (f >= zz) break;
+        const uint32_t u = uu[f];
         const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
         if (p >= 0 && p < P) atomicAdd(&acc[p], (u >> 31) ? -a : a);
       }
@@ -125,7 +144,7 @@ __global__ __launch_bounds__(256) void k_right_all_sparse(const int64_t* rowptr,
   }
 }
 
-template <int PCH>
+template <int PCH, int ZC>
 __global__ __launch_bounds__(256) void k_right_all_ell(const int64_t* rowptr, const int32_t* col, const double* val,
                                                        int64_t rows, const uint32_t* ell, int z, double mag,
                                                        int64_t p0, int64_t P, double eta, double nu, double* Y,
@@ -143,9 +162,13 @@ __global__ __launch_bounds__(256) void k_right_all_ell(const int64_t* rowptr, co
   for (int64_t e = e0 + tid; e < e1; e += 256) {
     while (rp[r + 1] <= e) ++r;
     const double a = val[e] * mag;
-    const uint32_t* me = ell + (int64_t)col[e] * z;
-    for (int f = 0; f < z; ++f) {
-      const uint32_t u = me[f];
+    uint32_t uu[16];
+    ell_row<ZC>(ell, (int64_t)col[e], z, uu);
+    const int zz = ZC ? ZC : z;
+#pragma unroll
+    for (int f = 0; f < (ZC ? ZC : 16); ++f) {
+      if (f >= zz) break;
+      const uint32_t u = uu[f];
       const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
       if (p >= 0 && p < P) atomicAdd(&acc[p * kPad + r], (u >> 31) ? -a : a);
     }
@@ -233,8 +256,12 @@ void left_sparse(hipStream_t st, const CscWindow& H, const Map& M, int64_t P, co
   const int64_t chunk = M.ell_zeta > 0 ? 1024 : 4096;
   for (int64_t p0 = 0; p0 < P; p0 += chunk) {
     const int64_t pc = std::min<int64_t>(chunk, P - p0);
-    if (M.ell_zeta > 0)
-      hipLaunchKernelGGL(k_left_all_ell, dim3(nblocks(H.cols, 4)), dim3(256), (size_t)4 * pc * sizeof(double), st,
+    if (M.ell_zeta == 8)
+      hipLaunchKernelGGL(k_left_all_ell<8>, dim3(nblocks(H.cols, 4)), dim3(256), (size_t)4 * pc * sizeof(double), st,
+                         H.ptr, H.row, H.val, H.cols, M.ell.as<uint32_t>(), M.ell_zeta, M.ell_mag, cw, window, p0, pc,
+                         out, ldo);
+    else if (M.ell_zeta > 0)
+      hipLaunchKernelGGL(k_left_all_ell<0>, dim3(nblocks(H.cols, 4)), dim3(256), (size_t)4 * pc * sizeof(double), st,
                          H.ptr, H.row, H.val, H.cols, M.ell.as<uint32_t>(), M.ell_zeta, M.ell_mag, cw, window, p0, pc,
                          out, ldo);
     else
@@ -314,12 +341,18 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
       for (int64_t p0 = 0; p0 < k; p0 += 128) {
         const int64_t pc = std::min<int64_t>(128, k - p0);
         const dim3 grid(nblocks(m, 64));
-        if (Om.ell_zeta > 0 && pc <= 64)
-          hipLaunchKernelGGL(k_right_all_ell<64>, grid, dim3(256), 0, st, rowptr, col, val, m, Om.ell.as<uint32_t>(),
-                             Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
+        if (Om.ell_zeta == 8 && pc <= 64)
+          hipLaunchKernelGGL((k_right_all_ell<64, 8>), grid, dim3(256), 0, st, rowptr, col, val, m,
+                             Om.ell.as<uint32_t>(), Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
+        else if (Om.ell_zeta == 8)
+          hipLaunchKernelGGL((k_right_all_ell<128, 8>), grid, dim3(256), 0, st, rowptr, col, val, m,
+                             Om.ell.as<uint32_t>(), Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
+        else if (Om.ell_zeta > 0 && pc <= 64)
+          hipLaunchKernelGGL((k_right_all_ell<64, 0>), grid, dim3(256), 0, st, rowptr, col, val, m,
+                             Om.ell.as<uint32_t>(), Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
         else if (Om.ell_zeta > 0)
-          hipLaunchKernelGGL(k_right_all_ell<128>, grid, dim3(256), 0, st, rowptr, col, val, m, Om.ell.as<uint32_t>(),
-                             Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
+          hipLaunchKernelGGL((k_right_all_ell<128, 0>), grid, dim3(256), 0, st, rowptr, col, val, m,
+                             Om.ell.as<uint32_t>(), Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
         else if (pc <= 64)
           hipLaunchKernelGGL(k_right_all_sparse<64>, grid, dim3(256), 0, st, rowptr, col, val, m, Ot.rp(), Ot.ci(),
                              Ot.v(), p0, pc, eta, 0.0, Y, m);

@@ -1181,70 +1181,103 @@ void prepare_lowp_maps(Sketch& sk) {
   SKEMA_HIP(hipStreamSynchronize(st));
 }
 
-void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows) {
+namespace {
+
+// Window scratch layout: L (column-major, ld ldl = rows padded), R
+// (row-major, ld ldr = n padded), for fp8 the row and column max |x| of the
+// window (floats) or the E8M0 block scales, and G (rows x s, column-major,
+// ld ldl) in the core product's element type.
+struct WinLayout {
+  bool fp8, split, f32, mx;
+  int core_prec, gtype;
+  int64_t ldl, ldr, ldls, ldrs;
+  size_t esz, planes, gsz, le, re, ge;
+  size_t o_l, o_r, o_rmax, o_cmax, o_g, o_ls, o_rs, total;
+};
+WinLayout win_layout(const Sketch& sk, int64_t row0, int64_t rows) {
+  WinLayout w{};
+  const int prec = sk.p.precision;
+  const int64_t n = sk.p.n, s = sk.s;
+  w.fp8 = prec == SKEMA_PREC_FP8;
+  w.split = prec == SKEMA_PREC_BF16X3;
+  w.f32 = prec == SKEMA_PREC_FP32;
+  // fp8 windows that start on a 16-row boundary (every window of a multiple-
+  // of-16 window size) take the block-scaled form: one read of A_w, E8M0
+  // scales per 32-element block of L's columns and R's rows; others take the
+  // two-pass form with one scale per row and per column.
+  w.mx = w.fp8 && (row0 % 16) == 0;
+  w.core_prec = w.f32 ? SKEMA_PREC_FP32 : (w.split ? SKEMA_PREC_BF16X3 : SKEMA_PREC_BF16);
+  w.gtype = w.f32 ? 3 : (w.split ? 2 : 1);
+  w.gsz = w.f32 ? 4 : 2;
+  w.ldl = cdiv(rows, 128) * 128;
+  w.ldr = cdiv(n, 128) * 128;
+  w.esz = w.fp8 ? 1 : (w.f32 ? 4 : 2);
+  w.planes = w.split ? 2 : 1;
+  // scale rows: 4 groups of ceil(steps / 4) * 4 bytes (steps of 128 elements)
+  w.ldls = 4 * align_up((size_t)cdiv(rows, 128), 4);
+  w.ldrs = 4 * align_up((size_t)cdiv(n, 128), 4);
+  w.le = (size_t)n * w.ldl;
+  w.re = (size_t)rows * w.ldr;
+  w.ge = (size_t)s * w.ldl;
+  w.o_l = 0;
+  w.o_r = align_up(w.le * w.esz * w.planes, 256);
+  w.o_rmax = w.o_r + align_up(w.re * w.esz * w.planes, 256);
+  w.o_cmax = w.o_rmax + (w.fp8 ? align_up((size_t)rows * 4, 256) : 0);
+  w.o_g = w.o_cmax + (w.fp8 ? align_up((size_t)n * 4, 256) : 0);
+  w.o_ls = w.o_g + align_up(w.ge * w.gsz * w.planes, 256);
+  w.o_rs = w.o_ls + (w.mx ? align_up((size_t)n * w.ldls, 256) : 0);
+  w.total = align_up(w.o_rs + (w.mx ? align_up((size_t)rows * w.ldrs, 256) : 0), 256);
+  return w;
+}
+
+// The quantise pass of one window into `base` on stream st.
+void quantize_window(Sketch& sk, const double* H, int64_t ldh, int64_t rows, const WinLayout& w, char* base,
+                     hipStream_t st) {
+  const int64_t n = sk.p.n;
+  float* rmax = (float*)(base + w.o_rmax);
+  float* cmax = (float*)(base + w.o_cmax);
+  Timed t(sk.ctx, "window", st);
+  dim3 grid((unsigned)cdiv(rows, kQ), (unsigned)cdiv(n, kQ));
+  if (w.mx) {
+    // unit scales (0x7f) wherever no block writes one: K padding past the window
+    fill_i32(st, (int*)(base + w.o_ls), (int64_t)(align_up((size_t)n * w.ldls, 256) / 4), 0x7f7f7f7f);
+    fill_i32(st, (int*)(base + w.o_rs), (int64_t)(align_up((size_t)rows * w.ldrs, 256) / 4), 0x7f7f7f7f);
+    dim3 gmx((unsigned)cdiv(rows, kMxR), (unsigned)cdiv(n, kMxC));
+    hipLaunchKernelGGL(k_quant_fp8_mx, gmx, dim3(kThreads), 0, st, H, ldh, rows, n, (uint8_t*)(base + w.o_l), w.ldl,
+                       (uint8_t*)(base + w.o_ls), w.ldls, (uint8_t*)(base + w.o_r), w.ldr, (uint8_t*)(base + w.o_rs),
+                       w.ldrs);
+  } else if (w.fp8) {
+    fill_i32(st, (int*)rmax, rows, 0);
+    fill_i32(st, (int*)cmax, n, 0);
+    hipLaunchKernelGGL(k_absmax_fp8, grid, dim3(kThreads), 0, st, H, ldh, rows, n, (unsigned*)rmax, (unsigned*)cmax);
+    hipLaunchKernelGGL(k_quant_fp8, grid, dim3(kThreads), 0, st, H, ldh, rows, n, (const float*)rmax,
+                       (const float*)cmax, (uint8_t*)(base + w.o_l), w.ldl, (uint8_t*)(base + w.o_r), w.ldr);
+  } else if (w.f32) {
+    hipLaunchKernelGGL((k_quant<float, false>), grid, dim3(kThreads), 0, st, H, ldh, rows, n, (float*)(base + w.o_l),
+                       (float*)nullptr, w.ldl, (float*)(base + w.o_r), (float*)nullptr, w.ldr);
+  } else if (w.split) {
+    __bf16* L = (__bf16*)(base + w.o_l);
+    __bf16* R = (__bf16*)(base + w.o_r);
+    hipLaunchKernelGGL((k_quant<__bf16, true>), grid, dim3(kThreads), 0, st, H, ldh, rows, n, L, L + w.le, w.ldl, R,
+                       R + w.re, w.ldr);
+  } else {
+    hipLaunchKernelGGL((k_quant<__bf16, false>), grid, dim3(kThreads), 0, st, H, ldh, rows, n, (__bf16*)(base + w.o_l),
+                       (__bf16*)nullptr, w.ldl, (__bf16*)(base + w.o_r), (__bf16*)nullptr, w.ldr);
+  }
+  check_launch("k_quant");
+}
+
+// The three window products from the quantised copies in `base`, on the
+// sketch's stream.
+void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w, char* base) {
   Ctx* c = sk.ctx;
   hipStream_t st = c->stream;
   const int prec = sk.p.precision;
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n;
   const double eta = sk.p.eta, nu = sk.p.nu;
-  const bool fp8 = prec == SKEMA_PREC_FP8, split = prec == SKEMA_PREC_BF16X3, f32 = prec == SKEMA_PREC_FP32;
-  // window copies: L (column-major, ld ldl = rows padded), R (row-major, ld ldr = n padded);
-  // fp8 adds the row and column max |x| of the window (floats); G (rows x s,
-  // column-major, ld ldl) in the core product's element type
-  const int64_t ldl = cdiv(rows, 128) * 128, ldr = cdiv(n, 128) * 128;
-  const size_t esz = fp8 ? 1 : (f32 ? 4 : 2), planes = split ? 2 : 1;
-  const int core_prec = f32 ? SKEMA_PREC_FP32 : (split ? SKEMA_PREC_BF16X3 : SKEMA_PREC_BF16);
-  const int gtype = f32 ? 3 : (split ? 2 : 1);
-  const size_t gsz = f32 ? 4 : 2;
-  // fp8 windows that start on a 16-row boundary (every window of a multiple-
-  // of-16 window size) take the block-scaled form: one read of A_w, E8M0
-  // scales per 32-element block of L's columns and R's rows; others take the
-  // two-pass form with one scale per row and per column.
-  const bool mx = fp8 && (row0 % 16) == 0;
-  // scale rows: 4 groups of ceil(steps / 4) * 4 bytes (steps of 128 elements)
-  const int64_t ldls = 4 * align_up((size_t)cdiv(rows, 128), 4), ldrs = 4 * align_up((size_t)cdiv(n, 128), 4);
-  const size_t le = (size_t)n * ldl, re = (size_t)rows * ldr, ge = (size_t)s * ldl;
-  const size_t o_l = 0, o_r = align_up(le * esz * planes, 256), o_rmax = o_r + align_up(re * esz * planes, 256),
-               o_cmax = o_rmax + (fp8 ? align_up((size_t)rows * 4, 256) : 0),
-               o_g = o_cmax + (fp8 ? align_up((size_t)n * 4, 256) : 0),
-               o_ls = o_g + align_up(ge * gsz * planes, 256),
-               o_rs = o_ls + (mx ? align_up((size_t)n * ldls, 256) : 0),
-               total = o_rs + (mx ? align_up((size_t)rows * ldrs, 256) : 0);
-  sk.win.reserve(total);
-  char* base = sk.win.as<char>();
-  float* rmax = (float*)(base + o_rmax);
-  float* cmax = (float*)(base + o_cmax);
-  {
-    Timed t(c, "window");
-    dim3 grid((unsigned)cdiv(rows, kQ), (unsigned)cdiv(n, kQ));
-    if (mx) {
-      // unit scales (0x7f) wherever no block writes one: K padding past the window
-      fill_i32(st, (int*)(base + o_ls), (int64_t)(align_up((size_t)n * ldls, 256) / 4), 0x7f7f7f7f);
-      fill_i32(st, (int*)(base + o_rs), (int64_t)(align_up((size_t)rows * ldrs, 256) / 4), 0x7f7f7f7f);
-      dim3 gmx((unsigned)cdiv(rows, kMxR), (unsigned)cdiv(n, kMxC));
-      hipLaunchKernelGGL(k_quant_fp8_mx, gmx, dim3(kThreads), 0, st, H, ldh, rows, n, (uint8_t*)(base + o_l), ldl,
-                         (uint8_t*)(base + o_ls), ldls, (uint8_t*)(base + o_r), ldr, (uint8_t*)(base + o_rs), ldrs);
-    } else if (fp8) {
-      fill_i32(st, (int*)rmax, rows, 0);
-      fill_i32(st, (int*)cmax, n, 0);
-      hipLaunchKernelGGL(k_absmax_fp8, grid, dim3(kThreads), 0, st, H, ldh, rows, n, (unsigned*)rmax,
-                         (unsigned*)cmax);
-      hipLaunchKernelGGL(k_quant_fp8, grid, dim3(kThreads), 0, st, H, ldh, rows, n, (const float*)rmax,
-                         (const float*)cmax, (uint8_t*)(base + o_l), ldl, (uint8_t*)(base + o_r), ldr);
-    } else if (f32) {
-      hipLaunchKernelGGL((k_quant<float, false>), grid, dim3(kThreads), 0, st, H, ldh, rows, n, (float*)(base + o_l),
-                         (float*)nullptr, ldl, (float*)(base + o_r), (float*)nullptr, ldr);
-    } else if (split) {
-      __bf16* L = (__bf16*)(base + o_l);
-      __bf16* R = (__bf16*)(base + o_r);
-      hipLaunchKernelGGL((k_quant<__bf16, true>), grid, dim3(kThreads), 0, st, H, ldh, rows, n, L, L + le, ldl, R,
-                         R + re, ldr);
-    } else {
-      hipLaunchKernelGGL((k_quant<__bf16, false>), grid, dim3(kThreads), 0, st, H, ldh, rows, n,
-                         (__bf16*)(base + o_l), (__bf16*)nullptr, ldl, (__bf16*)(base + o_r), (__bf16*)nullptr, ldr);
-    }
-    check_launch("k_quant");
-  }
+  const bool fp8 = w.fp8, split = w.split, mx = w.mx;
+  float* rmax = (float*)(base + w.o_rmax);
+  float* cmax = (float*)(base + w.o_cmax);
   const size_t ue = (size_t)k * sk.lp_ldups, oe = (size_t)(k + s) * sk.lp_ldomps;
   {
     Timed t(c, "sketch_left");
@@ -1254,11 +1287,11 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
     a.al = split ? (const void*)(sk.lp_ups.as<__bf16>() + ue) : nullptr;
     a.amax_a = fp8 ? (const float*)(sk.lp_ups.as<uint8_t>() + align_up(ue, 256)) : nullptr;
     a.lda = sk.lp_ldups; a.koff_a = row0;
-    a.bh = base + o_l;
-    a.bl = split ? (const void*)((const __bf16*)(base + o_l) + le) : nullptr;
+    a.bh = base + w.o_l;
+    a.bl = split ? (const void*)((const __bf16*)(base + w.o_l) + w.le) : nullptr;
     a.amax_b = (fp8 && !mx) ? cmax : nullptr;
-    if (mx) { a.bsc = (const uint8_t*)(base + o_ls); a.ldbs = ldls; }
-    a.ldb = ldl;
+    if (mx) { a.bsc = (const uint8_t*)(base + w.o_ls); a.ldbs = w.ldls; }
+    a.ldb = w.ldl;
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
     gemm_nt(st, prec, a, &c->gws.buf);
@@ -1267,17 +1300,17 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
     Timed t(c, "sketch_right");
     NtArgs a{};
     a.M = rows; a.N = k + s; a.K = n;
-    a.ah = base + o_r;
-    a.al = split ? (const void*)((const __bf16*)(base + o_r) + re) : nullptr;
+    a.ah = base + w.o_r;
+    a.al = split ? (const void*)((const __bf16*)(base + w.o_r) + w.re) : nullptr;
     a.amax_a = (fp8 && !mx) ? rmax : nullptr;
-    if (mx) { a.asc = (const uint8_t*)(base + o_rs); a.ldas = ldrs; }
-    a.lda = ldr; a.koff_a = 0;
+    if (mx) { a.asc = (const uint8_t*)(base + w.o_rs); a.ldas = w.ldrs; }
+    a.lda = w.ldr; a.koff_a = 0;
     a.bh = sk.lp_omps.p;
     a.bl = split ? (const void*)(sk.lp_omps.as<__bf16>() + oe) : nullptr;
     a.amax_b = fp8 ? (const float*)(sk.lp_omps.as<uint8_t>() + align_up(oe, 256)) : nullptr;
     a.ldb = sk.lp_ldomps;
     a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
-    a.gtype = gtype; a.gh = base + o_g; a.gl = base + o_g + ge * gsz; a.ldg = ldl;
+    a.gtype = w.gtype; a.gh = base + w.o_g; a.gl = base + w.o_g + w.ge * w.gsz; a.ldg = w.ldl;
     gemm_nt(st, prec, a, &c->gws.buf);
   }
   {
@@ -1288,12 +1321,68 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
     a.ah = sk.lp_phi.p;
     a.al = split ? (const void*)(sk.lp_phi.as<__bf16>() + pe) : nullptr;
     a.lda = sk.lp_ldups; a.koff_a = row0;
-    a.bh = base + o_g;
-    a.bl = split ? (const void*)(base + o_g + ge * gsz) : nullptr;
-    a.ldb = ldl;
+    a.bh = base + w.o_g;
+    a.bl = split ? (const void*)(base + w.o_g + w.ge * w.gsz) : nullptr;
+    a.ldb = w.ldl;
     a.out.C1 = sk.Z.as<double>(); a.out.ldc1 = s; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = s;
     a.out.C2 = a.out.C1; a.out.ldc2 = s;
-    gemm_nt(st, core_prec, a, &c->gws.buf);
+    gemm_nt(st, w.core_prec, a, &c->gws.buf);
+  }
+}
+
+}  // namespace
+
+void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows) {
+  const WinLayout w = win_layout(sk, row0, rows);
+  sk.win.reserve(w.total);
+  char* base = sk.win.as<char>();
+  quantize_window(sk, H, ldh, rows, w, base, sk.ctx->stream);
+  window_products(sk, row0, rows, w, base);
+}
+
+void linear_update_dense_lowp(Sketch& sk, const double* A, int64_t lda) {
+  Ctx* c = sk.ctx;
+  hipStream_t st = c->stream;
+  const int64_t m = sk.p.m;
+  const int64_t wsz = (sk.p.window >= m) ? m : sk.p.window;
+  const int64_t nw = cdiv(m, wsz);
+  if (nw < 2) {
+    update_dense_window_lowp(sk, A, lda, 0, m);
+    return;
+  }
+  // Two window slots: the quantise pass of window w + 1 runs on the side
+  // stream while window w's products run on the sketch's stream (it is
+  // HBM-bound, they are matrix-core and L2 bound).  Slot s is rewritten only
+  // after the products that read it are done (ev_done), and products start
+  // only once their slot is filled (ev_quant).
+  size_t slot = 0;
+  for (int64_t r0 = 0; r0 < m; r0 += wsz) slot = std::max(slot, win_layout(sk, r0, std::min(wsz, m - r0)).total);
+  sk.win.reserve(2 * slot);
+  char* bases[2] = {sk.win.as<char>(), sk.win.as<char>() + slot};
+  hipStream_t side = c->side_stream();
+  hipEvent_t ev_quant[2] = {c->event(0), c->event(1)}, ev_done[2] = {c->event(2), c->event(3)};
+  hipEvent_t ev_start = c->event(4);
+  SKEMA_HIP(hipEventRecord(ev_start, st));  // the state clear and anything before this call
+  SKEMA_HIP(hipStreamWaitEvent(side, ev_start, 0));
+  const WinLayout w0 = win_layout(sk, 0, std::min(wsz, m));
+  quantize_window(sk, A, lda, std::min(wsz, m), w0, bases[0], side);
+  SKEMA_HIP(hipEventRecord(ev_quant[0], side));
+  int64_t wi = 0;
+  for (int64_t r0 = 0; r0 < m; r0 += wsz, ++wi) {
+    const int64_t rows = std::min(wsz, m - r0);
+    const int sl = (int)(wi & 1);
+    const int64_t r1 = r0 + wsz;
+    if (r1 < m) {
+      const int ns = sl ^ 1;
+      if (wi >= 1) SKEMA_HIP(hipStreamWaitEvent(side, ev_done[ns], 0));
+      const int64_t rows1 = std::min(wsz, m - r1);
+      const WinLayout w1 = win_layout(sk, r1, rows1);
+      quantize_window(sk, A + r1, lda, rows1, w1, bases[ns], side);
+      SKEMA_HIP(hipEventRecord(ev_quant[ns], side));
+    }
+    SKEMA_HIP(hipStreamWaitEvent(st, ev_quant[sl], 0));
+    window_products(sk, r0, rows, win_layout(sk, r0, rows), bases[sl]);
+    SKEMA_HIP(hipEventRecord(ev_done[sl], st));
   }
 }
 

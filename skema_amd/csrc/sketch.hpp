@@ -37,6 +37,26 @@ struct Ctx {
   void sync_if_needed() {
     if (!async_calls) SKEMA_HIP(hipStreamSynchronize(stream));
   }
+  // A second stream (default priority) for work overlapped with `stream`,
+  // and a few ordering events; created on first use.
+  hipStream_t side = nullptr;
+  hipEvent_t events[8] = {};
+  hipStream_t side_stream() {
+    if (!side) SKEMA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    return side;
+  }
+  hipEvent_t event(int i) {
+    if (!events[i]) SKEMA_HIP(hipEventCreateWithFlags(&events[i], hipEventDisableTiming));
+    return events[i];
+  }
+  ~Ctx() {
+    for (hipEvent_t& e : events)
+      if (e) (void)hipEventDestroy(e);
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+  }
 };
 
 // RAII timing scope around a group of launches.
@@ -44,17 +64,19 @@ struct Timed {
   Ctx* c;
   std::string name;
   hipEvent_t a = nullptr;
-  Timed(Ctx* ctx, const char* n) : c(ctx), name(n) {
+  hipStream_t st = nullptr;
+  Timed(Ctx* ctx, const char* n) : Timed(ctx, n, ctx->stream) {}
+  Timed(Ctx* ctx, const char* n, hipStream_t s) : c(ctx), name(n), st(s) {
     if (c->profiling) {
       a = c->stats.get();
-      SKEMA_HIP(hipEventRecord(a, c->stream));
+      SKEMA_HIP(hipEventRecord(a, st));
     }
   }
   // Closes the interval now (else at scope exit).
   void stop() {
     if (a) {
       hipEvent_t b = c->stats.get();
-      (void)hipEventRecord(b, c->stream);
+      (void)hipEventRecord(b, st);
       c->stats.open[name].push_back({a, b});
       a = nullptr;
     }

@@ -1,0 +1,235 @@
+// SketchySPD (src/Skema_SketchySVD.cpp:757-1151) on one GPU.
+//
+// Update: Y[idx] = nu Y[idx] + eta A_w Omega for every row window (the
+// reference's axpy, :1153-1200; its window loop never resets the `pass`
+// counter, :795-858, so it sketches window 0 only -- not reproduced).
+// Products: dense A_w on the FP64 GEMM (or A_w Omega with a sparse Omega
+// through the dense x CSR^T kernel), CSR A_w through the row products, and
+// the Gaussian RBF window through the fused generator (K never written).
+//
+// Fixed-rank approximation (Tropp, Yurtsever, Udell, Cevher 2017, Alg. 3, as
+// the reference orders it, :934-1151):
+//   nrm = ||Y||_2            (largest singular value: sqrt of the top eigenvalue of Y^T Y)
+//   shift = eps * nrm,  Y_shift = Y + shift Omega
+//   B = Omega^T Y_shift,  C = (B + B^T) / 2,  C = R^T R (upper Cholesky)
+//   E = Y R^{-1}             (the reference solves with the unshifted Y)
+//   E = U S V^T,  U <- U[:, :r],  lambda = max(0, S^2 - shift)
+// With shift_removal = 1 the last step subtracts nrm instead, as the
+// reference does (:1141-1145: its local `nu` holds ||Y||_2).
+#include <cmath>
+#include <vector>
+
+#include "gemm.hpp"
+#include "linalg.hpp"
+#include "rbf.hpp"
+#include "rbf_fused.hpp"
+#include "sparse.hpp"
+#include "spd.hpp"
+#include "util.hpp"
+
+namespace skema {
+namespace {
+
+// Dense copy of a CSR (rows x cols, column-major, ld); duplicates add up.
+__global__ void k_csr_to_dense(int64_t rows, const int64_t* rp, const int32_t* ci, const double* v, double* D,
+                               int64_t ld) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  for (int64_t e = rp[r]; e < rp[r + 1]; ++e) D[r + (int64_t)ci[e] * ld] += v[e];
+}
+
+// C = (B + B^T) / 2 (k x k, ld k)
+__global__ void k_sym_half(int64_t k, const double* B, double* C) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k * k) return;
+  const int64_t i = e % k, j = e / k;
+  C[e] = 0.5 * (B[i + j * k] + B[j + i * k]);
+}
+
+// In-place upper Cholesky C = R^T R (k x k, ld k) in one block, right
+// looking: row j of R is scaled by its pivot, then the trailing upper
+// triangle loses the outer product of that row.  The strict lower triangle
+// is zeroed (the reference's chol, src/Skema_BlasLapack.hpp:245-272).
+// info = j + 1 when the pivot of row j is not positive.
+__global__ __launch_bounds__(1024) void k_chol_upper(int64_t k, double* C, int* info) {
+  __shared__ double piv;
+  __shared__ int bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int64_t j = 0; j < k; ++j) {
+    if (tid == 0) {
+      const double d = C[j + j * k];
+      if (!(d > 0.0)) {
+        bad = (int)(j + 1);
+        piv = 1.0;
+      } else {
+        piv = sqrt(d);
+        C[j + j * k] = piv;
+      }
+    }
+    __syncthreads();
+    if (bad) break;
+    const double p = piv;
+    for (int64_t i = j + 1 + tid; i < k; i += 1024) C[j + i * k] /= p;
+    __syncthreads();
+    // trailing update of the upper triangle: C[a, b] -= R[j, a] R[j, b], j < a <= b
+    const int64_t t = k - j - 1;
+    for (int64_t e = tid; e < t * t; e += 1024) {
+      const int64_t a = j + 1 + e % t, b = j + 1 + e / t;
+      if (a <= b) C[a + b * k] -= C[j + a * k] * C[j + b * k];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) *info = bad;
+  for (int64_t e = tid; e < k * k; e += 1024) {
+    const int64_t i = e % k, j = e / k;
+    if (i > j) C[e] = 0.0;
+  }
+}
+
+// out = Y + a Om elementwise (n x k, both ld n)
+__global__ void k_axpy_mat(int64_t total, const double* Y, double a, const double* Om, double* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) out[e] = Y[e] + a * Om[e];
+}
+
+__global__ void k_identity(int64_t k, double* I) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= k * k) return;
+  I[e] = (e % k == e / k) ? 1.0 : 0.0;
+}
+
+// lambda_c = max(0, S_c^2 - sub) for c < r
+__global__ void k_remove_shift(int64_t r, const double* S, double sub, double* out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= r) return;
+  const double v = S[c] * S[c] - sub;
+  out[c] = v > 0.0 ? v : 0.0;
+}
+
+unsigned g256(int64_t n) { return (unsigned)std::max<int64_t>(1, cdiv(n, 256)); }
+
+GemmOut y_out(Spd& sp, int64_t row0) {
+  GemmOut o;
+  o.C1 = sp.Y.as<double>() + row0; o.ldc1 = sp.p.m; o.alpha1 = sp.p.eta; o.beta1 = sp.p.nu; o.nsplit = sp.k;
+  o.C2 = o.C1; o.ldc2 = sp.p.m; o.alpha2 = sp.p.eta; o.beta2 = sp.p.nu;
+  return o;
+}
+
+}  // namespace
+
+void spd_prepare(Spd& sp) {
+  hipStream_t st = sp.ctx->stream;
+  Map& M = *sp.omega;
+  const int64_t n = sp.p.n, k = sp.k;
+  if (M.sparse) {
+    sp.om.alloc((size_t)n * k * sizeof(double));
+    fill_f64(st, sp.om.as<double>(), n * k, 0.0);
+    const Csr& L = M.logical_csr();  // n x k
+    hipLaunchKernelGGL(k_csr_to_dense, dim3(g256(L.rows)), dim3(256), 0, st, L.rows, L.rp(), L.ci(), L.v(),
+                       sp.om.as<double>(), n);
+    check_launch("k_csr_to_dense");
+    sp.omL = sp.om.as<double>();
+  } else if (M.transposed) {
+    sp.om.alloc((size_t)n * k * sizeof(double));
+    transpose(st, M.nrow, M.ncol, M.dense.as<double>(), M.nrow, sp.om.as<double>(), n);
+    sp.omL = sp.om.as<double>();
+  } else {
+    sp.omL = M.dense.as<double>();
+  }
+  sp.omT.alloc((size_t)k * n * sizeof(double));
+  transpose(st, n, k, sp.omL, n, sp.omT.as<double>(), k);
+}
+
+void spd_update_dense(Spd& sp, const double* H, int64_t ldh, int64_t row0, int64_t rows) {
+  Ctx* c = sp.ctx;
+  Timed t(c, "spd_update");
+  const GemmOut o = y_out(sp, row0);
+  if (sp.omega->sparse) spmm_dense_csrT(c->stream, rows, H, ldh, sp.omega->logical_csr_t(), o, &c->gws.buf);
+  else gemm_f64(c->stream, 'N', 'N', rows, sp.k, sp.p.n, H, ldh, sp.omL, sp.p.n, o, &c->gws);
+}
+
+void spd_update_csr(Spd& sp, const int64_t* rowptr, const int32_t* col, const double* val, int64_t row0,
+                    int64_t rows) {
+  Ctx* c = sp.ctx;
+  Timed t(c, "spd_update");
+  const GemmOut o = y_out(sp, row0);
+  if (sp.omega->sparse) csr_right_sparse(c->stream, rowptr, col, val, rows, sp.omega->logical_csr(), sp.k, o);
+  else csr_right_dense(c->stream, rowptr, col, val, rows, sp.omT.as<double>(), sp.k, sp.k, o);
+}
+
+void spd_update_rbf(Spd& sp, const double* points, int64_t ldp, int64_t row0, int64_t rows) {
+  Ctx* c = sp.ctx;
+  hipStream_t st = c->stream;
+  Timed t(c, "spd_update");
+  const int64_t n = sp.p.n;
+  sp.win.reserve((size_t)n * sizeof(double));
+  double* nrm = sp.win.as<double>();
+  row_norms2(st, points, n, sp.p.nfeat, ldp, nrm);
+  RbfArgs a;
+  a.X = points; a.ldp = ldp; a.d = sp.p.nfeat; a.nrm = nrm; a.gamma = sp.p.gamma; a.row0 = row0;
+  a.M = rows; a.N = sp.k; a.K = n; a.OP = sp.omT.as<double>(); a.ldop = sp.k;
+  a.out = y_out(sp, row0);
+  rbf_fused_right_f64(st, a, &c->gws.buf);
+}
+
+void spd_low_rank_approx(Spd& sp, int shift_removal, double* U, int64_t ldu, double* S) {
+  Ctx* c = sp.ctx;
+  hipStream_t st = c->stream;
+  const int64_t n = sp.p.n, k = sp.k, r = sp.p.rank;
+  const double* Y = sp.Y.as<double>();
+  DevBuf G((size_t)k * k * sizeof(double)), Ug((size_t)k * k * sizeof(double)), Vg((size_t)k * k * sizeof(double)),
+      Sg((size_t)k * sizeof(double)), Yv((size_t)n * k * sizeof(double)), Cm((size_t)k * k * sizeof(double)),
+      Ri((size_t)k * k * sizeof(double)), E((size_t)n * k * sizeof(double)), Q((size_t)n * k * sizeof(double)),
+      tau((size_t)k * sizeof(double)), Re((size_t)k * k * sizeof(double)), Ur((size_t)k * k * sizeof(double)),
+      Vr((size_t)k * k * sizeof(double)), Se((size_t)k * sizeof(double)), info(sizeof(int)), scratch;
+  double nrm = 0.0;
+  {
+    Timed t(c, "spd_norm2");
+    // ||Y||_2^2 = largest eigenvalue of Y^T Y (the Jacobi SVD of the Gram)
+    gemm_f64(st, 'T', 'N', k, k, n, 1.0, Y, n, Y, n, 0.0, G.as<double>(), k, &c->gws);
+    jacobi_svd(st, k, G.as<double>(), k, Ug.as<double>(), k, Sg.as<double>(), Vg.as<double>(), k, &scratch);
+    double top = 0.0;
+    SKEMA_HIP(hipMemcpyAsync(&top, Sg.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    SKEMA_HIP(hipStreamSynchronize(st));
+    nrm = std::sqrt(std::max(top, 0.0));
+  }
+  const double shift = std::numeric_limits<double>::epsilon() * nrm;
+  {
+    Timed t(c, "spd_core");
+    // Y_shift = Y + shift Omega; B = Omega^T Y_shift; C = (B + B^T) / 2
+    hipLaunchKernelGGL(k_axpy_mat, dim3(g256(n * k)), dim3(256), 0, st, n * k, Y, shift, sp.omL, Yv.as<double>());
+    check_launch("k_axpy_mat");
+    gemm_f64(st, 'T', 'N', k, k, n, 1.0, sp.omL, n, Yv.as<double>(), n, 0.0, G.as<double>(), k, &c->gws);
+    hipLaunchKernelGGL(k_sym_half, dim3(g256(k * k)), dim3(256), 0, st, k, G.as<double>(), Cm.as<double>());
+    check_launch("k_sym_half");
+    hipLaunchKernelGGL(k_chol_upper, dim3(1), dim3(1024), 0, st, k, Cm.as<double>(), info.as<int>());
+    check_launch("k_chol_upper");
+    int h = 0;
+    SKEMA_HIP(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    SKEMA_HIP(hipStreamSynchronize(st));
+    SKEMA_REQUIRE(h == 0, kErrNumeric,
+                  "spd low_rank_approx: (B + B^T)/2 is not positive definite (leading minor " + std::to_string(h) +
+                      ")");
+    // E = Y R^{-1}: R^{-1} by back-substitution on the identity, then one GEMM
+    hipLaunchKernelGGL(k_identity, dim3(g256(k * k)), dim3(256), 0, st, k, Ri.as<double>());
+    check_launch("k_identity");
+    trsm_upper(st, k, k, Cm.as<double>(), k, Ri.as<double>(), k);
+    gemm_f64(st, 'N', 'N', n, k, k, 1.0, Y, n, Ri.as<double>(), k, 0.0, E.as<double>(), n, &c->gws);
+  }
+  {
+    Timed t(c, "spd_svd");
+    // thin SVD of E: Householder QR, Jacobi SVD of R, U = Q U_R
+    householder_qr(st, n, k, E.as<double>(), n, tau.as<double>());
+    extract_r(st, k, E.as<double>(), n, Re.as<double>(), k);
+    householder_q(st, n, k, E.as<double>(), n, tau.as<double>(), Q.as<double>(), n);
+    jacobi_svd(st, k, Re.as<double>(), k, Ur.as<double>(), k, Se.as<double>(), Vr.as<double>(), k, &scratch);
+    gemm_f64(st, 'N', 'N', n, r, k, 1.0, Q.as<double>(), n, Ur.as<double>(), k, 0.0, U, ldu, &c->gws);
+    hipLaunchKernelGGL(k_remove_shift, dim3(g256(r)), dim3(256), 0, st, r, Se.as<double>(),
+                       shift_removal == 1 ? nrm : shift, S);
+    check_launch("k_remove_shift");
+  }
+}
+
+}  // namespace skema

@@ -430,6 +430,90 @@ class SketchySVD:
             pass
 
 
+class SketchySPD:
+    """SketchySPD<M,D> (src/Skema_SketchySVD.cpp:757-1151) on one GPU: the
+    Nystrom sketch Y = A Omega of a symmetric PSD matrix (m == n) and its
+    fixed-rank approximation (U, lambda).  Every window is sketched (the
+    reference's loop stops after window 0, :795-858); ``shift_removal="eps"``
+    removes the eps ||Y||_2 shift, ``"reference"`` subtracts ||Y||_2 as the
+    reference does (:1141-1145)."""
+
+    SHIFTS = {"eps": 0, "reference": 1}
+
+    def __init__(self, params: AlgParams, omega: Optional[_DimRedux] = None, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.params = params
+        self.rank = params.rank
+        self.n = params.matrix_n
+        self.range = 4 * params.rank + 1 if params.sketch_range < params.rank else params.sketch_range
+        if omega is None:  # DimReduxT(ncol, range, seeds[0]) (:768-773)
+            if params.dim_redux == "gauss":
+                omega = GaussDimRedux(self.n, self.range, seed=params.seeds[0], ctx=self.ctx)
+            else:
+                omega = SparseSignDimRedux(self.n, self.range, seed=params.seeds[0], ctx=self.ctx)
+        self.omega = omega
+        h = C.c_void_p()
+        cp = params.to_c()
+        L.call("skema_spd_create", self.ctx.h, C.byref(cp), omega.h, C.byref(h))
+        self.h = h
+
+    def linear_update(self, A) -> None:
+        """src/Skema_SketchySVD.cpp:790-880 (zeroes Y first)."""
+        if isinstance(A, CrsMatrix):
+            L.call("skema_spd_linear_update_csr", self.h, _ptr(A.rowptr), _ptr(A.col), _ptr(A.val))
+        elif self.params.kernel_func == "gaussrbf":
+            L.call("skema_spd_linear_update_rbf", self.h, _ptr(A), _ld(A))
+        else:
+            L.call("skema_spd_linear_update_dense", self.h, _ptr(A), _ld(A))
+
+    def update(self, A, idx: Tuple[int, int]) -> None:
+        r0, rows = idx[0], idx[1] - idx[0]
+        if isinstance(A, CrsMatrix):
+            L.call("skema_spd_update_csr", self.h, _ptr(A.rowptr) + 8 * r0, _ptr(A.col), _ptr(A.val), r0, rows)
+        elif self.params.kernel_func == "gaussrbf":
+            L.call("skema_spd_update_rbf", self.h, _ptr(A), _ld(A), r0, rows)
+        else:
+            L.call("skema_spd_update_dense", self.h, _ptr(A) + 8 * r0, _ld(A), r0, rows)
+
+    def state(self) -> np.ndarray:
+        Y = np.empty((self.range, self.params.matrix_m))
+        L.call("skema_spd_get_state", self.h, Y.ctypes.data)
+        return Y.T.copy()
+
+    def low_rank_approx(self, device: bool = False, shift_removal: str = "eps"):
+        """src/Skema_SketchySVD.cpp:934-1151 -> (U n x r, lambda r)."""
+        n, r = self.n, self.rank
+        sh = self.SHIFTS[shift_removal]
+        if device:
+            U = torch.empty((r, n), dtype=torch.float64, device="cuda").t()
+            S = torch.empty(r, dtype=torch.float64, device="cuda")
+            L.call("skema_spd_low_rank_approx_device", self.h, sh, _ptr(U), _ld(U), _ptr(S))
+            return U, S
+        U = np.empty((r, n)); S = np.empty(r)
+        L.call("skema_spd_low_rank_approx", self.h, sh, U.ctypes.data, S.ctypes.data)
+        return U.T.copy(), S
+
+    def compute_residuals(self, A, U=None, S=None) -> np.ndarray:
+        """Eigenpair residual norms (residuals(A, V, S, ...),
+        src/Skema_Residuals.hpp:123-158) of a dense A or an RBF point set."""
+        if U is None:
+            U, S = self.low_rank_approx(device=True)
+        gamma = self.params.kernel_gamma if self.params.kernel_func == "gaussrbf" else None
+        self.rnrms = residual_norms_sym(A, U, S, window=self.params.window, gamma=gamma, ctx=self.ctx)
+        return self.rnrms
+
+    def close(self) -> None:
+        if self.h:
+            L.call("skema_spd_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def residual_fro(A, U, S, V, window: int = 4096, ctx: Optional[Context] = None) -> Tuple[float, float]:
     """(||A - U diag(S) V^T||_F^2, ||A||_F^2) streamed over row windows (device
     tensors, column-major)."""

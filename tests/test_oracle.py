@@ -437,3 +437,77 @@ def test_residuals_sym_eigenpairs():
     assert O.residual_norms_sym(A, V, S, window=8).max() <= 1e-11 * S[0]
     d = np.array([0.0, 1e-2, -3.0, 0.25])
     assert np.allclose(O.residual_norms_sym(A, V, S + d), np.abs(d), rtol=1e-9, atol=1e-11 * S[0])
+
+
+# --------------------------------------------------------------------------
+# SketchySPD (src/Skema_SketchySVD.cpp:757-1151), pinned by the reference's
+# own symmetric and kernel fixtures (exact recovery at full range), the
+# closed form Y = eta A Omega, exact recovery of a rank-5 PSD matrix, and the
+# reference's shift removal written out directly.
+# --------------------------------------------------------------------------
+def test_spd_symmetric_fixture_exact_recovery(golden_dir):
+    A = np.loadtxt(f"{golden_dir}/symmetric_dense.txt")
+    sv = np.loadtxt(f"{golden_dir}/symmetric_svals.txt")
+    p = O.AlgParams(matrix_m=27, matrix_n=27, rank=27, sketch_range=27, window=4)
+    s = O.SketchySPD(p)
+    s.linear_update(A)
+    U, lam = s.low_rank_approx()
+    assert np.abs(lam - sv).max() <= 1e-12 * sv[0]
+    assert O.residual_norms_sym(A, U, lam).max() <= 1e-12 * sv[0]
+
+
+def test_spd_kernel_fixture_exact_recovery(golden_dir):
+    X = np.loadtxt(f"{golden_dir}/kernel_data.txt")
+    ks = np.loadtxt(f"{golden_dir}/kernel_svals.txt")
+    n = X.shape[0]
+    p = O.AlgParams(matrix_m=n, matrix_n=n, rank=n, sketch_range=n, window=7, kernel_func="gaussrbf",
+                    kernel_gamma=1.0)
+    s = O.SketchySPD(p)
+    s.linear_update(X)
+    _, lam = s.low_rank_approx()
+    assert np.abs(lam - ks[:n]).max() <= 1e-12 * ks[0]
+
+
+@pytest.mark.parametrize("window", [1, 7, 40])
+def test_spd_sketch_closed_form(window):
+    rng = np.random.default_rng(21)
+    B = rng.standard_normal((40, 40))
+    A = B @ B.T
+    p = O.AlgParams(matrix_m=40, matrix_n=40, rank=3, window=window, sketch_eta=1.7, sketch_nu=0.4)
+    s = O.SketchySPD(p)
+    s.linear_update(A)
+    # every row sits in one window and Y starts at zero: Y = eta A Omega
+    assert np.allclose(s.Y, 1.7 * A @ s.Omega.data, rtol=1e-13, atol=1e-12)
+    assert s.range == 13  # range < rank selects 4r + 1
+
+
+def test_spd_low_rank_exact_and_reference_shift():
+    rng = np.random.default_rng(22)
+    Q, _ = np.linalg.qr(rng.standard_normal((60, 5)))
+    lam_true = np.array([50.0, 20.0, 7.0, 3.0, 1.0])
+    A = (Q * lam_true) @ Q.T
+    p = O.AlgParams(matrix_m=60, matrix_n=60, rank=5, sketch_range=12, window=16)
+    s = O.SketchySPD(p)
+    s.linear_update(A)
+    U, lam = s.low_rank_approx()
+    assert np.allclose(lam, lam_true, rtol=1e-10)
+    assert np.allclose(U @ U.T, Q @ Q.T, atol=1e-10)
+    # the reference's removal subtracts ||Y||_2 from S^2 (its local `nu`)
+    s2 = O.SketchySPD(p, shift_removal="reference")
+    s2.linear_update(A)
+    _, lam2 = s2.low_rank_approx()
+    nrm = np.linalg.norm(s2.Y, 2)
+    assert np.allclose(lam2, np.maximum(lam_true + np.finfo(float).eps * nrm - nrm, 0.0), atol=1e-8 * nrm)
+
+
+def test_spd_sparse_sign_map_equals_dense_map():
+    rng = np.random.default_rng(23)
+    B = rng.standard_normal((30, 30))
+    A = B @ B.T
+    p = O.AlgParams(matrix_m=30, matrix_n=30, rank=4, sketch_range=9, window=8, dim_redux="sparse-sign")
+    s = O.SketchySPD(p)
+    s.linear_update(A)
+    d = O.SketchySPD(p, omega=O.GaussDimRedux(s.Omega.data.toarray()))
+    d.linear_update(A)
+    assert np.allclose(s.Y, d.Y, rtol=1e-13, atol=1e-12)
+    assert np.allclose(s.low_rank_approx()[1], d.low_rank_approx()[1], rtol=1e-10)
