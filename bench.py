@@ -60,6 +60,25 @@ MFMA_PER_FLOP = {"fp64": 1, "fp32": 1, "bf16": 1, "bf16x3": 3, "fp8": 1}
 DTYPE = {"fp64": "f64", "fp32": "f32", "bf16": "bf16", "bf16x3": "bf16x3", "fp8": "fp8e4m3"}
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "pmc_traffic_dense-65536x16384.txt")
+
+
+def committed_traffic(form):
+    """HBM bytes per launch of the dominant kernel for a form, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each scaled by the
+    copy-kernel calibration stated there); None when no summary covers it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == form:
+                    return float(parts[2])
+    except OSError:
+        return None
+    return None
+
+
 def haar(rows, cols, seed):
     G = np.random.default_rng(seed).standard_normal((rows, cols))
     Q, R = np.linalg.qr(G)
@@ -524,7 +543,7 @@ def main():
         },
         "roofline": {
             "bound": "mfma", "kernel": name, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": (achieved / peak) if achieved else None, "traffic": None,
+            "frac": (achieved / peak) if achieved else None, "traffic": committed_traffic(form),
             "flops_per_launch": mfma_flops, "avg_launch_ms": best["kernel_avg_ms"],
         },
         "cpu_baseline": cpu,

@@ -297,6 +297,8 @@ int skema_residuals_rbf(skema_ctx* ctx, const double* points, int64_t n, int64_t
  * SketchySPD). */
 int skema_residuals_sym_dense(skema_ctx* ctx, int64_t n, int64_t r, const double* A, int64_t lda, const double* V,
                               int64_t ldv, const double* S, double* rnorms);
+int skema_residuals_sym_csr(skema_ctx* ctx, int64_t n, int64_t r, const int64_t* rowptr, const int32_t* col,
+                            const double* val, const double* V, int64_t ldv, const double* S, double* rnorms);
 int skema_residuals_sym_rbf(skema_ctx* ctx, const double* points, int64_t n, int64_t d, int64_t ldp, double gamma,
                             int64_t r, const double* V, int64_t ldv, const double* S, int64_t window,
                             double* rnorms);

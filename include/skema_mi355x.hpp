@@ -306,4 +306,66 @@ class SketchySVD {
   std::vector<double> rnrms_;
 };
 
+// ---- SketchySPD ------------------------------------------------------------------
+// MatrixType: DeviceMatrix (dense A), CrsMatrix (CSR A) or PointSet (RBF);
+// A symmetric positive semidefinite, m == n.
+template <class MatrixType, class DimReduxT>
+class SketchySPD {
+ public:
+  // ctor (src/Skema_SketchySVD.cpp:757-789): Omega = DimReduxT(ncol, range, seeds[0])
+  SketchySPD(const AlgParams& p, const Context& ctx = Context(0))
+      : p_(p), ctx_(ctx), omega_(ctx, p.matrix_n, p.range(), p.seeds[0], false) {
+    skema_params cp = p.to_c();
+    skema_spd* h = nullptr;
+    check(skema_spd_create(ctx_.get(), &cp, omega_.handle(), &h));
+    h_.reset(h, [](skema_spd* q) { (void)skema_spd_destroy(q); });
+  }
+
+  // linear_update (src/Skema_SketchySVD.cpp:790-880), every window sketched
+  void linear_update(const MatrixType& A) { update_all(A); }
+
+  // low_rank_approx (src/Skema_SketchySVD.cpp:934-1151): U (n x r), eigenvalues (r)
+  std::tuple<DeviceMatrix, std::vector<double>> low_rank_approx(int shift_removal = SKEMA_SPD_SHIFT_EPS) {
+    DeviceMatrix U(p_.matrix_n, p_.rank), S(p_.rank, 1);
+    check(skema_spd_low_rank_approx_device(h_.get(), shift_removal, U.data(), U.ld(), S.data()));
+    return {U, S.to_host()};
+  }
+
+  // compute_residuals (src/Skema_SketchySVD.cpp:1205-1215 via
+  // src/Skema_Residuals.hpp:123-158): ||A u_i - lambda_i u_i||_2
+  const std::vector<double>& compute_residuals(const MatrixType& A, const DeviceMatrix& U,
+                                               const std::vector<double>& S) {
+    DeviceMatrix Sd = DeviceMatrix::from_host(S.data(), (int64_t)S.size(), 1, (int64_t)S.size());
+    rnrms_.assign(S.size(), 0.0);
+    residuals(A, U, Sd);
+    return rnrms_;
+  }
+  skema_spd* handle() const { return h_.get(); }
+
+ private:
+  void update_all(const DeviceMatrix& A) { check(skema_spd_linear_update_dense(h_.get(), A.data(), A.ld())); }
+  void update_all(const CrsMatrix& A) {
+    check(skema_spd_linear_update_csr(h_.get(), A.rowptr(), A.col(), A.val()));
+  }
+  void update_all(const PointSet& P) { check(skema_spd_linear_update_rbf(h_.get(), P.X.data(), P.X.ld())); }
+  void residuals(const DeviceMatrix& A, const DeviceMatrix& U, const DeviceMatrix& S) {
+    check(skema_residuals_sym_dense(ctx_.get(), p_.matrix_n, S.extent(0), A.data(), A.ld(), U.data(), U.ld(),
+                                    S.data(), rnrms_.data()));
+  }
+  void residuals(const CrsMatrix& A, const DeviceMatrix& U, const DeviceMatrix& S) {
+    check(skema_residuals_sym_csr(ctx_.get(), p_.matrix_n, S.extent(0), A.rowptr(), A.col(), A.val(), U.data(),
+                                  U.ld(), S.data(), rnrms_.data()));
+  }
+  void residuals(const PointSet& P, const DeviceMatrix& U, const DeviceMatrix& S) {
+    check(skema_residuals_sym_rbf(ctx_.get(), P.X.data(), P.X.extent(0), P.X.extent(1), P.X.ld(), p_.kernel_gamma,
+                                  S.extent(0), U.data(), U.ld(), S.data(), p_.window, rnrms_.data()));
+  }
+
+  AlgParams p_;
+  Context ctx_;
+  DimReduxT omega_;
+  std::shared_ptr<skema_spd> h_;
+  std::vector<double> rnrms_;
+};
+
 }  // namespace skema_mi355x

@@ -122,6 +122,7 @@ SIGNATURES = {
     "skema_residuals_csr": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _vp]),
     "skema_residuals_rbf": (_i32, [_vp, _vp, _i64, _i64, _i64, _d, _i64, _vp, _i64, _vp, _vp, _i64, _i64, _vp]),
     "skema_residuals_sym_dense": (_i32, [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "skema_residuals_sym_csr": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "skema_residuals_sym_rbf": (_i32, [_vp, _vp, _i64, _i64, _i64, _d, _i64, _vp, _i64, _vp, _i64, _vp]),
     "skema_rbf_sketch_loss_grad": (_i32, [_vp, _vp, _i64, _i64, _i64, _d, _vp, _vp, _vp, _i64, _i32,
                                           C.POINTER(_d), _vp, _i64, C.POINTER(_d)]),

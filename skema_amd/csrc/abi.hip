@@ -850,6 +850,18 @@ int skema_residuals_sym_dense(skema_ctx* ctx, int64_t n, int64_t r, const double
   });
 }
 
+int skema_residuals_sym_csr(skema_ctx* ctx, int64_t n, int64_t r, const int64_t* rowptr, const int32_t* col,
+                            const double* val, const double* V, int64_t ldv, const double* S, double* rnorms) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx, kErrArg, "residuals: null context");
+    Ctx* c = &ctx->c;
+    set_device(c);
+    check_factors(n, n, r, nullptr, 0, S, V, ldv, rnorms);
+    check_csr_window(rowptr, col, val, n, c->stream);
+    residuals_sym_csr(c, n, r, rowptr, col, val, V, ldv, S, rnorms);
+  });
+}
+
 int skema_residuals_sym_rbf(skema_ctx* ctx, const double* points, int64_t n, int64_t d, int64_t ldp, double gamma,
                             int64_t r, const double* V, int64_t ldv, const double* S, int64_t window,
                             double* rnorms) {

@@ -135,6 +135,19 @@ void residuals_sym_dense(Ctx* c, int64_t n, int64_t r, const double* A, int64_t 
   norms_to_host(st, r, n, AV.as<double>(), n, V, ldv, 0, nullptr, 1, nullptr, 1, S, rnorms);
 }
 
+void residuals_sym_csr(Ctx* c, int64_t n, int64_t r, const int64_t* rowptr, const int32_t* col, const double* val,
+                       const double* V, int64_t ldv, const double* S, double* rnorms) {
+  if (r == 0) return;
+  hipStream_t st = c->stream;
+  DevBuf AV((size_t)n * r * sizeof(double)), Vt((size_t)r * n * sizeof(double));
+  transpose(st, n, r, V, ldv, Vt.as<double>(), r);
+  GemmOut o;
+  o.C1 = AV.as<double>(); o.ldc1 = n; o.alpha1 = 1.0; o.beta1 = 0.0; o.nsplit = r;
+  o.C2 = o.C1; o.ldc2 = n;
+  csr_right_dense(st, rowptr, col, val, n, Vt.as<double>(), r, r, o);
+  norms_to_host(st, r, n, AV.as<double>(), n, V, ldv, 0, nullptr, 1, nullptr, 1, S, rnorms);
+}
+
 void residuals_sym_rbf(Ctx* c, const double* X, int64_t n, int64_t d, int64_t ldp, double gamma, int64_t r,
                        const double* V, int64_t ldv, const double* S, int64_t window, double* rnorms) {
   if (r == 0) return;

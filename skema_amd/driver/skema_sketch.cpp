@@ -6,6 +6,11 @@
 //                [--window w] [--model gauss|sparse-sign] [--seeds a,b,c,d]
 //                [--eta e] [--nu v] [--kernel gaussrbf --gamma g]
 //                [--precision fp64|fp32|bf16|bf16x3|fp8] [--history-file h.json]
+//                [--symmetric | --asymmetric] [--force-three-sketch]
+//
+// --symmetric (and kernel input unless --force-three-sketch) takes the
+// SketchySPD route (src/Skema_SketchySVD.cpp:1243-1277): eigenvalues as
+// "svals" and eigenpair residuals as "rnrms" in the history.
 //
 // Inputs: MatrixMarket "array" (dense, column-major values) or "coordinate"
 // (read as CSR with --sparse, densified otherwise), and the Kokkos-Kernels
@@ -154,6 +159,32 @@ int run(const skema_mi355x::AlgParams& p, const M& A, const std::string& history
   return 0;
 }
 
+// The SketchySPD route (sketchyspd, src/Skema_SketchySVD.cpp:1243-1277):
+// history with the eigenvalues as svals and the eigenpair residuals.
+template <class M, class D>
+int run_spd(const skema_mi355x::AlgParams& p, const M& A, const std::string& history,
+            const skema_mi355x::Context& ctx) {
+  skema_mi355x::SketchySPD<M, D> sk(p, ctx);
+  sk.linear_update(A);
+  auto us = sk.low_rank_approx();
+  const std::vector<double>& S = std::get<1>(us);
+  const std::vector<double>& R = sk.compute_residuals(A, std::get<0>(us), S);
+  std::ostringstream js;
+  js.precision(17);
+  js << "{\"svals\":[";
+  for (size_t i = 0; i < S.size(); ++i) js << (i ? "," : "") << S[i];
+  js << "],\"rnrms\":[";
+  for (size_t i = 0; i < R.size(); ++i) js << (i ? "," : "") << R[i];
+  js << "]}";
+  if (history.empty()) {
+    std::cout << js.str() << std::endl;
+  } else {
+    std::ofstream o(history);
+    o << js.str() << std::endl;
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -162,7 +193,8 @@ int main(int argc, char** argv) {
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     if (a.rfind("--", 0) != 0) fail("unexpected argument " + a);
-    const bool flag = (a == "--sparse" || a == "--dense" || a == "--force-three-sketch");
+    const bool flag = (a == "--sparse" || a == "--dense" || a == "--force-three-sketch" || a == "--symmetric" ||
+                       a == "--asymmetric");
     if (flag) kv[a] = "1";
     else if (i + 1 < argc) kv[a] = argv[++i];
     else fail("missing value for " + a);
@@ -197,6 +229,13 @@ int main(int argc, char** argv) {
   const bool is_bin = in.size() > 4 && in.substr(in.size() - 4) == ".bin";
   Matrix A = is_bin ? read_bin(in, sparse && !rbf) : read_mtx(in, sparse && !rbf);
   const std::string history = kv.count("--history-file") ? kv["--history-file"] : "";
+  // --symmetric selects SketchySPD; kernel input is symmetric unless
+  // --force-three-sketch (driver/main.cpp:99-105, src/Skema_SketchySVD.cpp:1249)
+  const bool force3 = kv.count("--force-three-sketch") > 0;
+  bool symmetric = kv.count("--symmetric") > 0 && !kv.count("--asymmetric");
+  if (rbf) symmetric = !force3;
+  const bool spd = symmetric && !force3;
+  const bool gauss = p.dim_redux == SKEMA_MAP_GAUSS;
   try {
     Context ctx(0);
     if (rbf) {
@@ -205,19 +244,26 @@ int main(int argc, char** argv) {
       p.matrix_m = p.matrix_n = A.m;
       p.nfeat = A.n;
       PointSet P{DeviceMatrix::from_host(A.dense.data(), A.m, A.n, A.m)};
-      if (p.dim_redux == SKEMA_MAP_GAUSS) return run<PointSet, GaussDimRedux>(p, P, history, ctx);
+      if (spd) return gauss ? run_spd<PointSet, GaussDimRedux>(p, P, history, ctx)
+                            : run_spd<PointSet, SparseSignDimRedux>(p, P, history, ctx);
+      if (gauss) return run<PointSet, GaussDimRedux>(p, P, history, ctx);
       return run<PointSet, SparseSignDimRedux>(p, P, history, ctx);
     }
     p.matrix_m = A.m;
     p.matrix_n = A.n;
     p.issparse = A.sparse;
+    if (spd && A.m != A.n) fail("--symmetric needs a square matrix");
     if (A.sparse) {
       CrsMatrix C(A.m, A.n, A.rowptr, A.col, A.val);
-      if (p.dim_redux == SKEMA_MAP_GAUSS) return run<CrsMatrix, GaussDimRedux>(p, C, history, ctx);
+      if (spd) return gauss ? run_spd<CrsMatrix, GaussDimRedux>(p, C, history, ctx)
+                            : run_spd<CrsMatrix, SparseSignDimRedux>(p, C, history, ctx);
+      if (gauss) return run<CrsMatrix, GaussDimRedux>(p, C, history, ctx);
       return run<CrsMatrix, SparseSignDimRedux>(p, C, history, ctx);
     }
     DeviceMatrix D = DeviceMatrix::from_host(A.dense.data(), A.m, A.n, A.m);
-    if (p.dim_redux == SKEMA_MAP_GAUSS) return run<DeviceMatrix, GaussDimRedux>(p, D, history, ctx);
+    if (spd) return gauss ? run_spd<DeviceMatrix, GaussDimRedux>(p, D, history, ctx)
+                          : run_spd<DeviceMatrix, SparseSignDimRedux>(p, D, history, ctx);
+    if (gauss) return run<DeviceMatrix, GaussDimRedux>(p, D, history, ctx);
     return run<DeviceMatrix, SparseSignDimRedux>(p, D, history, ctx);
   } catch (const Error& e) {
     fail(e.what());

@@ -495,7 +495,7 @@ class SketchySPD:
 
     def compute_residuals(self, A, U=None, S=None) -> np.ndarray:
         """Eigenpair residual norms (residuals(A, V, S, ...),
-        src/Skema_Residuals.hpp:123-158) of a dense A or an RBF point set."""
+        src/Skema_Residuals.hpp:123-158) of a dense or CSR A or an RBF point set."""
         if U is None:
             U, S = self.low_rank_approx(device=True)
         gamma = self.params.kernel_gamma if self.params.kernel_func == "gaussrbf" else None
@@ -561,12 +561,15 @@ def residual_norms_rbf(points, gamma: float, U, S, V, window: int = 4096,
 def residual_norms_sym(A, V, S, window: int = 4096, gamma: Optional[float] = None,
                        ctx: Optional[Context] = None) -> np.ndarray:
     """Eigenpair residuals ||A v_i - s_i v_i||_2 (src/Skema_Residuals.hpp:123-158):
-    A a dense symmetric device matrix, or (gamma given) the RBF matrix of the
-    device point set A."""
+    A a dense symmetric device matrix, a CrsMatrix, or (gamma given) the RBF
+    matrix of the device point set A."""
     ctx = ctx or default_context()
     r = V.shape[1]
     out = np.empty(r)
-    if gamma is None:
+    if isinstance(A, CrsMatrix):
+        L.call("skema_residuals_sym_csr", ctx.h, A.m, r, _ptr(A.rowptr), _ptr(A.col), _ptr(A.val), _ptr(V), _ld(V),
+               _ptr(S), out.ctypes.data)
+    elif gamma is None:
         n = A.shape[0]
         L.call("skema_residuals_sym_dense", ctx.h, n, r, _ptr(A), _ld(A), _ptr(V), _ld(V), _ptr(S), out.ctypes.data)
     else:

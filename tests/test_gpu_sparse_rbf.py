@@ -292,3 +292,43 @@ def test_cli_driver_on_reference_fixtures(golden_dir, tmp_path):
         assert set(h["timings"]) >= {"init", "update", "approx"}
         # compute_residuals: exact recovery leaves residuals at rounding level
         assert len(h["rnrms"]) == r and max(h["rnrms"]) <= 1e-10 * sv[0], h["rnrms"]
+
+
+def test_cli_driver_spd_route(golden_dir, tmp_path):
+    """--symmetric (and kernel input unless --force-three-sketch) takes the
+    SketchySPD route: the stored eigenvalues of the reference's symmetric and
+    kernel fixtures come back as svals, with eigenpair residuals."""
+    import json
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "skema_amd", "skema_sketch")
+    if not os.path.exists(exe):
+        pytest.skip("driver not built")
+    X = np.loadtxt(f"{golden_dir}/kernel_data.txt")
+    kmtx = tmp_path / "kernel_points.mtx"
+    with open(kmtx, "w") as f:
+        f.write("%%MatrixMarket matrix array real general\n")
+        f.write(f"{X.shape[0]} {X.shape[1]}\n")
+        for v in X.T.reshape(-1):
+            f.write(f"{v:.17g}\n")
+    n = X.shape[0]
+    cases = [
+        (["--input", f"{golden_dir}/symmetric_sparse.mtx", "--sparse", "--symmetric", "--rank", "27", "--range", "27",
+          "--window", "4"], "symmetric_svals.txt", 27, True),
+        (["--input", f"{golden_dir}/symmetric_sparse.mtx", "--symmetric", "--rank", "27", "--range", "27",
+          "--window", "5"], "symmetric_svals.txt", 27, True),
+        (["--input", str(kmtx), "--kernel", "gaussrbf", "--gamma", "1.0", "--rank", str(n), "--range", str(n),
+          "--window", "7"], "kernel_svals.txt", n, True),
+        (["--input", str(kmtx), "--kernel", "gaussrbf", "--gamma", "1.0", "--force-three-sketch", "--rank", str(n),
+          "--range", str(n), "--core", str(2 * n + 1), "--window", "7"], "kernel_svals.txt", n, False),
+    ]
+    for args, svals, r, spd in cases:
+        print("case cli", " ".join(args), flush=True)
+        hist = tmp_path / "hist.json"
+        subprocess.run([exe, *args, "--history-file", str(hist)], check=True, timeout=120)
+        h = json.loads(hist.read_text())
+        sv = np.loadtxt(f"{golden_dir}/{svals}")
+        assert np.max(np.abs(np.array(h["svals"]) - sv[:r])) <= 1e-12 * sv[0], (h["svals"][:3], sv[:3])
+        assert len(h["rnrms"]) == r and max(h["rnrms"]) <= 1e-9 * sv[0], h["rnrms"]
+        assert ("timings" in h) != spd
