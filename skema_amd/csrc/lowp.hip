@@ -223,21 +223,24 @@ __global__ __launch_bounds__(kThreads) void k_quant_fp8_mx(const double* A, int6
   // a wave reads one column's 128 rows per instruction (two rows per lane)
   const int64_t gi = i0 + 2 * lane;
   const bool vec = ((lda & 1) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && gi + 1 < rows;
-#pragma unroll 4
+  // all 16 loads of a lane are issued before the first LDS store
+  double2 v[kMxC / 4];
+#pragma unroll
   for (int e = 0; e < kMxC / 4; ++e) {
-    const int j = wave + 4 * e;
-    const int64_t gj = j0 + j;
-    double2 v = {0.0, 0.0};
+    const int64_t gj = j0 + wave + 4 * e;
+    v[e] = double2{0.0, 0.0};
     if (gj < n) {
       const double* src = A + gi + gj * lda;
-      if (vec) v = *reinterpret_cast<const double2*>(src);
+      if (vec) v[e] = *reinterpret_cast<const double2*>(src);
       else {
-        if (gi < rows) v.x = src[0];
-        if (gi + 1 < rows) v.y = src[1];
+        if (gi < rows) v[e].x = src[0];
+        if (gi + 1 < rows) v[e].y = src[1];
       }
     }
-    *reinterpret_cast<float2*>(&t[j][2 * lane]) = float2{(float)v.x, (float)v.y};
   }
+#pragma unroll
+  for (int e = 0; e < kMxC / 4; ++e)
+    *reinterpret_cast<float2*>(&t[wave + 4 * e][2 * lane]) = float2{(float)v[e].x, (float)v[e].y};
   __syncthreads();
   {  // L: column j, rows 32 rb .. 32 rb + 31 of the tile (4 lanes = one 128-byte run)
     const int j = tid >> 2, rb = tid & 3;
@@ -449,9 +452,13 @@ __device__ __forceinline__ void nt_epilogue_at(const NtArgs& g, const f4v (&acc)
       for (int v = 0; v < 4; ++v) val[v] = (double)acc[x][y][v] * sa[v] * sb[y];
       if (g.part) {
         float* p = g.part + (int64_t)blockIdx.z * g.M * g.N + j * g.M;
+        if ((g.M & 3) == 0 && i0 + 4 <= g.M) {
+          *reinterpret_cast<float4*>(p + i0) = float4{(float)val[0], (float)val[1], (float)val[2], (float)val[3]};
+        } else {
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-          if (i0 + v < g.M) p[i0 + v] = (float)val[v];
+          for (int v = 0; v < 4; ++v)
+            if (i0 + v < g.M) p[i0 + v] = (float)val[v];
+        }
       } else {
         store_nt4(g, i0, j, val);
       }
@@ -956,6 +963,176 @@ __global__ __launch_bounds__(kThreads) void k_ntx(NtArgs g) {
   nt_epilogue_at<TM, TN, FORM == kNxFp8 && !ASC, FORM == kNxFp8 && !BSC>(g, acc, m0 + wave * (16 * TM), n0, lane);
 }
 
+// ---- Direct-to-LDS fp8 NT kernel ---------------------------------------------
+// C = A * B^T on e4m3 operands with 2 x 2 waves, wave tile (16 TM) x (16 TN),
+// block tile BM = 32 TM by BN = 32 TN.  Both tiles of a 128-byte K step go
+// global -> LDS through global_load_lds_dwordx4 (no staging registers) into
+// an NS-slot ring, so NS - 1 steps are in flight while one is consumed.
+// LDS rows are 128 bytes with the eight 16-byte chunks XOR-swizzled (chunk c
+// of row r sits at position c ^ ((r >> 1) & 7)): a ds_read_b128 of 16
+// consecutive rows at one chunk then touches 16 distinct 16-byte bank groups.
+// One wave DMA instruction fills 1 KB = 8 rows; lane l lands at byte 16 l,
+// i.e. row l / 8, position l % 8, and fetches the chunk that belongs there.
+// A slot is ready after an explicit vmcnt wait (loads return in issue order)
+// and s_barrier; no workgroup fence is issued in the loop, because a fence
+// would drain every load in flight.
+// Fragment K assignment and block scales as in k_ntx.  Requires 16-byte
+// aligned rows and every split's K range a multiple of 128 bytes.
+#define SKEMA_LDS __attribute__((address_space(3)))
+
+// s_waitcnt vmcnt(N) with the other counters left alone (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+// Barrier that orders this wave's LDS accesses around it in the compiler's
+// view (memory clobber) without the fence of __syncthreads.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <int TM, int TN, int NS, bool ASC, bool BSC>
+__global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
+  constexpr int BM = 32 * TM, BN = 32 * TN, SB = 128;
+  // rows with block scales and their DMA instructions per wave and step
+  // (16 rows x 4 lane groups = 64 dwords per instruction)
+  constexpr int SR = ASC ? BM : (BSC ? BN : 0), SC = SR / 64;
+  static_assert(SR % 64 == 0, "block-scaled tile rows must be a multiple of 64");
+  constexpr int D = TM + TN + SC;  // DMA instructions per wave and step
+  constexpr int SCOFF = (BM + BN) * SB, STAGE = SCOFF + SR * 16;
+  static_assert(NS >= 2 && NS <= 4 && (NS - 2) * D < 64, "ring depth");
+  __shared__ __attribute__((aligned(1024))) uint8_t smem[NS * STAGE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, q = lane >> 4;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int tiles_n = (int)((g.N + BN - 1) / BN);
+  const int64_t m0 = (int64_t)(g.n_fast ? t / tiles_n : t % g.tiles_m) * BM;
+  const int64_t n0 = (int64_t)(g.n_fast ? t % tiles_n : t / g.tiles_m) * BN;
+  const int64_t kb0 = (int64_t)blockIdx.z * g.k_per_split;
+  const int64_t kb1 = g.K < kb0 + g.k_per_split ? g.K : kb0 + g.k_per_split;
+  const int nsteps = (int)((kb1 - kb0) / SB);
+
+  // DMA sources: wave w fills rows [8 w TM, 8 (w + 1) TM) of the A tile and
+  // [8 w TN, 8 (w + 1) TN) of the B tile; rows past M / N repeat the last row
+  // (computed, never stored).
+  const int lr = lane >> 3, lp = lane & 7;
+  const uint8_t* ga[TM];
+  const uint8_t* gb[TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int r = 8 * (wave * TM + i) + lr;
+    int64_t row = m0 + r;
+    row = row < g.M ? row : g.M - 1;
+    ga[i] = static_cast<const uint8_t*>(g.ah) + row * g.lda + g.koff_a + kb0 + 16 * (lp ^ ((r >> 1) & 7));
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int r = 8 * (wave * TN + j) + lr;
+    int64_t row = n0 + r;
+    row = row < g.N ? row : g.N - 1;
+    gb[j] = static_cast<const uint8_t*>(g.bh) + row * g.ldb + kb0 + 16 * (lp ^ ((r >> 1) & 7));
+  }
+  // Block scales (E8M0): byte (row, K block b) at row * ld + (b % 4) (ld / 4)
+  // + b / 4, so the dword at row * ld + q (ld / 4) + 4 (st / 4) holds lane
+  // group q's scales of steps 4 (st / 4) .. +3.  Each step copies that dword
+  // of every (row, q) of the tile into the slot (row-major, 16 bytes a row).
+  const uint8_t* gs[SC > 0 ? SC : 1];
+#pragma unroll
+  for (int i = 0; i < SC; ++i) {
+    const int r = 16 * (wave * SC + i) + (lane >> 2);
+    const int64_t lim = ASC ? g.M : g.N;
+    int64_t row = (ASC ? m0 : n0) + r;
+    row = row < lim ? row : lim - 1;
+    const int64_t ld = ASC ? g.ldas : g.ldbs;
+    gs[i] = (ASC ? g.asc : g.bsc) + row * ld + (lane & 3) * (ld / 4) + kb0 / SB;
+  }
+  auto issue = [&](int st) {
+    uint8_t* slot = smem + (st % NS) * STAGE;
+    const int64_t off = (int64_t)st * SB;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(ga[i] + off),
+                                       (SKEMA_LDS void*)(slot + 1024 * (wave * TM + i)), 16, 0, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+      __builtin_amdgcn_global_load_lds((const void*)(gb[j] + off),
+                                       (SKEMA_LDS void*)(slot + BM * SB + 1024 * (wave * TN + j)), 16, 0, 0);
+#pragma unroll
+    for (int i = 0; i < SC; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(gs[i] + 4 * (st >> 2)),
+                                       (SKEMA_LDS void*)(slot + SCOFF + 256 * (wave * SC + i)), 4, 0, 0);
+  };
+
+  f4v acc[TM][TN];
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < TN; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  // swizzled byte offsets of this lane's two chunks (q and 4 + q) in a row
+  const int so0 = 16 * (q ^ ((l16 >> 1) & 7)), so1 = so0 ^ 64;
+  auto frag = [&](const uint8_t* tile, int r) {
+    const i4v lo = *reinterpret_cast<const i4v*>(tile + r * SB + so0);
+    const i4v hi = *reinterpret_cast<const i4v*>(tile + r * SB + so1);
+    return i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  };
+
+#pragma unroll
+  for (int p = 0; p < NS - 1; ++p)
+    if (p < nsteps) issue(p);
+
+  for (int st = 0; st < nsteps; ++st) {
+    // slot st is complete once at most the DMA groups issued after it are
+    // outstanding (fewer of those near the end)
+    const int after = nsteps - 1 - st;
+    if (NS >= 4 && after >= 2) wait_vm<(NS >= 4 ? (NS - 2) * D : 0)>();
+    else if (NS >= 3 && after >= 1) wait_vm<(NS >= 3 ? D : 0)>();
+    else wait_vm<0>();
+    lds_barrier();
+    if (st + NS - 1 < nsteps) issue(st + NS - 1);
+    const uint8_t* slot = smem + (st % NS) * STAGE;
+    const uint8_t* tA = slot + (wm * 16 * TM) * SB;
+    const uint8_t* tB = slot + BM * SB + (wn * 16 * TN) * SB;
+    const uint32_t* ts = reinterpret_cast<const uint32_t*>(slot + SCOFF);
+    const int sh = 8 * (st & 3);
+    int sa[TM], sb[TN];
+#pragma unroll
+    for (int x = 0; x < TM; ++x)
+      sa[x] = ASC ? (int)((ts[(wm * 16 * TM + 16 * x + l16) * 4 + q] >> sh) & 0xffu) : 127;
+#pragma unroll
+    for (int y = 0; y < TN; ++y)
+      sb[y] = BSC ? (int)((ts[(wn * 16 * TN + 16 * y + l16) * 4 + q] >> sh) & 0xffu) : 127;
+    if constexpr (TM <= TN) {
+      i8v af[TM];
+#pragma unroll
+      for (int x = 0; x < TM; ++x) af[x] = frag(tA, 16 * x + l16);
+#pragma unroll
+      for (int y = 0; y < TN; ++y) {
+        const i8v bf = frag(tB, 16 * y + l16);
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[x], bf, acc[x][y], 0, 0, 0, sa[x], 0, sb[y]);
+      }
+    } else {
+      i8v bfr[TN];
+#pragma unroll
+      for (int y = 0; y < TN; ++y) bfr[y] = frag(tB, 16 * y + l16);
+#pragma unroll
+      for (int x = 0; x < TM; ++x) {
+        const i8v af = frag(tA, 16 * x + l16);
+#pragma unroll
+        for (int y = 0; y < TN; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af, bfr[y], acc[x][y], 0, 0, 0, sa[x], 0, sb[y]);
+      }
+    }
+  }
+  nt_epilogue_at<TM, TN, !ASC, !BSC>(g, acc, m0 + wm * (16 * TM), n0 + wn * (16 * TN), lane);
+}
+
 // Sum of the FP32 partials of the K splits (in FP64, fixed order), then the
 // epilogue.
 __global__ void k_nt_reduce(NtArgs g, int splits) {
@@ -976,6 +1153,7 @@ __global__ void k_nt_reduce(NtArgs g, int splits) {
 struct NtPlan {
   int bm, bn, splits;
   int tm = 0, tn = 0;
+  int ns = 0;  // > 0: direct-to-LDS fp8 kernel k_mx<tm, tn, ns> (BM = 32 tm, BN = 32 tn)
 };
 NtPlan plan_ntx(int prec, int64_t M, int64_t N, int64_t K) {
   // wide-kernel tiles with the blocks per CU their registers admit
@@ -1001,12 +1179,14 @@ NtPlan plan_ntx(int prec, int64_t M, int64_t N, int64_t K) {
 NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   const int ks = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
   if (prec == SKEMA_PREC_FP8 || prec == SKEMA_PREC_BF16) {
-    // wide kernels, measured per window product (w = 4096, n = 16384,
-    // k = 401, s = 803; right 0.25 / 0.32 ms, left 0.155 / 0.153 ms, core
-    // 0.047 ms for fp8 / bf16), otherwise the round model
-    if (M >= 2048 && N <= 2048 && K >= 8192) return NtPlan{0, 0, 2, 2, 8};
+    // measured per window product (w = 4096, n = 16384, k = 401, s = 803):
+    // fp8 on the direct-to-LDS kernel (right 4x8 tiles, 3 slots, K split 3:
+    // 0.18 ms; left 8x4, 3 slots, split 2: 0.12 ms), bf16 on the wide kernel
+    // (right 0.32 ms, left 0.15 ms), core 0.047 ms; otherwise the round model
+    if (M >= 2048 && N <= 2048 && K >= 8192)
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 3, 4, 8, 3} : NtPlan{0, 0, 2, 2, 8};
     if (M <= 512 && N >= 4096 && K >= 2048)
-      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 4, 4} : NtPlan{0, 0, 1, 2, 8};
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 8, 4, 3} : NtPlan{0, 0, 1, 2, 8};
     if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4};
     return plan_ntx(prec, M, N, K);
   }
@@ -1064,6 +1244,41 @@ void launch_ntx_tile(hipStream_t st, int prec, const NtArgs& a, dim3 grid) {
     hipLaunchKernelGGL((k_ntx<kNxBf16, TM, TN, false, false>), grid, dim3(kThreads), 0, st, a);
   }
 }
+template <int TM, int TN, int NS>
+void launch_mx_a(hipStream_t st, const NtArgs& a, dim3 grid) {
+  hipLaunchKernelGGL((k_mx<TM, TN, NS, true, false>), grid, dim3(kThreads), 0, st, a);
+}
+template <int TM, int TN, int NS>
+void launch_mx_b(hipStream_t st, const NtArgs& a, dim3 grid) {
+  hipLaunchKernelGGL((k_mx<TM, TN, NS, false, true>), grid, dim3(kThreads), 0, st, a);
+}
+// k_mx variants built: A-scaled (right product) 8x8x2, 8x4x3, 4x8x3, 4x4x4;
+// B-scaled (left product) 13x4x2, 8x4x3, 4x8x3, 4x4x4.
+bool launch_mx(hipStream_t st, const NtArgs& a, const NtPlan& pl, dim3 grid) {
+  const int key = pl.tm * 100 + pl.tn * 10 + pl.ns;
+  if (a.asc) {
+    switch (key) {
+      case 882: launch_mx_a<8, 8, 2>(st, a, grid); return true;
+      case 843: launch_mx_a<8, 4, 3>(st, a, grid); return true;
+      case 483: launch_mx_a<4, 8, 3>(st, a, grid); return true;
+      case 444: launch_mx_a<4, 4, 4>(st, a, grid); return true;
+    }
+  } else if (a.bsc) {
+    switch (key) {
+      case 1342: launch_mx_b<13, 4, 2>(st, a, grid); return true;
+      case 843: launch_mx_b<8, 4, 3>(st, a, grid); return true;
+      case 483: launch_mx_b<4, 8, 3>(st, a, grid); return true;
+      case 444: launch_mx_b<4, 4, 4>(st, a, grid); return true;
+    }
+  }
+  return false;
+}
+bool mx_built(const NtArgs& a, const NtPlan& pl) {
+  const int key = pl.tm * 100 + pl.tn * 10 + pl.ns;
+  if (a.asc) return key == 882 || key == 843 || key == 483 || key == 444;
+  if (a.bsc) return key == 1342 || key == 843 || key == 483 || key == 444;
+  return false;
+}
 // The wide kernels need 16-byte aligned rows of both operands.
 bool ntx_eligible(int prec, const NtArgs& a) {
   if (prec != SKEMA_PREC_FP8 && prec != SKEMA_PREC_BF16) return false;
@@ -1072,6 +1287,11 @@ bool ntx_eligible(int prec, const NtArgs& a) {
          (a.ldb * es) % 16 == 0 && (a.koff_a * es) % 16 == 0;
 }
 void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid) {
+  if (pl.ns > 0) {
+    SKEMA_REQUIRE(prec == SKEMA_PREC_FP8 && launch_mx(st, a, pl, grid), kErrUnsupported,
+                  "no direct-to-LDS fp8 kernel for this tile");
+    return;
+  }
   if (pl.tm > 0) {
     if (pl.tm == 4 && pl.tn == 8) launch_ntx_tile<4, 8>(st, prec, a, grid);
     else if (pl.tm == 4 && pl.tn == 4) launch_ntx_tile<4, 4>(st, prec, a, grid);
@@ -1092,9 +1312,13 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
   if ((a.asc || a.bsc) && pl.tm == 0) pl = plan_ntx(prec, a.M, a.N, a.K);
   SKEMA_REQUIRE(!(a.asc || a.bsc) || (ntx_eligible(prec, a) && (!a.asc || a.koff_a == 0)), kErrUnsupported,
                 "block-scaled fp8 operands need 16-byte aligned rows (and no K offset on A)");
-  if (pl.tm > 0 && !ntx_eligible(prec, a)) pl = NtPlan{64, 64, pl.splits};
-  if (pl.tm > 0) { pl.bm = 64 * pl.tm; pl.bn = 16 * pl.tn; }
   const int kstep = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
+  // k_mx: block-scaled fp8 operands, 16-byte aligned rows, whole K steps
+  if (pl.ns > 0 && !(ntx_eligible(prec, a) && a.K % kstep == 0 && mx_built(a, pl)))
+    pl = plan_ntx(prec, a.M, a.N, a.K);
+  if (pl.tm > 0 && !ntx_eligible(prec, a)) pl = NtPlan{64, 64, pl.splits};
+  if (pl.ns > 0) { pl.bm = 32 * pl.tm; pl.bn = 32 * pl.tn; }
+  else if (pl.tm > 0) { pl.bm = 64 * pl.tm; pl.bn = 16 * pl.tn; }
   int splits = ws ? pl.splits : 1;
   // block-scaled operands: splits start on a 4-step boundary (one scale dword)
   const int64_t kgrain = (a.asc || a.bsc) ? 4 * kstep : kstep;
