@@ -995,9 +995,11 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
-template <int TM, int TN, int NS, bool ASC, bool BSC>
+template <int FORM, int TM, int TN, int NS, bool ASC, bool BSC>
 __global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
   constexpr int BM = 32 * TM, BN = 32 * TN, SB = 128;
+  constexpr int64_t ES = FORM == kNxFp8 ? 1 : 2;  // bytes per element
+  static_assert(FORM == kNxFp8 || (!ASC && !BSC), "block scales are fp8 only");
   // rows with block scales and their DMA instructions per wave and step
   // (16 rows x 4 lane groups = 64 dwords per instruction)
   constexpr int SR = ASC ? BM : (BSC ? BN : 0), SC = SR / 64;
@@ -1012,8 +1014,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
   const int tiles_n = (int)((g.N + BN - 1) / BN);
   const int64_t m0 = (int64_t)(g.n_fast ? t / tiles_n : t % g.tiles_m) * BM;
   const int64_t n0 = (int64_t)(g.n_fast ? t % tiles_n : t / g.tiles_m) * BN;
-  const int64_t kb0 = (int64_t)blockIdx.z * g.k_per_split;
-  const int64_t kb1 = g.K < kb0 + g.k_per_split ? g.K : kb0 + g.k_per_split;
+  // byte range of K for this split
+  const int64_t kb0 = (int64_t)blockIdx.z * g.k_per_split * ES;
+  const int64_t kb1 = (g.K < ((int64_t)blockIdx.z + 1) * g.k_per_split ? g.K : ((int64_t)blockIdx.z + 1) * g.k_per_split) * ES;
   const int nsteps = (int)((kb1 - kb0) / SB);
 
   // DMA sources: wave w fills rows [8 w TM, 8 (w + 1) TM) of the A tile and
@@ -1027,14 +1030,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
     const int r = 8 * (wave * TM + i) + lr;
     int64_t row = m0 + r;
     row = row < g.M ? row : g.M - 1;
-    ga[i] = static_cast<const uint8_t*>(g.ah) + row * g.lda + g.koff_a + kb0 + 16 * (lp ^ ((r >> 1) & 7));
+    ga[i] = static_cast<const uint8_t*>(g.ah) + (row * g.lda + g.koff_a) * ES + kb0 + 16 * (lp ^ ((r >> 1) & 7));
   }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     const int r = 8 * (wave * TN + j) + lr;
     int64_t row = n0 + r;
     row = row < g.N ? row : g.N - 1;
-    gb[j] = static_cast<const uint8_t*>(g.bh) + row * g.ldb + kb0 + 16 * (lp ^ ((r >> 1) & 7));
+    gb[j] = static_cast<const uint8_t*>(g.bh) + row * g.ldb * ES + kb0 + 16 * (lp ^ ((r >> 1) & 7));
   }
   // Block scales (E8M0): byte (row, K block b) at row * ld + (b % 4) (ld / 4)
   // + b / 4, so the dword at row * ld + q (ld / 4) + 4 (st / 4) holds lane
@@ -1106,6 +1109,19 @@ __global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
 #pragma unroll
     for (int y = 0; y < TN; ++y)
       sb[y] = BSC ? (int)((ts[(wn * 16 * TN + 16 * y + l16) * 4 + q] >> sh) & 0xffu) : 127;
+    // fp8: one 16x16x128 scaled MFMA per 128-byte step; bf16: the step's two
+    // 64-byte halves (chunks q and 4 + q of each lane) feed two 16x16x32
+    // MFMAs -- A and B use the same K assignment, so the sum is unchanged.
+    auto mma = [&](f4v& c, const i8v& a, const i8v& b, int sx, int sy) {
+      if constexpr (FORM == kNxFp8) {
+        c = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 0, 0, 0, sx, 0, sy);
+      } else {
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, i4v{a[0], a[1], a[2], a[3]}),
+                                                    __builtin_bit_cast(bf16x8, i4v{b[0], b[1], b[2], b[3]}), c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, i4v{a[4], a[5], a[6], a[7]}),
+                                                    __builtin_bit_cast(bf16x8, i4v{b[4], b[5], b[6], b[7]}), c, 0, 0, 0);
+      }
+    };
     if constexpr (TM <= TN) {
       i8v af[TM];
 #pragma unroll
@@ -1114,8 +1130,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
       for (int y = 0; y < TN; ++y) {
         const i8v bf = frag(tB, 16 * y + l16);
 #pragma unroll
-        for (int x = 0; x < TM; ++x)
-          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af[x], bf, acc[x][y], 0, 0, 0, sa[x], 0, sb[y]);
+        for (int x = 0; x < TM; ++x) mma(acc[x][y], af[x], bf, sa[x], sb[y]);
       }
     } else {
       i8v bfr[TN];
@@ -1125,19 +1140,37 @@ __global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
       for (int x = 0; x < TM; ++x) {
         const i8v af = frag(tA, 16 * x + l16);
 #pragma unroll
-        for (int y = 0; y < TN; ++y)
-          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af, bfr[y], acc[x][y], 0, 0, 0, sa[x], 0, sb[y]);
+        for (int y = 0; y < TN; ++y) mma(acc[x][y], af, bfr[y], sa[x], sb[y]);
       }
     }
   }
-  nt_epilogue_at<TM, TN, !ASC, !BSC>(g, acc, m0 + wm * (16 * TM), n0 + wn * (16 * TN), lane);
+  nt_epilogue_at<TM, TN, FORM == kNxFp8 && !ASC, FORM == kNxFp8 && !BSC>(g, acc, m0 + wm * (16 * TM),
+                                                                          n0 + wn * (16 * TN), lane);
 }
 
 // Sum of the FP32 partials of the K splits (in FP64, fixed order), then the
 // epilogue.
+// M a multiple of 4: one thread per four consecutive rows of a column
+// (16-byte partial loads, one 32-byte read-modify-write of the state);
+// otherwise one thread per element (consecutive threads, consecutive rows).
 __global__ void k_nt_reduce(NtArgs g, int splits) {
   const int64_t total = g.M * g.N;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if ((g.M & 3) == 0) {
+    const int64_t qm = g.M / 4, quads = qm * g.N;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < quads; e += stride) {
+      const int64_t j = e / qm, i0 = 4 * (e % qm);
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      const float* p = g.part + j * g.M + i0;
+      for (int z = 0; z < splits; ++z, p += total) {
+        const float4 f = *reinterpret_cast<const float4*>(p);
+        v[0] += (double)f.x; v[1] += (double)f.y; v[2] += (double)f.z; v[3] += (double)f.w;
+      }
+      store_nt4(g, i0, j, v);
+    }
+    return;
+  }
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
     double v = 0.0;
     for (int z = 0; z < splits; ++z) v += (double)g.part[(int64_t)z * total + e];
     store_nt(g, e % g.M, e / g.M, v);
@@ -1180,14 +1213,15 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   const int ks = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
   if (prec == SKEMA_PREC_FP8 || prec == SKEMA_PREC_BF16) {
     // measured per window product (w = 4096, n = 16384, k = 401, s = 803):
-    // fp8 on the direct-to-LDS kernel (right 4x8 tiles, 3 slots, K split 3:
-    // 0.18 ms; left 8x4, 3 slots, split 2: 0.12 ms), bf16 on the wide kernel
-    // (right 0.32 ms, left 0.15 ms), core 0.047 ms; otherwise the round model
-    if (M >= 2048 && N <= 2048 && K >= 8192)
-      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 3, 4, 8, 3} : NtPlan{0, 0, 2, 2, 8};
+    // right on the direct-to-LDS kernel (4x8 tiles, 3 slots, K split 3;
+    // 0.17 ms fp8, 0.27 ms bf16), left fp8 there too (8x4, 3 slots, split 2;
+    // 0.13 ms) and bf16 on the wide kernel (0.15 ms), core (bf16) on the
+    // direct-to-LDS kernel (4x4, 4 slots, split 4; 0.034 ms); otherwise the
+    // round model
+    if (M >= 2048 && N <= 2048 && K >= 8192) return NtPlan{0, 0, 3, 4, 8, 3};
     if (M <= 512 && N >= 4096 && K >= 2048)
       return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 8, 4, 3} : NtPlan{0, 0, 1, 2, 8};
-    if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4};
+    if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4, 4};
     return plan_ntx(prec, M, N, K);
   }
   // Shapes of the window products, with the tile and split measured fastest
@@ -1244,39 +1278,32 @@ void launch_ntx_tile(hipStream_t st, int prec, const NtArgs& a, dim3 grid) {
     hipLaunchKernelGGL((k_ntx<kNxBf16, TM, TN, false, false>), grid, dim3(kThreads), 0, st, a);
   }
 }
-template <int TM, int TN, int NS>
-void launch_mx_a(hipStream_t st, const NtArgs& a, dim3 grid) {
-  hipLaunchKernelGGL((k_mx<TM, TN, NS, true, false>), grid, dim3(kThreads), 0, st, a);
+template <int FORM, int TM, int TN, int NS, bool ASC, bool BSC>
+void launch_mx_t(hipStream_t st, const NtArgs& a, dim3 grid) {
+  hipLaunchKernelGGL((k_mx<FORM, TM, TN, NS, ASC, BSC>), grid, dim3(kThreads), 0, st, a);
 }
-template <int TM, int TN, int NS>
-void launch_mx_b(hipStream_t st, const NtArgs& a, dim3 grid) {
-  hipLaunchKernelGGL((k_mx<TM, TN, NS, false, true>), grid, dim3(kThreads), 0, st, a);
-}
-// k_mx variants built: A-scaled (right product) 8x8x2, 8x4x3, 4x8x3, 4x4x4;
-// B-scaled (left product) 13x4x2, 8x4x3, 4x8x3, 4x4x4.
-bool launch_mx(hipStream_t st, const NtArgs& a, const NtPlan& pl, dim3 grid) {
+// k_mx variants built (tm tn ns): fp8 with A scales (right product) 4 8 3,
+// 8 8 2; fp8 with B scales (left product) 8 4 3, 13 4 2; bf16 4 8 3, 8 4 3,
+// 4 4 4.  launch_mx returns false for any other combination.
+bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid, bool dry = false) {
   const int key = pl.tm * 100 + pl.tn * 10 + pl.ns;
-  if (a.asc) {
+  if (prec == SKEMA_PREC_FP8 && a.asc) {
     switch (key) {
-      case 882: launch_mx_a<8, 8, 2>(st, a, grid); return true;
-      case 843: launch_mx_a<8, 4, 3>(st, a, grid); return true;
-      case 483: launch_mx_a<4, 8, 3>(st, a, grid); return true;
-      case 444: launch_mx_a<4, 4, 4>(st, a, grid); return true;
+      case 483: if (!dry) launch_mx_t<kNxFp8, 4, 8, 3, true, false>(st, a, grid); return true;
+      case 882: if (!dry) launch_mx_t<kNxFp8, 8, 8, 2, true, false>(st, a, grid); return true;
     }
-  } else if (a.bsc) {
+  } else if (prec == SKEMA_PREC_FP8 && a.bsc) {
     switch (key) {
-      case 1342: launch_mx_b<13, 4, 2>(st, a, grid); return true;
-      case 843: launch_mx_b<8, 4, 3>(st, a, grid); return true;
-      case 483: launch_mx_b<4, 8, 3>(st, a, grid); return true;
-      case 444: launch_mx_b<4, 4, 4>(st, a, grid); return true;
+      case 843: if (!dry) launch_mx_t<kNxFp8, 8, 4, 3, false, true>(st, a, grid); return true;
+      case 1342: if (!dry) launch_mx_t<kNxFp8, 13, 4, 2, false, true>(st, a, grid); return true;
+    }
+  } else if (prec == SKEMA_PREC_BF16) {
+    switch (key) {
+      case 483: if (!dry) launch_mx_t<kNxBf16, 4, 8, 3, false, false>(st, a, grid); return true;
+      case 843: if (!dry) launch_mx_t<kNxBf16, 8, 4, 3, false, false>(st, a, grid); return true;
+      case 444: if (!dry) launch_mx_t<kNxBf16, 4, 4, 4, false, false>(st, a, grid); return true;
     }
   }
-  return false;
-}
-bool mx_built(const NtArgs& a, const NtPlan& pl) {
-  const int key = pl.tm * 100 + pl.tn * 10 + pl.ns;
-  if (a.asc) return key == 882 || key == 843 || key == 483 || key == 444;
-  if (a.bsc) return key == 1342 || key == 843 || key == 483 || key == 444;
   return false;
 }
 // The wide kernels need 16-byte aligned rows of both operands.
@@ -1288,8 +1315,7 @@ bool ntx_eligible(int prec, const NtArgs& a) {
 }
 void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid) {
   if (pl.ns > 0) {
-    SKEMA_REQUIRE(prec == SKEMA_PREC_FP8 && launch_mx(st, a, pl, grid), kErrUnsupported,
-                  "no direct-to-LDS fp8 kernel for this tile");
+    SKEMA_REQUIRE(launch_mx(st, prec, a, pl, grid), kErrUnsupported, "no direct-to-LDS kernel for this tile");
     return;
   }
   if (pl.tm > 0) {
@@ -1313,8 +1339,8 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
   SKEMA_REQUIRE(!(a.asc || a.bsc) || (ntx_eligible(prec, a) && (!a.asc || a.koff_a == 0)), kErrUnsupported,
                 "block-scaled fp8 operands need 16-byte aligned rows (and no K offset on A)");
   const int kstep = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
-  // k_mx: block-scaled fp8 operands, 16-byte aligned rows, whole K steps
-  if (pl.ns > 0 && !(ntx_eligible(prec, a) && a.K % kstep == 0 && mx_built(a, pl)))
+  // k_mx: 16-byte aligned rows, whole 128-byte K steps, a built variant
+  if (pl.ns > 0 && !(ntx_eligible(prec, a) && a.K % kstep == 0 && launch_mx(st, prec, a, pl, dim3(), true)))
     pl = plan_ntx(prec, a.M, a.N, a.K);
   if (pl.tm > 0 && !ntx_eligible(prec, a)) pl = NtPlan{64, 64, pl.splits};
   if (pl.ns > 0) { pl.bm = 32 * pl.tm; pl.bn = 32 * pl.tn; }
@@ -1336,7 +1362,8 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
   launch_nt(st, prec, a, pl, grid);
   check_launch("k_nt");
   if (splits > 1) {
-    const int blocks = (int)std::min<int64_t>(cdiv(a.M * a.N, 256), 8192);
+    const int64_t items = (a.M % 4 == 0) ? a.M / 4 * a.N : a.M * a.N;
+    const int blocks = (int)std::min<int64_t>(cdiv(items, 256), 8192);
     hipLaunchKernelGGL(k_nt_reduce, dim3(blocks), dim3(256), 0, st, a, splits);
     check_launch("k_nt_reduce");
   }

@@ -106,31 +106,33 @@ def test_lowp_window_products_at_sketch_scale(S, form):
     assert O.rel_err(Z, osk.Z) <= 2 * tol, O.rel_err(Z, osk.Z)
 
 
-def test_lowp_fp8_direct_lds_products(S):
-    """Shapes that take the direct-to-LDS fp8 kernel for both window products:
+@pytest.mark.parametrize("form", ["fp8", "bf16"])
+def test_lowp_direct_lds_products(S, form):
+    """Shapes that take the direct-to-LDS kernel for all three window products:
     right product 4096 x 1204 over K = 8320 (three K splits, the last one
     shorter; the last column tile ragged), left product 401 x 8320 over
-    K = 4096 (one 416-row tile, column scales); the second, 1000-row window
-    takes the other kernels.  Besides the normwise bound, every row and
+    K = 4096 (two 256-row tiles, column scales for fp8), core 803 x 803
+    over K = 4096; the second, 1000-row window takes the other kernels.  Besides the normwise bound, every row and
     column of X and Y is checked on its own, so one wrong tile cannot hide
     in the norm."""
-    print("case lowp fp8 direct-to-LDS products", flush=True)
+    print(f"case lowp {form} direct-to-LDS products", flush=True)
     m, n = 5096, 8320
     rng = np.random.default_rng(21)
     A = rng.standard_normal((m, 48)) @ rng.standard_normal((48, n)) + 0.1 * rng.standard_normal((m, n))
     p = O.AlgParams(matrix_m=m, matrix_n=n, rank=100, window=4096, sketch_nu=0.9, sketch_eta=1.1)
     osk = O.SketchySVD(p)
     osk.linear_update(A)
-    g = _run(S, A, p, osk, "fp8")
+    g = _run(S, A, p, osk, form)
     X, Y, Z = g.state()
+    tol = TOLS[form]
     ex, ey, ez = O.rel_err(X, osk.X), O.rel_err(Y, osk.Y), O.rel_err(Z, osk.Z)
-    print(f"fp8 rel err X {ex:.3e} Y {ey:.3e} Z {ez:.3e}", flush=True)
-    assert ex <= TOLS["fp8"] and ey <= TOLS["fp8"] and ez <= 2 * TOLS["fp8"], (ex, ey, ez)
+    print(f"{form} rel err X {ex:.3e} Y {ey:.3e} Z {ez:.3e}", flush=True)
+    assert ex <= tol and ey <= tol and ez <= 2 * tol, (ex, ey, ez)
 
     def worst(G, R, axis):
         return float((np.linalg.norm(G - R, axis=axis) / np.linalg.norm(R, axis=axis)).max())
 
     wx = max(worst(X, osk.X, 0), worst(X, osk.X, 1))
     wy = max(worst(Y, osk.Y, 0), worst(Y, osk.Y, 1))
-    print(f"fp8 worst row/column rel err X {wx:.3e} Y {wy:.3e}", flush=True)
-    assert wx <= 2 * TOLS["fp8"] and wy <= 2 * TOLS["fp8"], (wx, wy)
+    print(f"{form} worst row/column rel err X {wx:.3e} Y {wy:.3e}", flush=True)
+    assert wx <= 2 * tol and wy <= 2 * tol, (wx, wy)
