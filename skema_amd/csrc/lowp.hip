@@ -48,6 +48,7 @@ typedef int i8v __attribute__((ext_vector_type(8)));
 typedef int i4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef double d4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 
 constexpr int kThreads = 256;
 constexpr int kQ = 64;  // quantize tile
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(kThreads) void k_quant(const double* A, int64_t lda
   for (int e = 0; e < kQ * kQ / kThreads; ++e) {
     const int i = tid % kQ, j = tid / kQ + e * (kThreads / kQ);
     const int64_t gi = i0 + i, gj = j0 + j;
-    const float x = (gi < rows && gj < n) ? (float)A[gi + gj * lda] : 0.f;
+    const float x = (gi < rows && gj < n) ? (float)__builtin_nontemporal_load(A + gi + gj * lda) : 0.f;
     t[j][i] = x;
     if (gi < rows && gj < n) {
       const T h = (T)x;
@@ -231,7 +232,12 @@ __global__ __launch_bounds__(kThreads) void k_quant_fp8_mx(const double* A, int6
     v[e] = double2{0.0, 0.0};
     if (gj < n) {
       const double* src = A + gi + gj * lda;
-      if (vec) v[e] = *reinterpret_cast<const double2*>(src);
+      // non-temporal: the window is read once, and streaming it through the
+      // caches would evict the maps and the copies the products read next
+      if (vec) {
+        const d2v t2 = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(src));
+        v[e] = double2{t2[0], t2[1]};
+      }
       else {
         if (gi < rows) v[e].x = src[0];
         if (gi + 1 < rows) v[e].y = src[1];
@@ -1186,7 +1192,7 @@ __global__ void k_nt_reduce(NtArgs g, int splits) {
 struct NtPlan {
   int bm, bn, splits;
   int tm = 0, tn = 0;
-  int ns = 0;  // > 0: direct-to-LDS fp8 kernel k_mx<tm, tn, ns> (BM = 32 tm, BN = 32 tn)
+  int ns = 0;  // > 0: direct-to-LDS kernel k_mx<tm, tn, ns> (BM = 32 tm, BN = 32 tn)
 };
 NtPlan plan_ntx(int prec, int64_t M, int64_t N, int64_t K) {
   // wide-kernel tiles with the blocks per CU their registers admit
