@@ -970,8 +970,8 @@ __global__ __launch_bounds__(kThreads) void k_ntx(NtArgs g) {
 }
 
 // ---- Direct-to-LDS fp8 NT kernel ---------------------------------------------
-// C = A * B^T on e4m3 operands with 2 x 2 waves, wave tile (16 TM) x (16 TN),
-// block tile BM = 32 TM by BN = 32 TN.  Both tiles of a 128-byte K step go
+// C = A * B^T on e4m3 or bf16 operands with WM x WN waves, wave tile
+// (16 TM) x (16 TN), block tile BM = 16 TM WM by BN = 16 TN WN.  Both tiles of a 128-byte K step go
 // global -> LDS through global_load_lds_dwordx4 (no staging registers) into
 // an NS-slot ring, so NS - 1 steps are in flight while one is consumed.
 // LDS rows are 128 bytes with the eight 16-byte chunks XOR-swizzled (chunk c
@@ -1001,21 +1001,25 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
-template <int FORM, int TM, int TN, int NS, bool ASC, bool BSC>
-__global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
-  constexpr int BM = 32 * TM, BN = 32 * TN, SB = 128;
+template <int FORM, int WM, int WN, int TM, int TN, int NS, bool ASC, bool BSC>
+__global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
+  constexpr int NW = WM * WN;  // waves: WM along M by WN along N
+  constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN, SB = 128;
+  // DMA instructions per wave and step: 8 rows of A or B, or 16 scale rows
+  constexpr int IA = BM / 8 / NW, IB = BN / 8 / NW;
+  static_assert(IA * 8 * NW == BM && IB * 8 * NW == BN, "tile rows must split evenly over the waves");
   constexpr int64_t ES = FORM == kNxFp8 ? 1 : 2;  // bytes per element
   static_assert(FORM == kNxFp8 || (!ASC && !BSC), "block scales are fp8 only");
   // rows with block scales and their DMA instructions per wave and step
   // (16 rows x 4 lane groups = 64 dwords per instruction)
-  constexpr int SR = ASC ? BM : (BSC ? BN : 0), SC = SR / 64;
-  static_assert(SR % 64 == 0, "block-scaled tile rows must be a multiple of 64");
-  constexpr int D = TM + TN + SC;  // DMA instructions per wave and step
+  constexpr int SR = ASC ? BM : (BSC ? BN : 0), SC = SR / 16 / NW;
+  static_assert(SC * 16 * NW == SR, "block-scaled tile rows must split evenly over the waves");
+  constexpr int D = IA + IB + SC;  // DMA instructions per wave and step
   constexpr int SCOFF = (BM + BN) * SB, STAGE = SCOFF + SR * 16;
   static_assert(NS >= 2 && NS <= 4 && (NS - 2) * D < 64, "ring depth");
   __shared__ __attribute__((aligned(1024))) uint8_t smem[NS * STAGE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, q = lane >> 4;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WM, wn = wave / WM;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int tiles_n = (int)((g.N + BN - 1) / BN);
   const int64_t m0 = (int64_t)(g.n_fast ? t / tiles_n : t % g.tiles_m) * BM;
@@ -1025,22 +1029,22 @@ __global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
   const int64_t kb1 = (g.K < ((int64_t)blockIdx.z + 1) * g.k_per_split ? g.K : ((int64_t)blockIdx.z + 1) * g.k_per_split) * ES;
   const int nsteps = (int)((kb1 - kb0) / SB);
 
-  // DMA sources: wave w fills rows [8 w TM, 8 (w + 1) TM) of the A tile and
-  // [8 w TN, 8 (w + 1) TN) of the B tile; rows past M / N repeat the last row
+  // DMA sources: wave w fills rows [8 w IA, 8 (w + 1) IA) of the A tile and
+  // [8 w IB, 8 (w + 1) IB) of the B tile; rows past M / N repeat the last row
   // (computed, never stored).
   const int lr = lane >> 3, lp = lane & 7;
-  const uint8_t* ga[TM];
-  const uint8_t* gb[TN];
+  const uint8_t* ga[IA];
+  const uint8_t* gb[IB];
 #pragma unroll
-  for (int i = 0; i < TM; ++i) {
-    const int r = 8 * (wave * TM + i) + lr;
+  for (int i = 0; i < IA; ++i) {
+    const int r = 8 * (wave * IA + i) + lr;
     int64_t row = m0 + r;
     row = row < g.M ? row : g.M - 1;
     ga[i] = static_cast<const uint8_t*>(g.ah) + (row * g.lda + g.koff_a) * ES + kb0 + 16 * (lp ^ ((r >> 1) & 7));
   }
 #pragma unroll
-  for (int j = 0; j < TN; ++j) {
-    const int r = 8 * (wave * TN + j) + lr;
+  for (int j = 0; j < IB; ++j) {
+    const int r = 8 * (wave * IB + j) + lr;
     int64_t row = n0 + r;
     row = row < g.N ? row : g.N - 1;
     gb[j] = static_cast<const uint8_t*>(g.bh) + row * g.ldb * ES + kb0 + 16 * (lp ^ ((r >> 1) & 7));
@@ -1063,13 +1067,13 @@ __global__ __launch_bounds__(kThreads, 1) void k_mx(NtArgs g) {
     uint8_t* slot = smem + (st % NS) * STAGE;
     const int64_t off = (int64_t)st * SB;
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < IA; ++i)
       __builtin_amdgcn_global_load_lds((const void*)(ga[i] + off),
-                                       (SKEMA_LDS void*)(slot + 1024 * (wave * TM + i)), 16, 0, 0);
+                                       (SKEMA_LDS void*)(slot + 1024 * (wave * IA + i)), 16, 0, 0);
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
+    for (int j = 0; j < IB; ++j)
       __builtin_amdgcn_global_load_lds((const void*)(gb[j] + off),
-                                       (SKEMA_LDS void*)(slot + BM * SB + 1024 * (wave * TN + j)), 16, 0, 0);
+                                       (SKEMA_LDS void*)(slot + BM * SB + 1024 * (wave * IB + j)), 16, 0, 0);
 #pragma unroll
     for (int i = 0; i < SC; ++i)
       __builtin_amdgcn_global_load_lds((const void*)(gs[i] + 4 * (st >> 2)),
@@ -1192,7 +1196,8 @@ __global__ void k_nt_reduce(NtArgs g, int splits) {
 struct NtPlan {
   int bm, bn, splits;
   int tm = 0, tn = 0;
-  int ns = 0;  // > 0: direct-to-LDS kernel k_mx<tm, tn, ns> (BM = 32 tm, BN = 32 tn)
+  int ns = 0;  // > 0: direct-to-LDS kernel k_mx (wave tile 16 tm x 16 tn, ns LDS slots)
+  int wm = 2, wn = 2;  // k_mx waves along M and N (BM = 16 tm wm, BN = 16 tn wn)
 };
 NtPlan plan_ntx(int prec, int64_t M, int64_t N, int64_t K) {
   // wide-kernel tiles with the blocks per CU their registers admit
@@ -1219,14 +1224,15 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   const int ks = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
   if (prec == SKEMA_PREC_FP8 || prec == SKEMA_PREC_BF16) {
     // measured per window product (w = 4096, n = 16384, k = 401, s = 803):
-    // right on the direct-to-LDS kernel (4x8 tiles, 3 slots, K split 3;
-    // 0.17 ms fp8, 0.27 ms bf16), left fp8 there too (8x4, 3 slots, split 2;
-    // 0.13 ms) and bf16 on the wide kernel (0.15 ms), core (bf16) on the
-    // direct-to-LDS kernel (4x4, 4 slots, split 4; 0.034 ms); otherwise the
-    // round model
-    if (M >= 2048 && N <= 2048 && K >= 8192) return NtPlan{0, 0, 3, 4, 8, 3};
+    // fp8 right and left on the 8-wave direct-to-LDS kernel (128 x 256 and
+    // 256 x 128 block tiles, 3 slots, K split 3 / 2; 0.163 / 0.116 ms), bf16
+    // right on the 4-wave one (4x8 wave tiles, 3 slots, split 3; 0.26 ms) and
+    // left on the wide kernel (0.15 ms), core (bf16) on the 4-wave direct-to-
+    // LDS kernel (4x4, 4 slots, split 4; 0.032 ms); otherwise the round model
+    if (M >= 2048 && N <= 2048 && K >= 8192)
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 3, 4, 4, 3, 2, 4} : NtPlan{0, 0, 3, 4, 8, 3};
     if (M <= 512 && N >= 4096 && K >= 2048)
-      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 8, 4, 3} : NtPlan{0, 0, 1, 2, 8};
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 4, 4, 3, 4, 2} : NtPlan{0, 0, 1, 2, 8};
     if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4, 4};
     return plan_ntx(prec, M, N, K);
   }
@@ -1284,15 +1290,27 @@ void launch_ntx_tile(hipStream_t st, int prec, const NtArgs& a, dim3 grid) {
     hipLaunchKernelGGL((k_ntx<kNxBf16, TM, TN, false, false>), grid, dim3(kThreads), 0, st, a);
   }
 }
-template <int FORM, int TM, int TN, int NS, bool ASC, bool BSC>
+template <int FORM, int TM, int TN, int NS, bool ASC, bool BSC, int WM = 2, int WN = 2>
 void launch_mx_t(hipStream_t st, const NtArgs& a, dim3 grid) {
-  hipLaunchKernelGGL((k_mx<FORM, TM, TN, NS, ASC, BSC>), grid, dim3(kThreads), 0, st, a);
+  hipLaunchKernelGGL((k_mx<FORM, WM, WN, TM, TN, NS, ASC, BSC>), grid, dim3(64 * WM * WN), 0, st, a);
 }
 // k_mx variants built (tm tn ns): fp8 with A scales (right product) 4 8 3,
 // 8 8 2; fp8 with B scales (left product) 8 4 3, 13 4 2; bf16 4 8 3, 8 4 3,
 // 4 4 4.  launch_mx returns false for any other combination.
 bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid, bool dry = false) {
   const int key = pl.tm * 100 + pl.tn * 10 + pl.ns;
+  if (pl.wm * pl.wn == 8) {  // 8 waves (two per SIMD): fp8 A scales 2x4 waves of 4x4 (128 x 256),
+                             // fp8 B scales 4x2 waves of 4x4 (256 x 128), 3 slots
+    if (prec == SKEMA_PREC_FP8 && a.asc && key == 443 && pl.wm == 2) {
+      if (!dry) launch_mx_t<kNxFp8, 4, 4, 3, true, false, 2, 4>(st, a, grid);
+      return true;
+    }
+    if (prec == SKEMA_PREC_FP8 && a.bsc && key == 443 && pl.wm == 4) {
+      if (!dry) launch_mx_t<kNxFp8, 4, 4, 3, false, true, 4, 2>(st, a, grid);
+      return true;
+    }
+    return false;
+  }
   if (prec == SKEMA_PREC_FP8 && a.asc) {
     switch (key) {
       case 483: if (!dry) launch_mx_t<kNxFp8, 4, 8, 3, true, false>(st, a, grid); return true;
@@ -1349,7 +1367,7 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
   if (pl.ns > 0 && !(ntx_eligible(prec, a) && a.K % kstep == 0 && launch_mx(st, prec, a, pl, dim3(), true)))
     pl = plan_ntx(prec, a.M, a.N, a.K);
   if (pl.tm > 0 && !ntx_eligible(prec, a)) pl = NtPlan{64, 64, pl.splits};
-  if (pl.ns > 0) { pl.bm = 32 * pl.tm; pl.bn = 32 * pl.tn; }
+  if (pl.ns > 0) { pl.bm = 16 * pl.tm * pl.wm; pl.bn = 16 * pl.tn * pl.wn; }
   else if (pl.tm > 0) { pl.bm = 64 * pl.tm; pl.bn = 16 * pl.tn; }
   int splits = ws ? pl.splits : 1;
   // block-scaled operands: splits start on a 4-step boundary (one scale dword)
