@@ -581,10 +581,7 @@ int skema_linear_update_dense(skema_sketch* h, const double* A, int64_t lda) {
     SKEMA_REQUIRE(sk.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "linear_update_dense on an RBF sketch");
     check_mat(A, sk.p.m, sk.p.n, lda, "linear_update_dense A");
     zero_state(sk);
-    if (!sk.sparse_maps && sk.p.precision != SKEMA_PREC_FP64)
-      linear_update_dense_lowp(sk, A, lda);  // quantise pass of window w+1 overlapped with window w
-    else
-      for_each_window(sk, [&](int64_t r0, int64_t w) { update_dense_window(sk, A + r0, lda, r0, w); });
+    for_each_window(sk, [&](int64_t r0, int64_t w) { update_dense_window(sk, A + r0, lda, r0, w); });
     sk.ctx->sync_if_needed();
   });
 }

@@ -1224,15 +1224,16 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   const int ks = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
   if (prec == SKEMA_PREC_FP8 || prec == SKEMA_PREC_BF16) {
     // measured per window product (w = 4096, n = 16384, k = 401, s = 803):
-    // fp8 right and left on the 8-wave direct-to-LDS kernel (128 x 256 and
-    // 256 x 128 block tiles, 3 slots, K split 3 / 2; 0.163 / 0.116 ms), bf16
+    // fp8 right and left on the 8-wave direct-to-LDS kernel (right 256 x 256
+    // tile, 2 slots, K split 3: 0.132 ms; left 256 x 128, 3 slots, no split:
+    // 0.111 ms; 16 waves, K split 4, 128 x 256 tiles all slower), bf16
     // right on the 4-wave one (4x8 wave tiles, 3 slots, split 3; 0.26 ms) and
     // left on the wide kernel (0.15 ms), core (bf16) on the 4-wave direct-to-
     // LDS kernel (4x4, 4 slots, split 4; 0.032 ms); otherwise the round model
     if (M >= 2048 && N <= 2048 && K >= 8192)
-      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 3, 4, 4, 3, 2, 4} : NtPlan{0, 0, 3, 4, 8, 3};
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 3, 8, 4, 2, 2, 4} : NtPlan{0, 0, 3, 4, 8, 3};
     if (M <= 512 && N >= 4096 && K >= 2048)
-      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 4, 4, 3, 4, 2} : NtPlan{0, 0, 1, 2, 8};
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 1, 4, 4, 3, 4, 2} : NtPlan{0, 0, 1, 2, 8};
     if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4, 4};
     return plan_ntx(prec, M, N, K);
   }
@@ -1299,14 +1300,26 @@ void launch_mx_t(hipStream_t st, const NtArgs& a, dim3 grid) {
 // 4 4 4.  launch_mx returns false for any other combination.
 bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid, bool dry = false) {
   const int key = pl.tm * 100 + pl.tn * 10 + pl.ns;
-  if (pl.wm * pl.wn == 8) {  // 8 waves (two per SIMD): fp8 A scales 2x4 waves of 4x4 (128 x 256),
-                             // fp8 B scales 4x2 waves of 4x4 (256 x 128), 3 slots
+  if (pl.wm * pl.wn == 16) {  // 16 waves (four per SIMD): fp8 A scales 4x4 waves of 4x4 (256 x 256), 2 slots
+    if (prec == SKEMA_PREC_FP8 && a.asc && key == 442) {
+      if (!dry) launch_mx_t<kNxFp8, 4, 4, 2, true, false, 4, 4>(st, a, grid);
+      return true;
+    }
+    return false;
+  }
+  if (pl.wm * pl.wn == 8) {  // 8 waves (two per SIMD), fp8: A scales 2x4 waves of 4x4 (128 x 256,
+                             // 3 slots) or of 8x4 (256 x 256, 2 slots); B scales 4x2 waves of 4x4
+                             // (256 x 128, 3 slots)
     if (prec == SKEMA_PREC_FP8 && a.asc && key == 443 && pl.wm == 2) {
       if (!dry) launch_mx_t<kNxFp8, 4, 4, 3, true, false, 2, 4>(st, a, grid);
       return true;
     }
     if (prec == SKEMA_PREC_FP8 && a.bsc && key == 443 && pl.wm == 4) {
       if (!dry) launch_mx_t<kNxFp8, 4, 4, 3, false, true, 4, 2>(st, a, grid);
+      return true;
+    }
+    if (prec == SKEMA_PREC_FP8 && a.asc && key == 842 && pl.wm == 2) {
+      if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, true, false, 2, 4>(st, a, grid);
       return true;
     }
     return false;
@@ -1613,52 +1626,6 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
   char* base = sk.win.as<char>();
   quantize_window(sk, H, ldh, rows, w, base, sk.ctx->stream);
   window_products(sk, row0, rows, w, base);
-}
-
-void linear_update_dense_lowp(Sketch& sk, const double* A, int64_t lda) {
-  Ctx* c = sk.ctx;
-  hipStream_t st = c->stream;
-  const int64_t m = sk.p.m;
-  const int64_t wsz = (sk.p.window >= m) ? m : sk.p.window;
-  const int64_t nw = cdiv(m, wsz);
-  if (nw < 2) {
-    update_dense_window_lowp(sk, A, lda, 0, m);
-    return;
-  }
-  // Two window slots: the quantise pass of window w + 1 runs on the side
-  // stream while window w's products run on the sketch's stream (it is
-  // HBM-bound, they are matrix-core and L2 bound).  Slot s is rewritten only
-  // after the products that read it are done (ev_done), and products start
-  // only once their slot is filled (ev_quant).
-  size_t slot = 0;
-  for (int64_t r0 = 0; r0 < m; r0 += wsz) slot = std::max(slot, win_layout(sk, r0, std::min(wsz, m - r0)).total);
-  sk.win.reserve(2 * slot);
-  char* bases[2] = {sk.win.as<char>(), sk.win.as<char>() + slot};
-  hipStream_t side = c->side_stream();
-  hipEvent_t ev_quant[2] = {c->event(0), c->event(1)}, ev_done[2] = {c->event(2), c->event(3)};
-  hipEvent_t ev_start = c->event(4);
-  SKEMA_HIP(hipEventRecord(ev_start, st));  // the state clear and anything before this call
-  SKEMA_HIP(hipStreamWaitEvent(side, ev_start, 0));
-  const WinLayout w0 = win_layout(sk, 0, std::min(wsz, m));
-  quantize_window(sk, A, lda, std::min(wsz, m), w0, bases[0], side);
-  SKEMA_HIP(hipEventRecord(ev_quant[0], side));
-  int64_t wi = 0;
-  for (int64_t r0 = 0; r0 < m; r0 += wsz, ++wi) {
-    const int64_t rows = std::min(wsz, m - r0);
-    const int sl = (int)(wi & 1);
-    const int64_t r1 = r0 + wsz;
-    if (r1 < m) {
-      const int ns = sl ^ 1;
-      if (wi >= 1) SKEMA_HIP(hipStreamWaitEvent(side, ev_done[ns], 0));
-      const int64_t rows1 = std::min(wsz, m - r1);
-      const WinLayout w1 = win_layout(sk, r1, rows1);
-      quantize_window(sk, A + r1, lda, rows1, w1, bases[ns], side);
-      SKEMA_HIP(hipEventRecord(ev_quant[ns], side));
-    }
-    SKEMA_HIP(hipStreamWaitEvent(st, ev_quant[sl], 0));
-    window_products(sk, r0, rows, win_layout(sk, r0, rows), bases[sl]);
-    SKEMA_HIP(hipEventRecord(ev_done[sl], st));
-  }
 }
 
 void gemm_nt_bf16(hipStream_t st, int64_t M, int64_t N, int64_t K, const __bf16* A, int64_t lda, const __bf16* B,

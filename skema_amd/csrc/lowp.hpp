@@ -7,9 +7,6 @@ namespace skema {
 
 void prepare_lowp_maps(Sketch& sk);
 void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows);
-// linear_update of a dense A in a reduced form: the window loop with the
-// quantise pass of the next window overlapped with this window's products.
-void linear_update_dense_lowp(Sketch& sk, const double* A, int64_t lda);
 // Z <- nu Z + eta Phi[:, row0:row0+rows] G for a BF16 G (rows x s,
 // column-major, ld ldg) on the BF16 NT GEMM (bf16 form maps).
 void lowp_core_bf16(Sketch& sk, const __bf16* G, int64_t ldg, int64_t row0, int64_t rows);

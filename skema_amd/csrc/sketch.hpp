@@ -37,26 +37,6 @@ struct Ctx {
   void sync_if_needed() {
     if (!async_calls) SKEMA_HIP(hipStreamSynchronize(stream));
   }
-  // A second stream (default priority) for work overlapped with `stream`,
-  // and a few ordering events; created on first use.
-  hipStream_t side = nullptr;
-  hipEvent_t events[8] = {};
-  hipStream_t side_stream() {
-    if (!side) SKEMA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    return side;
-  }
-  hipEvent_t event(int i) {
-    if (!events[i]) SKEMA_HIP(hipEventCreateWithFlags(&events[i], hipEventDisableTiming));
-    return events[i];
-  }
-  ~Ctx() {
-    for (hipEvent_t& e : events)
-      if (e) (void)hipEventDestroy(e);
-    if (side) {
-      (void)hipStreamSynchronize(side);
-      (void)hipStreamDestroy(side);
-    }
-  }
 };
 
 // RAII timing scope around a group of launches.
