@@ -107,52 +107,78 @@ __global__ void k_count_cols(const int64_t* rowptr, int64_t rows, const int32_t*
     atomicAdd(&counts[col[base + e]], 1);
 }
 
-// Column counts per row block in LDS (cols <= kHistCols): block b takes rows
-// [b R, (b+1) R) (thread per row), counts its nonzeros per column in an LDS
-// histogram and writes it to T[b][0..cols).  No global atomics.
+// Column counts per row block in LDS (cols <= kHistCols, the histogram sized
+// to cols in dynamic LDS): block b takes rows [b R, (b+1) R), walks its
+// nonzeros in storage order (coalesced), counts them per column in LDS and
+// writes the histogram to T[b][0..cols).  No global atomics.
 constexpr int kHistCols = 32768;
 __global__ __launch_bounds__(1024) void k_count_cols_blocked(const int64_t* rowptr, int64_t rows, const int32_t* col,
                                                             int64_t cols, int64_t R, int* T) {
-  __shared__ int hist[kHistCols];
+  extern __shared__ int hist[];
   const int64_t b = blockIdx.x;
   for (int64_t c = threadIdx.x; c < cols; c += 1024) hist[c] = 0;
   __syncthreads();
   const int64_t r0 = b * R, r1 = (r0 + R < rows) ? r0 + R : rows;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += 1024)
-    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) atomicAdd(&hist[col[e]], 1);
+  if (r0 < r1) {
+    const int64_t e1 = rowptr[r1];
+    for (int64_t e = rowptr[r0] + threadIdx.x; e < e1; e += 1024) atomicAdd(&hist[col[e]], 1);
+  }
   __syncthreads();
   for (int64_t c = threadIdx.x; c < cols; c += 1024) T[b * cols + c] = hist[c];
 }
 // Per column: T[b][c] <- sum_{b' < b} T[b'][c] (exclusive, in block order),
 // counts[c] <- the column total.
+// Sixteen row blocks' counts are loaded before any is overwritten, so the
+// loads of a batch are in flight together.
 __global__ void k_scan_blocks(int64_t nb, int64_t cols, int* T, int* counts) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
+  constexpr int kB = 16;
   int run = 0;
-  for (int64_t b = 0; b < nb; ++b) {
-    const int t = T[b * cols + c];
-    T[b * cols + c] = run;
-    run += t;
+  for (int64_t b0 = 0; b0 < nb; b0 += kB) {
+    int t[kB];
+#pragma unroll
+    for (int j = 0; j < kB; ++j) t[j] = (b0 + j < nb) ? T[(b0 + j) * cols + c] : 0;
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      if (b0 + j < nb) T[(b0 + j) * cols + c] = run;
+      run += t[j];
+    }
   }
   counts[c] = run;
 }
 // Scatter of block b's rows into its own sub-range of every column:
-// cursor[c] = ptr[c] + T[b][c] in LDS, one LDS atomic per nonzero.
+// cursor[c] = ptr[c] + T[b][c] in LDS, one LDS atomic per nonzero.  The
+// block's nonzeros are walked in storage order (coalesced reads); the row of
+// nonzero e is found by binary search in the block's slice of rowptr, which
+// is staged in LDS after the cursors (R + 1 entries, relative to the slice
+// start).
 __global__ __launch_bounds__(1024) void k_fill_csc_blocked(const int64_t* rowptr, int64_t rows, const int32_t* col,
                                                           const double* val, int64_t cols, int64_t R, const int* T,
                                                           const int64_t* ptr, int32_t* orow, double* oval) {
-  __shared__ int cur[kHistCols];
+  extern __shared__ int lds[];
+  int* cur = lds;
+  int* rp = lds + cols;  // rowptr[r0 .. r1] - rowptr[r0]
   const int64_t b = blockIdx.x;
-  for (int64_t c = threadIdx.x; c < cols; c += 1024) cur[c] = (int)ptr[c] + T[b * cols + c];
-  __syncthreads();
   const int64_t r0 = b * R, r1 = (r0 + R < rows) ? r0 + R : rows;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += 1024) {
-    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
-      const int32_t c = col[e];
-      const int pos = atomicAdd(&cur[c], 1);
-      orow[pos] = (int32_t)(i);
-      oval[pos] = val[e];
+  if (r0 >= r1) return;  // whole block: no barrier is skipped by part of it
+  const int64_t e0 = rowptr[r0];
+  const int nr = (int)(r1 - r0);
+  for (int64_t c = threadIdx.x; c < cols; c += 1024) cur[c] = (int)ptr[c] + T[b * cols + c];
+  for (int i = threadIdx.x; i <= nr; i += 1024) rp[i] = (int)(rowptr[r0 + i] - e0);
+  __syncthreads();
+  const int ne = rp[nr];
+  for (int e = threadIdx.x; e < ne; e += 1024) {
+    // last row start <= e
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rp[mid] <= e) lo = mid; else hi = mid - 1;
     }
+    const int32_t c = col[e0 + e];
+    const int pos = atomicAdd(&cur[c], 1);
+    orow[pos] = (int32_t)(r0 + lo);
+    oval[pos] = val[e0 + e];
   }
 }
 
@@ -390,10 +416,12 @@ CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t
   const size_t o_ptr = align_up((size_t)cols * sizeof(int) * 2, 256);
   const size_t o_row = o_ptr + align_up((size_t)(cols + 1) * sizeof(int64_t), 256);
   const size_t o_val = o_row + align_up((size_t)nnz * sizeof(int32_t), 256);
-  // Row-blocked counts and placement in LDS when the columns fit (128 row
+  // Row-blocked counts and placement in LDS when the columns fit (512 row
   // blocks; no global atomics); global atomic counters otherwise.
-  const int64_t nbk = 128, R = cdiv(rows, nbk);
-  const bool blocked = cols <= kHistCols && rows >= 4096 && nnz > 0;
+  const int64_t nbk = 512, R = cdiv(rows, nbk);
+  // (the fill keeps cols cursors and the R + 1 row starts of its slice in LDS)
+  const bool blocked =
+      cols <= kHistCols && rows >= 4096 && nnz > 0 && (size_t)(cols + R + 1) * sizeof(int) <= 160 * 1024;
   const size_t o_T = o_val + align_up((size_t)nnz * sizeof(double), 256);
   const size_t total = o_T + (blocked ? align_up((size_t)nbk * cols * sizeof(int), 256) : 0);
   scratch.reserve(total);
@@ -409,7 +437,8 @@ CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t
   w.val = reinterpret_cast<double*>(base + o_val);
   int* T = blocked ? reinterpret_cast<int*>(base + o_T) : nullptr;
   if (blocked) {
-    hipLaunchKernelGGL(k_count_cols_blocked, dim3((unsigned)nbk), dim3(1024), 0, st, rowptr, rows, col, cols, R, T);
+    hipLaunchKernelGGL(k_count_cols_blocked, dim3((unsigned)nbk), dim3(1024), (size_t)cols * sizeof(int), st, rowptr,
+                       rows, col, cols, R, T);
     hipLaunchKernelGGL(k_scan_blocks, dim3(blocks_for(cols)), dim3(256), 0, st, nbk, cols, T, counts);
   } else {
     fill_i32(st, counts, 2 * cols, 0);
@@ -421,8 +450,8 @@ CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, counts, cols, w.ptr);
   if (nnz > 0) {
     if (blocked)
-      hipLaunchKernelGGL(k_fill_csc_blocked, dim3((unsigned)nbk), dim3(1024), 0, st, rowptr, rows, col, val, cols, R,
-                         T, w.ptr, w.row, w.val);
+      hipLaunchKernelGGL(k_fill_csc_blocked, dim3((unsigned)nbk), dim3(1024), (size_t)(cols + R + 1) * sizeof(int), st,
+                         rowptr, rows, col, val, cols, R, T, w.ptr, w.row, w.val);
     else
       hipLaunchKernelGGL(k_fill_csc, dim3(blocks_for(rows)), dim3(256), 0, st, rowptr, rows, col, val, cursor, w.ptr,
                          w.row, w.val);

@@ -101,6 +101,29 @@ def test_csr_empty_rows_and_columns(S):
     assert O.rel_err(X, osk.X) <= TOL and O.rel_err(Y, osk.Y) <= TOL and O.rel_err(Z, osk.Z) <= TOL
 
 
+@pytest.mark.parametrize("model", ["sparse-sign", "gauss"])
+def test_csr_whole_pass_row_blocked_csc(S, model):
+    """A matrix tall enough (>= 4096 rows) for the row-blocked CSC build of
+    the whole-matrix pass (LDS column histograms per row block, nonzeros
+    walked in storage order, rows found by binary search in the block's
+    slice of rowptr), with an empty band of rows and empty columns."""
+    print(f"case csr whole pass blocked csc {model}", flush=True)
+    m, n = 20000, 3000
+    A = _sparse(m, n, 2e-3, 17).tolil()
+    A[7000:7600, :] = 0.0
+    A[:, 100:140] = 0.0
+    A = A.tocsr()
+    A.eliminate_zeros()
+    A.sort_indices()
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=8, window=5000, issparse=True, dim_redux=model)
+    osk = O.SketchySVD(p)
+    osk.linear_update(A)
+    g = S.SketchySVD(to_gpu_params(p), maps=gpu_maps(_omaps(osk)))
+    g.linear_update(S.CrsMatrix.from_scipy(A))
+    X, Y, Z = g.state()
+    assert O.rel_err(X, osk.X) <= TOL and O.rel_err(Y, osk.Y) <= TOL and O.rel_err(Z, osk.Z) <= TOL
+
+
 def test_csr_bad_column_index_is_rejected(S):
     """A column index >= n is caught by the validation pass (an error, not a
     fault): the pass only reads the col entries rowptr addresses."""
