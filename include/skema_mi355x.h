@@ -124,6 +124,12 @@ int skema_ctx_synchronize(skema_ctx* ctx);
 /* Kernel timing with HIP events recorded around every library kernel launch
  * group; see skema_ctx_kernel_stats. */
 int skema_ctx_set_profiling(skema_ctx* ctx, int enable);
+/* Restrict profiling to the launch groups named `name` (NULL: all).  Each
+ * recorded event pair costs the stream a few microseconds, so timing one
+ * group over a run perturbs it far less than timing all of them.  No
+ * counterpart in the reference (its timers are host-side,
+ * src/Skema_SketchySVD.cpp:126-246). */
+int skema_ctx_set_profiling_only(skema_ctx* ctx, const char* name);
 /* Average/total milliseconds and launch count recorded under `name` since
  * profiling was enabled (names: "window", "sketch_left", "sketch_right",
  * "sketch_core", "rbf_left", "rbf_right", "rbf_adjoint"). */

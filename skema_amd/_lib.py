@@ -75,6 +75,7 @@ SIGNATURES = {
     "skema_ctx_set_async": (_i32, [_vp, _i32]),
     "skema_ctx_synchronize": (_i32, [_vp]),
     "skema_ctx_set_profiling": (_i32, [_vp, _i32]),
+    "skema_ctx_set_profiling_only": (_i32, [_vp, C.c_char_p]),
     "skema_ctx_kernel_stats": (_i32, [_vp, C.c_char_p, C.POINTER(_i64), C.POINTER(_d)]),
     "skema_ctx_reset_stats": (_i32, [_vp]),
     "skema_map_dense": (_i32, [_vp, _i64, _i64, _vp, _i64, _i32, _pp]),

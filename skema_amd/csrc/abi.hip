@@ -269,6 +269,13 @@ int skema_ctx_set_profiling(skema_ctx* ctx, int enable) {
   });
 }
 
+int skema_ctx_set_profiling_only(skema_ctx* ctx, const char* name) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx, kErrArg, "null ctx");
+    ctx->c.profile_only = name ? name : "";
+  });
+}
+
 int skema_ctx_kernel_stats(skema_ctx* ctx, const char* name, int64_t* count, double* total_ms) {
   return guarded([&] {
     SKEMA_REQUIRE(ctx && name, kErrArg, "kernel_stats: null argument");

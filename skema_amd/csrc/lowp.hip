@@ -1230,10 +1230,8 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
     // right on the 4-wave one (4x8 wave tiles, 3 slots, split 3; 0.26 ms) and
     // left on the wide kernel (0.15 ms), core (bf16) on the 4-wave direct-to-
     // LDS kernel (4x4, 4 slots, split 4; 0.032 ms); otherwise the round model
-    if (M >= 2048 && N <= 2048 && K >= 8192)
-      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 3, 8, 4, 2, 2, 4} : NtPlan{0, 0, 3, 4, 8, 3};
-    if (M <= 512 && N >= 4096 && K >= 2048)
-      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 1, 4, 4, 3, 4, 2} : NtPlan{0, 0, 1, 2, 8};
+    if (M >= 2048 && N <= 2048 && K >= 8192) return NtPlan{0, 0, 3, 8, 4, 2, 2, 4};
+    if (M <= 512 && N >= 4096 && K >= 2048) return NtPlan{0, 0, 1, 4, 4, 3, 4, 2};
     if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4, 4};
     return plan_ntx(prec, M, N, K);
   }
@@ -1322,6 +1320,15 @@ bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
       if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, true, false, 2, 4>(st, a, grid);
       return true;
     }
+    // bf16: the same two shapes without scales
+    if (prec == SKEMA_PREC_BF16 && !a.asc && !a.bsc && key == 842 && pl.wm == 2) {
+      if (!dry) launch_mx_t<kNxBf16, 8, 4, 2, false, false, 2, 4>(st, a, grid);
+      return true;
+    }
+    if (prec == SKEMA_PREC_BF16 && !a.asc && !a.bsc && key == 443 && pl.wm == 4) {
+      if (!dry) launch_mx_t<kNxBf16, 4, 4, 3, false, false, 4, 2>(st, a, grid);
+      return true;
+    }
     return false;
   }
   if (prec == SKEMA_PREC_FP8 && a.asc) {
@@ -1369,7 +1376,9 @@ void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
   else launch_nt_tile<64, 64>(st, prec, a, grid);
 }
 
-void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
+// t (optional): a timing scope closed right after the product kernel, so
+// it covers that kernel and not the split-K reduction that follows.
+void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr) {
   NtPlan pl = plan_nt(prec, a.M, a.N, a.K);
   SKEMA_REQUIRE(!(a.asc && a.bsc), kErrUnsupported, "block scales on both NT operands");
   if ((a.asc || a.bsc) && pl.tm == 0) pl = plan_ntx(prec, a.M, a.N, a.K);
@@ -1398,6 +1407,7 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws) {
   const dim3 grid((unsigned)(a.tiles_m * cdiv(a.N, pl.bn)), 1, (unsigned)splits);
   launch_nt(st, prec, a, pl, grid);
   check_launch("k_nt");
+  if (t) t->stop();
   if (splits > 1) {
     const int64_t items = (a.M % 4 == 0) ? a.M / 4 * a.N : a.M * a.N;
     const int blocks = (int)std::min<int64_t>(cdiv(items, 256), 8192);
@@ -1582,7 +1592,7 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.ldb = w.ldl;
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
-    gemm_nt(st, prec, a, &c->gws.buf);
+    gemm_nt(st, prec, a, &c->gws.buf, &t);
   }
   {
     Timed t(c, "sketch_right");
@@ -1599,7 +1609,7 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.ldb = sk.lp_ldomps;
     a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
     a.gtype = w.gtype; a.gh = base + w.o_g; a.gl = base + w.o_g + w.ge * w.gsz; a.ldg = w.ldl;
-    gemm_nt(st, prec, a, &c->gws.buf);
+    gemm_nt(st, prec, a, &c->gws.buf, &t);
   }
   {
     Timed t(c, "sketch_core");
@@ -1614,7 +1624,7 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.ldb = w.ldl;
     a.out.C1 = sk.Z.as<double>(); a.out.ldc1 = s; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = s;
     a.out.C2 = a.out.C1; a.out.ldc2 = s;
-    gemm_nt(st, w.core_prec, a, &c->gws.buf);
+    gemm_nt(st, w.core_prec, a, &c->gws.buf, &t);
   }
 }
 

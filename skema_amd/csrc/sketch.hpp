@@ -31,6 +31,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   bool async_calls = false;
   bool profiling = false;
+  std::string profile_only;  // when set, only timing scopes of this name record events
   GemmWorkspace gws;
   DevBuf scratch;  // general grow-only scratch
   KernelStats stats;
@@ -47,7 +48,7 @@ struct Timed {
   hipStream_t st = nullptr;
   Timed(Ctx* ctx, const char* n) : Timed(ctx, n, ctx->stream) {}
   Timed(Ctx* ctx, const char* n, hipStream_t s) : c(ctx), name(n), st(s) {
-    if (c->profiling) {
+    if (c->profiling && (c->profile_only.empty() || c->profile_only == name)) {
       a = c->stats.get();
       SKEMA_HIP(hipEventRecord(a, st));
     }

@@ -63,7 +63,10 @@ class Context:
     def synchronize(self) -> None:
         L.call("skema_ctx_synchronize", self.h)
 
-    def set_profiling(self, on: bool) -> None:
+    def set_profiling(self, on: bool, only: Optional[str] = None) -> None:
+        """Kernel timing with HIP events; `only` restricts it to one launch
+        group name (each event pair costs the stream a few microseconds)."""
+        L.call("skema_ctx_set_profiling_only", self.h, only.encode() if only else None)
         L.call("skema_ctx_set_profiling", self.h, int(on))
 
     def kernel_stats(self, name: str) -> Tuple[int, float]:
