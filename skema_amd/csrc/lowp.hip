@@ -1225,13 +1225,15 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   if (prec == SKEMA_PREC_FP8 || prec == SKEMA_PREC_BF16) {
     // measured per window product (w = 4096, n = 16384, k = 401, s = 803):
     // fp8 right and left on the 8-wave direct-to-LDS kernel (right 256 x 256
-    // tile, 2 slots, K split 3: 0.132 ms; left 256 x 128, 3 slots, no split:
-    // 0.111 ms; 16 waves, K split 4, 128 x 256 tiles all slower), bf16
+    // tile, 2 slots, K split 3: 0.111 ms + 0.018 ms split sum; left 256 x 256,
+    // 2 slots, K split 2: 0.076 ms + its sum, 1% ahead of 256 x 128 without
+    // a split; 16 waves, K split 4, 128 x 256 tiles all slower), bf16
     // right on the 4-wave one (4x8 wave tiles, 3 slots, split 3; 0.26 ms) and
     // left on the wide kernel (0.15 ms), core (bf16) on the 4-wave direct-to-
     // LDS kernel (4x4, 4 slots, split 4; 0.032 ms); otherwise the round model
     if (M >= 2048 && N <= 2048 && K >= 8192) return NtPlan{0, 0, 3, 8, 4, 2, 2, 4};
-    if (M <= 512 && N >= 4096 && K >= 2048) return NtPlan{0, 0, 1, 4, 4, 3, 4, 2};
+    if (M <= 512 && N >= 4096 && K >= 2048)
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 8, 4, 2, 2, 4} : NtPlan{0, 0, 1, 4, 4, 3, 4, 2};
     if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4, 4};
     return plan_ntx(prec, M, N, K);
   }
@@ -1318,6 +1320,14 @@ bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
     }
     if (prec == SKEMA_PREC_FP8 && a.asc && key == 842 && pl.wm == 2) {
       if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, true, false, 2, 4>(st, a, grid);
+      return true;
+    }
+    if (prec == SKEMA_PREC_FP8 && a.bsc && key == 842 && pl.wm == 2) {
+      if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, false, true, 2, 4>(st, a, grid);
+      return true;
+    }
+    if (prec == SKEMA_PREC_FP8 && a.bsc && key == 443 && pl.wm == 2) {
+      if (!dry) launch_mx_t<kNxFp8, 4, 4, 3, false, true, 2, 4>(st, a, grid);
       return true;
     }
     // bf16: the same two shapes without scales
