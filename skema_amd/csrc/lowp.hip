@@ -341,9 +341,11 @@ struct NtArgs {
   int n_fast;  // wide kernels: tile order (see k_ntx)
   int64_t k_per_split;
   float* part;  // FP32 partials [split][M x N] when K is split
-  // fp8 with E8M0 block scales (wide kernels only): byte (row r, 32-element
-  // K block b) at asc[r * ldas + b] for A, bsc[...] likewise for B.  An
-  // operand with block scales has no row scale in the epilogue.
+  // fp8 with E8M0 block scales (k_ntx, k_mx): the byte of (row r, 32-element
+  // K block b) is at asc[r * ldas + (b % 4) * (ldas / 4) + b / 4] for A
+  // (grouped by the lane group b % 4 that supplies it to a 16x16x128 MFMA,
+  // so one dword holds four consecutive K steps), bsc[...] likewise for B.
+  // An operand with block scales has no row scale in the epilogue.
   const uint8_t* asc;
   int64_t ldas;
   const uint8_t* bsc;
@@ -1295,67 +1297,35 @@ template <int FORM, int TM, int TN, int NS, bool ASC, bool BSC, int WM = 2, int 
 void launch_mx_t(hipStream_t st, const NtArgs& a, dim3 grid) {
   hipLaunchKernelGGL((k_mx<FORM, WM, WN, TM, TN, NS, ASC, BSC>), grid, dim3(64 * WM * WN), 0, st, a);
 }
-// k_mx variants built (tm tn ns): fp8 with A scales (right product) 4 8 3,
-// 8 8 2; fp8 with B scales (left product) 8 4 3, 13 4 2; bf16 4 8 3, 8 4 3,
-// 4 4 4.  launch_mx returns false for any other combination.
+// k_mx variants built (the ones the plans select; launch_mx returns false
+// for any other combination):
+//   fp8, A scales (right product):  2 x 4 waves of 8 x 4 (256 x 256), 2 slots
+//   fp8, B scales (left product):   2 x 4 waves of 8 x 4 (256 x 256), 2 slots
+//   bf16 right: 2 x 4 waves of 8 x 4, 2 slots; bf16 left: 4 x 2 waves of 4 x 4
+//   (256 x 128), 3 slots; bf16 core: 2 x 2 waves of 4 x 4 (128 x 128), 4 slots
 bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid, bool dry = false) {
   const int key = pl.tm * 100 + pl.tn * 10 + pl.ns;
-  if (pl.wm * pl.wn == 16) {  // 16 waves (four per SIMD): fp8 A scales 4x4 waves of 4x4 (256 x 256), 2 slots
-    if (prec == SKEMA_PREC_FP8 && a.asc && key == 442) {
-      if (!dry) launch_mx_t<kNxFp8, 4, 4, 2, true, false, 4, 4>(st, a, grid);
-      return true;
-    }
-    return false;
+  const bool w24 = pl.wm == 2 && pl.wn == 4, w42 = pl.wm == 4 && pl.wn == 2, w22 = pl.wm == 2 && pl.wn == 2;
+  if (prec == SKEMA_PREC_FP8 && a.asc && w24 && key == 842) {
+    if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, true, false, 2, 4>(st, a, grid);
+    return true;
   }
-  if (pl.wm * pl.wn == 8) {  // 8 waves (two per SIMD), fp8: A scales 2x4 waves of 4x4 (128 x 256,
-                             // 3 slots) or of 8x4 (256 x 256, 2 slots); B scales 4x2 waves of 4x4
-                             // (256 x 128, 3 slots)
-    if (prec == SKEMA_PREC_FP8 && a.asc && key == 443 && pl.wm == 2) {
-      if (!dry) launch_mx_t<kNxFp8, 4, 4, 3, true, false, 2, 4>(st, a, grid);
-      return true;
-    }
-    if (prec == SKEMA_PREC_FP8 && a.bsc && key == 443 && pl.wm == 4) {
-      if (!dry) launch_mx_t<kNxFp8, 4, 4, 3, false, true, 4, 2>(st, a, grid);
-      return true;
-    }
-    if (prec == SKEMA_PREC_FP8 && a.asc && key == 842 && pl.wm == 2) {
-      if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, true, false, 2, 4>(st, a, grid);
-      return true;
-    }
-    if (prec == SKEMA_PREC_FP8 && a.bsc && key == 842 && pl.wm == 2) {
-      if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, false, true, 2, 4>(st, a, grid);
-      return true;
-    }
-    if (prec == SKEMA_PREC_FP8 && a.bsc && key == 443 && pl.wm == 2) {
-      if (!dry) launch_mx_t<kNxFp8, 4, 4, 3, false, true, 2, 4>(st, a, grid);
-      return true;
-    }
-    // bf16: the same two shapes without scales
-    if (prec == SKEMA_PREC_BF16 && !a.asc && !a.bsc && key == 842 && pl.wm == 2) {
+  if (prec == SKEMA_PREC_FP8 && a.bsc && w24 && key == 842) {
+    if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, false, true, 2, 4>(st, a, grid);
+    return true;
+  }
+  if (prec == SKEMA_PREC_BF16 && !a.asc && !a.bsc) {
+    if (w24 && key == 842) {
       if (!dry) launch_mx_t<kNxBf16, 8, 4, 2, false, false, 2, 4>(st, a, grid);
       return true;
     }
-    if (prec == SKEMA_PREC_BF16 && !a.asc && !a.bsc && key == 443 && pl.wm == 4) {
+    if (w42 && key == 443) {
       if (!dry) launch_mx_t<kNxBf16, 4, 4, 3, false, false, 4, 2>(st, a, grid);
       return true;
     }
-    return false;
-  }
-  if (prec == SKEMA_PREC_FP8 && a.asc) {
-    switch (key) {
-      case 483: if (!dry) launch_mx_t<kNxFp8, 4, 8, 3, true, false>(st, a, grid); return true;
-      case 882: if (!dry) launch_mx_t<kNxFp8, 8, 8, 2, true, false>(st, a, grid); return true;
-    }
-  } else if (prec == SKEMA_PREC_FP8 && a.bsc) {
-    switch (key) {
-      case 843: if (!dry) launch_mx_t<kNxFp8, 8, 4, 3, false, true>(st, a, grid); return true;
-      case 1342: if (!dry) launch_mx_t<kNxFp8, 13, 4, 2, false, true>(st, a, grid); return true;
-    }
-  } else if (prec == SKEMA_PREC_BF16) {
-    switch (key) {
-      case 483: if (!dry) launch_mx_t<kNxBf16, 4, 8, 3, false, false>(st, a, grid); return true;
-      case 843: if (!dry) launch_mx_t<kNxBf16, 8, 4, 3, false, false>(st, a, grid); return true;
-      case 444: if (!dry) launch_mx_t<kNxBf16, 4, 4, 4, false, false>(st, a, grid); return true;
+    if (w22 && key == 444) {
+      if (!dry) launch_mx_t<kNxBf16, 4, 4, 4, false, false, 2, 2>(st, a, grid);
+      return true;
     }
   }
   return false;

@@ -183,27 +183,50 @@ __global__ __launch_bounds__(1024) void k_fill_csc_blocked(const int64_t* rowptr
 }
 
 // Exclusive scan of counts[0..n) into ptr[0..n] by one block of 1024 threads.
+// Tiles of 8192 counts: coalesced loads into LDS, each thread sums its 8
+// consecutive entries, a block scan of the 1024 sums, coalesced writes of the
+// exclusive prefixes, and the tile total carried to the next tile.
 __global__ __launch_bounds__(1024) void k_scan_counts(const int* counts, int64_t n, int64_t* ptr) {
+  constexpr int kPer = 8, kTile = 1024 * kPer;
   __shared__ int64_t part[1024];
+  __shared__ int64_t tile[kTile];
   const int t = threadIdx.x;
-  const int64_t chunk = cdiv(n, 1024);
-  const int64_t lo = t * chunk, hi = (lo + chunk < n) ? lo + chunk : n;
-  int64_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += counts[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int64_t v = (t >= off) ? part[t - off] : 0;
+  int64_t carry = 0;
+  for (int64_t t0 = 0; t0 < n; t0 += kTile) {
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int64_t i = t0 + j * 1024 + t;
+      tile[j * 1024 + t] = i < n ? counts[i] : 0;
+    }
     __syncthreads();
-    part[t] += v;
+    int64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) s += tile[t * kPer + j];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int64_t v = (t >= off) ? part[t - off] : 0;
+      __syncthreads();
+      part[t] += v;
+      __syncthreads();
+    }
+    int64_t run = carry + part[t] - s;  // exclusive prefix of this thread's 8 entries
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int64_t c = tile[t * kPer + j];
+      tile[t * kPer + j] = run;
+      run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int64_t i = t0 + j * 1024 + t;
+      if (i < n) ptr[i] = tile[j * 1024 + t];
+    }
+    carry += part[1023];
     __syncthreads();
   }
-  int64_t run = part[t] - s;  // exclusive prefix of this chunk
-  for (int64_t i = lo; i < hi; ++i) {
-    ptr[i] = run;
-    run += counts[i];
-  }
-  if (t == 1023) ptr[n] = part[1023];
+  if (t == 0) ptr[n] = carry;
 }
 
 __global__ void k_fill_csc(const int64_t* rowptr, int64_t rows, const int32_t* col, const double* val, int* cursor,
