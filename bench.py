@@ -205,6 +205,34 @@ def cpu_baseline(S, cfg, A, maps, budget_s):
                        f"{m} rows ({rows // w} windows of {w}), same A and maps; {spent:.1f} s")
 
 
+def cpu_baseline_windows(op, H_of, m, w, budget_s, what, sparse_convention="rows"):
+    """The oracle's window update on host cores, window after window of the
+    workload (H_of(r0, r1) builds the window the oracle's Window::get would),
+    until budget_s of CPU time has been spent; rows/s of the windows done."""
+    from oracle import skema_oracle as O  # the checker, timed as the baseline
+
+    try:
+        from threadpoolctl import threadpool_info
+
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    osk = O.SketchySVD(op, sparse_convention=sparse_convention)
+    osk.init_state()
+    spent, rows = 0.0, 0
+    for r0 in range(0, m, w):
+        r1 = min(m, r0 + w)
+        t0 = time.perf_counter()
+        osk.update_window(H_of(r0, r1), (r0, r1))
+        spent += time.perf_counter() - t0
+        rows = r1
+        if spent >= budget_s:
+            break
+    return dict(value=rows / spent, unit="rows/s", cores=threads, kind="port",
+                sample=f"oracle (numpy/scipy/OpenBLAS, {threads} threads) {what} over the first {rows} of {m} rows "
+                       f"(windows of {w}, the oracle's own maps of the same kind and sizes); {spent:.1f} s")
+
+
 def make_sparse(cfg):
     import scipy.sparse as sp
 
@@ -287,6 +315,15 @@ def bench_sparse(args, S, torch):
         "kernel_ms": {kname: (v[1] / v[0] if v[0] else None) for kname, v in stats.items()},
         "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        from oracle import skema_oracle as O  # the checker, timed as the baseline
+
+        op = O.AlgParams(matrix_m=m, matrix_n=n, rank=cfg["rank"], sketch_range=k, sketch_core=s, window=w,
+                         dim_redux="sparse-sign", issparse=True)
+        out["cpu_baseline"] = cpu_baseline_windows(op, lambda r0, r1: A[r0:r1], m, w, args.cpu_budget,
+                                                   "CSR window update (scipy sparse x sparse-sign maps)",
+                                                   sparse_convention="cols")
     print(json.dumps(out))
 
 
@@ -380,6 +417,16 @@ def bench_rbf(args, S, torch):
                       "kernel_ms": r["kernel_ms"], "svals_head": r["svals_head"]} for f, r in res.items()},
         "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        from oracle import skema_oracle as O  # the checker, timed as the baseline
+
+        op = O.AlgParams(matrix_m=n, matrix_n=n, rank=cfg["rank"], sketch_range=k, sketch_core=s, window=w,
+                         kernel_func="gaussrbf", kernel_gamma=gamma)
+        out["cpu_baseline"] = cpu_baseline_windows(
+            op, lambda r0, r1: O.gauss_rbf_compute(X[r0:r1], X, gamma, (r0, r1)), n, 64, args.cpu_budget,
+            "RBF window (K rows by direct differences, as the reference's GaussRBFWindow) and update, "
+            "64-row windows")
     print(json.dumps(out))
 
 
