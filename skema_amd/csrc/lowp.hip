@@ -779,7 +779,7 @@ __global__ __launch_bounds__(kThreads) void k_nt_fp8(NtArgs g) {
 // the E8M0 scale of (row l%16, K block [32b, 32b+32)) is read from lane
 // 16b + l%16 (both checked with single-lane probes on gfx950), so fp8
 // operands may carry one scale per row and 32-element K block (ASC, BSC).
-enum { kNxFp8 = 0, kNxBf16 = 1 };
+enum { kNxFp8 = 0, kNxBf16 = 1, kNxBf16x3 = 2 };  // kNxBf16x3: hi + lo planes (k_mx only)
 
 __device__ __forceinline__ i4v ld16_masked(const uint8_t* p, int64_t rem) {
   if (rem >= 16) return *reinterpret_cast<const i4v*>(p);
@@ -1010,14 +1010,16 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   // DMA instructions per wave and step: 8 rows of A or B, or 16 scale rows
   constexpr int IA = BM / 8 / NW, IB = BN / 8 / NW;
   static_assert(IA * 8 * NW == BM && IB * 8 * NW == BN, "tile rows must split evenly over the waves");
-  constexpr int64_t ES = FORM == kNxFp8 ? 1 : 2;  // bytes per element
+  constexpr int64_t ES = FORM == kNxFp8 ? 1 : 2;  // bytes per element (per plane)
   static_assert(FORM == kNxFp8 || (!ASC && !BSC), "block scales are fp8 only");
   // rows with block scales and their DMA instructions per wave and step
   // (16 rows x 4 lane groups = 64 dwords per instruction)
   constexpr int SR = ASC ? BM : (BSC ? BN : 0), SC = SR / 16 / NW;
   static_assert(SC * 16 * NW == SR, "block-scaled tile rows must split evenly over the waves");
-  constexpr int D = IA + IB + SC;  // DMA instructions per wave and step
-  constexpr int SCOFF = (BM + BN) * SB, STAGE = SCOFF + SR * 16;
+  // split-BF16 carries both planes of both operands: A hi | B hi | A lo | B lo
+  constexpr int PL = FORM == kNxBf16x3 ? 2 : 1;
+  constexpr int D = PL * (IA + IB) + SC;  // DMA instructions per wave and step
+  constexpr int LOOFF = (BM + BN) * SB, SCOFF = PL * LOOFF, STAGE = SCOFF + SR * 16;
   static_assert(NS >= 2 && NS <= 4 && (NS - 2) * D < 64, "ring depth");
   __shared__ __attribute__((aligned(1024))) uint8_t smem[NS * STAGE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, q = lane >> 4;
@@ -1037,6 +1039,11 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   const int lr = lane >> 3, lp = lane & 7;
   const uint8_t* ga[IA];
   const uint8_t* gb[IB];
+  int64_t dal = 0, dbl = 0;  // byte distance from the hi to the lo plane (split-BF16)
+  if (FORM == kNxBf16x3) {
+    dal = static_cast<const uint8_t*>(g.al) - static_cast<const uint8_t*>(g.ah);
+    dbl = static_cast<const uint8_t*>(g.bl) - static_cast<const uint8_t*>(g.bh);
+  }
 #pragma unroll
   for (int i = 0; i < IA; ++i) {
     const int r = 8 * (wave * IA + i) + lr;
@@ -1076,6 +1083,17 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
     for (int j = 0; j < IB; ++j)
       __builtin_amdgcn_global_load_lds((const void*)(gb[j] + off),
                                        (SKEMA_LDS void*)(slot + BM * SB + 1024 * (wave * IB + j)), 16, 0, 0);
+    if constexpr (FORM == kNxBf16x3) {
+#pragma unroll
+      for (int i = 0; i < IA; ++i)
+        __builtin_amdgcn_global_load_lds((const void*)(ga[i] + dal + off),
+                                         (SKEMA_LDS void*)(slot + LOOFF + 1024 * (wave * IA + i)), 16, 0, 0);
+#pragma unroll
+      for (int j = 0; j < IB; ++j)
+        __builtin_amdgcn_global_load_lds((const void*)(gb[j] + dbl + off),
+                                         (SKEMA_LDS void*)(slot + LOOFF + BM * SB + 1024 * (wave * IB + j)), 16,
+                                         0, 0);
+    }
 #pragma unroll
     for (int i = 0; i < SC; ++i)
       __builtin_amdgcn_global_load_lds((const void*)(gs[i] + 4 * (st >> 2)),
@@ -1123,18 +1141,41 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
       sb[y] = BSC ? (int)((ts[(wn * 16 * TN + 16 * y + l16) * 4 + q] >> sh) & 0xffu) : 127;
     // fp8: one 16x16x128 scaled MFMA per 128-byte step; bf16: the step's two
     // 64-byte halves (chunks q and 4 + q of each lane) feed two 16x16x32
-    // MFMAs -- A and B use the same K assignment, so the sum is unchanged.
+    // MFMAs -- A and B use the same K assignment, so the sum is unchanged;
+    // split-BF16: hi.hi + hi.lo + lo.hi, each as those two MFMAs.
+    auto bf2 = [&](f4v& c, const i8v& a, const i8v& b) {
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, i4v{a[0], a[1], a[2], a[3]}),
+                                                  __builtin_bit_cast(bf16x8, i4v{b[0], b[1], b[2], b[3]}), c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, i4v{a[4], a[5], a[6], a[7]}),
+                                                  __builtin_bit_cast(bf16x8, i4v{b[4], b[5], b[6], b[7]}), c, 0, 0, 0);
+    };
     auto mma = [&](f4v& c, const i8v& a, const i8v& b, int sx, int sy) {
       if constexpr (FORM == kNxFp8) {
         c = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 0, 0, 0, sx, 0, sy);
       } else {
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, i4v{a[0], a[1], a[2], a[3]}),
-                                                    __builtin_bit_cast(bf16x8, i4v{b[0], b[1], b[2], b[3]}), c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, i4v{a[4], a[5], a[6], a[7]}),
-                                                    __builtin_bit_cast(bf16x8, i4v{b[4], b[5], b[6], b[7]}), c, 0, 0, 0);
+        bf2(c, a, b);
       }
     };
-    if constexpr (TM <= TN) {
+    if constexpr (FORM == kNxBf16x3) {
+      const uint8_t* tAl = tA + LOOFF;
+      const uint8_t* tBl = tB + LOOFF;
+      i8v ah[TM], al[TM];
+#pragma unroll
+      for (int x = 0; x < TM; ++x) {
+        ah[x] = frag(tA, 16 * x + l16);
+        al[x] = frag(tAl, 16 * x + l16);
+      }
+#pragma unroll
+      for (int y = 0; y < TN; ++y) {
+        const i8v bh = frag(tB, 16 * y + l16), bl = frag(tBl, 16 * y + l16);
+#pragma unroll
+        for (int x = 0; x < TM; ++x) {
+          bf2(acc[x][y], ah[x], bh);
+          bf2(acc[x][y], ah[x], bl);
+          bf2(acc[x][y], al[x], bh);
+        }
+      }
+    } else if constexpr (TM <= TN) {
       i8v af[TM];
 #pragma unroll
       for (int x = 0; x < TM; ++x) af[x] = frag(tA, 16 * x + l16);
@@ -1222,8 +1263,12 @@ NtPlan plan_ntx(int prec, int64_t M, int64_t N, int64_t K) {
   return best;
 }
 
+NtPlan plan_nt_2x2(int prec, int64_t M, int64_t N, int64_t K);
+// Shapes of the window products:
+//   right  [y | G] = A_w [Omega; Psi]^T : M = w, N = k+s, K = n (long)
+//   left   x = Upsilon_w A_w            : M = k, N = n, K = w
+//   core   z = Phi_w G                  : M = N = s, K = w
 NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
-  const int ks = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
   if (prec == SKEMA_PREC_FP8 || prec == SKEMA_PREC_BF16) {
     // measured per window product (w = 4096, n = 16384, k = 401, s = 803):
     // fp8 right and left on the 8-wave direct-to-LDS kernel (right 256 x 256
@@ -1239,11 +1284,21 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
     if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4, 4};
     return plan_ntx(prec, M, N, K);
   }
-  // Shapes of the window products, with the tile and split measured fastest
-  // for each form at w = 4096, n = 16384, k = 401, s = 803 (one MI355X):
-  //   right  [y | G] = A_w [Omega; Psi]^T : M = w, N = k+s, K = n (long)
-  //   left   x = Upsilon_w A_w            : M = k, N = n, K = w
-  //   core   z = Phi_w G                  : M = N = s, K = w
+  // split-BF16: the direct-to-LDS kernel with both planes in its ring
+  // (128 x 128 tiles, 2 x 2 waves, 2 slots)
+  if (prec == SKEMA_PREC_BF16X3) {
+    if (M >= 2048 && N <= 2048 && K >= 8192) return NtPlan{0, 0, 3, 4, 4, 2};
+    if (M <= 512 && N >= 4096 && K >= 2048) return NtPlan{0, 0, 2, 4, 4, 2};
+    if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4, 2};
+  }
+  return plan_nt_2x2(prec, M, N, K);
+}
+
+// The 2 x 2-wave kernels (k_nt_*): every form, any alignment; the tile and
+// split measured fastest for each window product at w = 4096, n = 16384,
+// k = 401, s = 803, otherwise a round model.
+NtPlan plan_nt_2x2(int prec, int64_t M, int64_t N, int64_t K) {
+  const int ks = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
   if (M >= 2048 && N <= 2048 && K >= 8192) {
     if (prec == SKEMA_PREC_FP8) return {128, 64, 1};
     if (prec == SKEMA_PREC_FP32) return {128, 128, 4};
@@ -1303,6 +1358,7 @@ void launch_mx_t(hipStream_t st, const NtArgs& a, dim3 grid) {
 //   fp8, B scales (left product):   2 x 4 waves of 8 x 4 (256 x 256), 2 slots
 //   bf16 right: 2 x 4 waves of 8 x 4, 2 slots; bf16 left: 4 x 2 waves of 4 x 4
 //   (256 x 128), 3 slots; bf16 core: 2 x 2 waves of 4 x 4 (128 x 128), 4 slots
+//   split-BF16 (both planes): 2 x 2 waves of 4 x 4 (128 x 128), 2 slots
 bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid, bool dry = false) {
   const int key = pl.tm * 100 + pl.tn * 10 + pl.ns;
   const bool w24 = pl.wm == 2 && pl.wn == 4, w42 = pl.wm == 4 && pl.wn == 2, w22 = pl.wm == 2 && pl.wn == 2;
@@ -1312,6 +1368,10 @@ bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
   }
   if (prec == SKEMA_PREC_FP8 && a.bsc && w24 && key == 842) {
     if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, false, true, 2, 4>(st, a, grid);
+    return true;
+  }
+  if (prec == SKEMA_PREC_BF16X3 && !a.asc && !a.bsc && w22 && key == 442) {
+    if (!dry) launch_mx_t<kNxBf16x3, 4, 4, 2, false, false, 2, 2>(st, a, grid);
     return true;
   }
   if (prec == SKEMA_PREC_BF16 && !a.asc && !a.bsc) {
@@ -1332,9 +1392,10 @@ bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
 }
 // The wide kernels need 16-byte aligned rows of both operands.
 bool ntx_eligible(int prec, const NtArgs& a) {
-  if (prec != SKEMA_PREC_FP8 && prec != SKEMA_PREC_BF16) return false;
+  if (prec != SKEMA_PREC_FP8 && prec != SKEMA_PREC_BF16 && prec != SKEMA_PREC_BF16X3) return false;
   const int64_t es = prec == SKEMA_PREC_FP8 ? 1 : 2;
-  return ((uintptr_t)a.ah % 16) == 0 && ((uintptr_t)a.bh % 16) == 0 && (a.lda * es) % 16 == 0 &&
+  const bool lo = prec != SKEMA_PREC_BF16X3 || (a.al && a.bl && ((uintptr_t)a.al % 16) == 0 && ((uintptr_t)a.bl % 16) == 0);
+  return lo && ((uintptr_t)a.ah % 16) == 0 && ((uintptr_t)a.bh % 16) == 0 && (a.lda * es) % 16 == 0 &&
          (a.ldb * es) % 16 == 0 && (a.koff_a * es) % 16 == 0;
 }
 void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3 grid) {
@@ -1364,10 +1425,13 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr)
   if ((a.asc || a.bsc) && pl.tm == 0) pl = plan_ntx(prec, a.M, a.N, a.K);
   SKEMA_REQUIRE(!(a.asc || a.bsc) || (ntx_eligible(prec, a) && (!a.asc || a.koff_a == 0)), kErrUnsupported,
                 "block-scaled fp8 operands need 16-byte aligned rows (and no K offset on A)");
-  const int kstep = prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32);
-  // k_mx: 16-byte aligned rows, whole 128-byte K steps, a built variant
-  if (pl.ns > 0 && !(ntx_eligible(prec, a) && a.K % kstep == 0 && launch_mx(st, prec, a, pl, dim3(), true)))
-    pl = plan_ntx(prec, a.M, a.N, a.K);
+  // k_mx: 16-byte aligned rows, whole 128-byte K steps, a built variant;
+  // otherwise the wide kernel, or the 2 x 2-wave one for split-BF16 (the
+  // wide kernel has no lo planes)
+  const int kstep_mx = prec == SKEMA_PREC_FP8 ? 128 : 64;
+  if (pl.ns > 0 && !(ntx_eligible(prec, a) && a.K % kstep_mx == 0 && launch_mx(st, prec, a, pl, dim3(), true)))
+    pl = prec == SKEMA_PREC_BF16X3 ? plan_nt_2x2(prec, a.M, a.N, a.K) : plan_ntx(prec, a.M, a.N, a.K);
+  const int kstep = pl.ns > 0 ? kstep_mx : (prec == SKEMA_PREC_FP8 ? 128 : (prec == SKEMA_PREC_BF16 ? 64 : 32));
   if (pl.tm > 0 && !ntx_eligible(prec, a)) pl = NtPlan{64, 64, pl.splits};
   if (pl.ns > 0) { pl.bm = 16 * pl.tm * pl.wm; pl.bn = 16 * pl.tn * pl.wn; }
   else if (pl.tm > 0) { pl.bm = 64 * pl.tm; pl.bn = 16 * pl.tn; }
