@@ -106,7 +106,7 @@ def test_lowp_window_products_at_sketch_scale(S, form):
     assert O.rel_err(Z, osk.Z) <= 2 * tol, O.rel_err(Z, osk.Z)
 
 
-@pytest.mark.parametrize("form", ["fp8", "bf16", "bf16x3"])
+@pytest.mark.parametrize("form", ["fp8", "bf16", "bf16x3", "fp32"])
 def test_lowp_direct_lds_products(S, form):
     """Shapes that take the direct-to-LDS kernel for all three window products:
     right product 4096 x 1204 over K = 8320 (three K splits, the last one
