@@ -1566,9 +1566,10 @@ WinLayout win_layout(const Sketch& sk, int64_t row0, int64_t rows) {
   w.o_rmax = w.o_r + align_up(w.re * w.esz * w.planes, 256);
   w.o_cmax = w.o_rmax + (w.fp8 ? align_up((size_t)rows * 4, 256) : 0);
   w.o_g = w.o_cmax + (w.fp8 ? align_up((size_t)n * 4, 256) : 0);
-  w.o_ls = w.o_g + align_up(w.ge * w.gsz * w.planes, 256);
-  w.o_rs = w.o_ls + (w.mx ? align_up((size_t)n * w.ldls, 256) : 0);
-  w.total = align_up(w.o_rs + (w.mx ? align_up((size_t)rows * w.ldrs, 256) : 0), 256);
+  w.total = align_up(w.o_g + align_up(w.ge * w.gsz * w.planes, 256), 256);
+  // block scales (mx) in sk.lp_scales: L's at 0, R's at o_rs
+  w.o_ls = 0;
+  w.o_rs = w.mx ? align_up((size_t)n * w.ldls, 256) : 0;
   return w;
 }
 
@@ -1581,13 +1582,19 @@ void quantize_window(Sketch& sk, const double* H, int64_t ldh, int64_t rows, con
   Timed t(sk.ctx, "window", st);
   dim3 grid((unsigned)cdiv(rows, kQ), (unsigned)cdiv(n, kQ));
   if (w.mx) {
-    // unit scales (0x7f) wherever no block writes one: K padding past the window
-    fill_i32(st, (int*)(base + w.o_ls), (int64_t)(align_up((size_t)n * w.ldls, 256) / 4), 0x7f7f7f7f);
-    fill_i32(st, (int*)(base + w.o_rs), (int64_t)(align_up((size_t)rows * w.ldrs, 256) / 4), 0x7f7f7f7f);
+    // Unit scales (0x7f) wherever no block writes one (K padding past the
+    // window): every window of the same row count writes the same blocks, so
+    // the padding is refilled only when the row count (the layout) changes.
+    const size_t sbytes = w.o_rs + align_up((size_t)rows * w.ldrs, 256);
+    if (sk.lp_scales.bytes < sbytes || sk.lp_scales_rows != rows) {
+      sk.lp_scales.reserve(sbytes);
+      fill_i32(st, sk.lp_scales.as<int>(), (int64_t)(sbytes / 4), 0x7f7f7f7f);
+      sk.lp_scales_rows = rows;
+    }
+    uint8_t* sc = sk.lp_scales.as<uint8_t>();
     dim3 gmx((unsigned)cdiv(rows, kMxR), (unsigned)cdiv(n, kMxC));
     hipLaunchKernelGGL(k_quant_fp8_mx, gmx, dim3(kThreads), 0, st, H, ldh, rows, n, (uint8_t*)(base + w.o_l), w.ldl,
-                       (uint8_t*)(base + w.o_ls), w.ldls, (uint8_t*)(base + w.o_r), w.ldr, (uint8_t*)(base + w.o_rs),
-                       w.ldrs);
+                       sc + w.o_ls, w.ldls, (uint8_t*)(base + w.o_r), w.ldr, sc + w.o_rs, w.ldrs);
   } else if (w.fp8) {
     fill_i32(st, (int*)rmax, rows, 0);
     fill_i32(st, (int*)cmax, n, 0);
@@ -1632,7 +1639,7 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.bh = base + w.o_l;
     a.bl = split ? (const void*)((const __bf16*)(base + w.o_l) + w.le) : nullptr;
     a.amax_b = (fp8 && !mx) ? cmax : nullptr;
-    if (mx) { a.bsc = (const uint8_t*)(base + w.o_ls); a.ldbs = w.ldls; }
+    if (mx) { a.bsc = sk.lp_scales.as<uint8_t>() + w.o_ls; a.ldbs = w.ldls; }
     a.ldb = w.ldl;
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
@@ -1645,7 +1652,7 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.ah = base + w.o_r;
     a.al = split ? (const void*)((const __bf16*)(base + w.o_r) + w.re) : nullptr;
     a.amax_a = (fp8 && !mx) ? rmax : nullptr;
-    if (mx) { a.asc = (const uint8_t*)(base + w.o_rs); a.ldas = w.ldrs; }
+    if (mx) { a.asc = sk.lp_scales.as<uint8_t>() + w.o_rs; a.ldas = w.ldrs; }
     a.lda = w.ldr; a.koff_a = 0;
     a.bh = sk.lp_omps.p;
     a.bl = split ? (const void*)(sk.lp_omps.as<__bf16>() + oe) : nullptr;

@@ -118,6 +118,11 @@ struct Sketch {
   Csr sp_omps, sp_ompsT;
   // Low-precision map copies (precision forms).
   DevBuf lp_ups, lp_omps, lp_phi;
+  // fp8 E8M0 block scales of the window copies (L then R), kept apart from the
+  // window scratch so their padding (2^0 past the window's K) is written only
+  // when the window row count changes
+  DevBuf lp_scales;
+  int64_t lp_scales_rows = -1;
   int64_t lp_ldups = 0, lp_ldomps = 0;
   // State.
   DevBuf X, Y, Z;  // k x n, m x k, s x s
