@@ -194,9 +194,14 @@ __global__ __launch_bounds__(kThreads) void k_quant_fp8(const double* A, int64_t
 // max |x| (max * 2^-e in [128, 256)) and its scale byte 127 + e goes to row
 // j of Ls (i of Rs) at position (b % 4) ld/4 + b / 4 for block b (the order
 // the NT kernels read: four consecutive K steps of one lane group in one
-// dword).  Tile 128 rows x 64 columns; each thread then owns one (column,
-// row block) of L and one (row, column block) of R, so no maximum crosses a
-// block.
+// dword).  Tile 128 rows x 64 columns, 4 waves.  A wave loads one column's
+// 128 rows per instruction (two rows a lane), so 16 lanes hold one L block:
+// L is quantised in registers (the block maximum over those 16 lanes by DPP
+// moves) and stored two bytes a lane, 128 contiguous bytes per wave.  R
+// needs the transpose: the tile goes through LDS once, then each thread owns
+// one (row, column block) of R, and a wave's 64 lanes take 64 consecutive
+// rows of the same column block, so each of its column reads of the tile
+// covers 64 consecutive words.
 constexpr int kMxR = 128, kMxC = 64;
 __device__ __forceinline__ void quant32_store(const float (&v)[32], int valid, uint8_t* dst, uint8_t* sdst) {
   float m = 0.f;
@@ -215,64 +220,76 @@ __device__ __forceinline__ void quant32_store(const float (&v)[32], int valid, u
   }
   *sdst = (uint8_t)(127 + e);
 }
+// Maximum over the 16 lanes of a DPP row, every lane active: quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror (lane i of 8 with 7 - i),
+// row_mirror (lane i of 16 with 15 - i).
+__device__ __forceinline__ float max_over_16_lanes(float m) {
+  m = fmaxf(m, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(m), 0xb1, 0xf, 0xf, false)));
+  m = fmaxf(m, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(m), 0x4e, 0xf, 0xf, false)));
+  m = fmaxf(m, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(m), 0x141, 0xf, 0xf, false)));
+  return fmaxf(m, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(m), 0x140, 0xf, 0xf, false)));
+}
 __global__ __launch_bounds__(kThreads) void k_quant_fp8_mx(const double* A, int64_t lda, int64_t rows, int64_t n,
                                                            uint8_t* Lq, int64_t ldl, uint8_t* Ls, int64_t ldls,
                                                            uint8_t* Rq, int64_t ldr, uint8_t* Rs, int64_t ldrs) {
-  __shared__ __attribute__((aligned(16))) float t[kMxC][kMxR + 4];  // t[j][i]
+  static_assert(kThreads == 2 * kMxR && kMxC == 2 * 32 && kMxR == 2 * 64, "tile / block shape");
+  __shared__ __attribute__((aligned(16))) float t[kMxC][kMxR];  // t[j][i]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t i0 = (int64_t)blockIdx.x * kMxR, j0 = (int64_t)blockIdx.y * kMxC;
-  // a wave reads one column's 128 rows per instruction (two rows per lane)
+  // the lane's rows gi, gi + 1 of columns wave + 4 e, e < 16
   const int64_t gi = i0 + 2 * lane;
   const bool vec = ((lda & 1) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && gi + 1 < rows;
-  // all 16 loads of a lane are issued before the first LDS store
-  double2 v[kMxC / 4];
+  constexpr int NE = kMxC / 4;
+  // all 16 loads of a lane are issued before the first use; non-temporal:
+  // the window is read once, and streaming it through the caches would
+  // evict the maps and the copies the products read next
+  double2 v[NE];
 #pragma unroll
-  for (int e = 0; e < kMxC / 4; ++e) {
+  for (int e = 0; e < NE; ++e) {
     const int64_t gj = j0 + wave + 4 * e;
     v[e] = double2{0.0, 0.0};
     if (gj < n) {
       const double* src = A + gi + gj * lda;
-      // non-temporal: the window is read once, and streaming it through the
-      // caches would evict the maps and the copies the products read next
       if (vec) {
-        const d2v t2 = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(src));
-        v[e] = double2{t2[0], t2[1]};
-      }
-      else {
+        const d2v x = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(src));
+        v[e] = double2{x[0], x[1]};
+      } else {
         if (gi < rows) v[e].x = src[0];
         if (gi + 1 < rows) v[e].y = src[1];
       }
     }
   }
+  const int64_t bl = (i0 + 32 * (lane >> 4)) / 32;  // the lane's L block (first row / 32)
 #pragma unroll
-  for (int e = 0; e < kMxC / 4; ++e)
-    *reinterpret_cast<float2*>(&t[wave + 4 * e][2 * lane]) = float2{(float)v[e].x, (float)v[e].y};
-  __syncthreads();
-  {  // L: column j, rows 32 rb .. 32 rb + 31 of the tile (4 lanes = one 128-byte run)
-    const int j = tid >> 2, rb = tid & 3;
-    const int64_t gj = j0 + j, gr = i0 + 32 * rb;
-    if (gj < n && gr < rows) {
-      float v[32];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const float4 f = *reinterpret_cast<const float4*>(&t[j][32 * rb + 4 * c]);
-        v[4 * c] = f.x; v[4 * c + 1] = f.y; v[4 * c + 2] = f.z; v[4 * c + 3] = f.w;
-      }
-      const int valid = (int)(rows - gr < 32 ? rows - gr : 32);
-      const int64_t b = gr / 32;
-      quant32_store(v, valid, Lq + gj * ldl + gr, Ls + gj * ldls + (b & 3) * (ldls / 4) + (b >> 2));
+  for (int e = 0; e < NE; ++e) {
+    const int jt = wave + 4 * e;
+    const float2 f = float2{(float)v[e].x, (float)v[e].y};
+    *reinterpret_cast<float2*>(&t[jt][2 * lane]) = f;
+    // rows past the window and columns past n hold zeros: no effect on a maximum
+    const float m = max_over_16_lanes(fmaxf(fabsf(f.x), fabsf(f.y)));
+    const int64_t gj = j0 + jt;
+    if (gj < n && gi < rows) {
+      const int ex = block_exponent(m);
+      const float inv = ldexpf(1.f, -ex);
+      const uint32_t w = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(f.x * inv, f.y * inv, 0, false);
+      uint8_t* dst = Lq + gj * ldl + gi;
+      if (gi + 1 < rows) *reinterpret_cast<uint16_t*>(dst) = (uint16_t)w;
+      else dst[0] = (uint8_t)w;
+      // the block's first lane (its first row is inside the window)
+      if ((lane & 15) == 0) Ls[gj * ldls + (bl & 3) * (ldls / 4) + (bl >> 2)] = (uint8_t)(127 + ex);
     }
   }
+  __syncthreads();
   {  // R: row i, columns 32 cb .. 32 cb + 31 of the tile
-    const int i = tid >> 1, cb = tid & 1;
+    const int i = tid & (kMxR - 1), cb = tid / kMxR;
     const int64_t gr = i0 + i, gc = j0 + 32 * cb;
     if (gr < rows && gc < n) {
-      float v[32];
+      float r[32];
 #pragma unroll
-      for (int q = 0; q < 32; ++q) v[q] = t[32 * cb + q][i];
+      for (int q = 0; q < 32; ++q) r[q] = t[32 * cb + q][i];
       const int valid = (int)(n - gc < 32 ? n - gc : 32);
       const int64_t b = gc / 32;
-      quant32_store(v, valid, Rq + gr * ldr + gc, Rs + gr * ldrs + (b & 3) * (ldrs / 4) + (b >> 2));
+      quant32_store(r, valid, Rq + gr * ldr + gc, Rs + gr * ldrs + (b & 3) * (ldrs / 4) + (b >> 2));
     }
   }
 }
