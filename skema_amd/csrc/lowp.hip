@@ -1289,15 +1289,16 @@ NtPlan plan_nt(int prec, int64_t M, int64_t N, int64_t K) {
   if (prec == SKEMA_PREC_FP8 || prec == SKEMA_PREC_BF16) {
     // measured per window product (w = 4096, n = 16384, k = 401, s = 803):
     // fp8 right and left on the 8-wave direct-to-LDS kernel (right 256 x 256
-    // tile, 2 slots, K split 3: 0.111 ms + 0.018 ms split sum; left 256 x 256,
-    // 2 slots, K split 2: 0.076 ms + its sum, 1% ahead of 256 x 128 without
-    // a split; 16 waves, K split 4, 128 x 256 tiles all slower), bf16
+    // tile, 2 slots, K split 3: 0.111 ms + 0.018 ms split sum; left 448 x 128,
+    // one tile over all k <= 448 rows, 2 slots, K split 2: 0.073 ms + its
+    // sum, 2% per window ahead of 256 x 256 and of 448 x 128 on 4 x 2 waves;
+    // 16 waves, K split 4, 128 x 256 tiles all slower), bf16
     // right on the 4-wave one (4x8 wave tiles, 3 slots, split 3; 0.26 ms) and
     // left on the wide kernel (0.15 ms), core (bf16) on the 4-wave direct-to-
     // LDS kernel (4x4, 4 slots, split 4; 0.032 ms); otherwise the round model
     if (M >= 2048 && N <= 2048 && K >= 8192) return NtPlan{0, 0, 3, 8, 4, 2, 2, 4};
     if (M <= 512 && N >= 4096 && K >= 2048)
-      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 8, 4, 2, 2, 4} : NtPlan{0, 0, 1, 4, 4, 3, 4, 2};
+      return prec == SKEMA_PREC_FP8 ? NtPlan{0, 0, 2, 14, 2, 2, 2, 4} : NtPlan{0, 0, 1, 4, 4, 3, 4, 2};
     if (M <= 1024 && N <= 1024 && K >= 2048) return NtPlan{0, 0, 4, 4, 4, 4};
     return plan_ntx(prec, M, N, K);
   }
@@ -1372,7 +1373,7 @@ void launch_mx_t(hipStream_t st, const NtArgs& a, dim3 grid) {
 // k_mx variants built (the ones the plans select; launch_mx returns false
 // for any other combination):
 //   fp8, A scales (right product):  2 x 4 waves of 8 x 4 (256 x 256), 2 slots
-//   fp8, B scales (left product):   2 x 4 waves of 8 x 4 (256 x 256), 2 slots
+//   fp8, B scales (left product):   2 x 4 waves of 14 x 2 (448 x 128), 2 slots
 //   bf16 right: 2 x 4 waves of 8 x 4, 2 slots; bf16 left: 4 x 2 waves of 4 x 4
 //   (256 x 128), 3 slots; bf16 core: 2 x 2 waves of 4 x 4 (128 x 128), 4 slots
 //   split-BF16 (both planes): 2 x 2 waves of 4 x 4 (128 x 128), 2 slots
@@ -1383,8 +1384,8 @@ bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
     if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, true, false, 2, 4>(st, a, grid);
     return true;
   }
-  if (prec == SKEMA_PREC_FP8 && a.bsc && w24 && key == 842) {
-    if (!dry) launch_mx_t<kNxFp8, 8, 4, 2, false, true, 2, 4>(st, a, grid);
+  if (prec == SKEMA_PREC_FP8 && a.bsc && w24 && key == 1422) {
+    if (!dry) launch_mx_t<kNxFp8, 14, 2, 2, false, true, 2, 4>(st, a, grid);
     return true;
   }
   if (prec == SKEMA_PREC_BF16X3 && !a.asc && !a.bsc && w22 && key == 442) {

@@ -111,7 +111,7 @@ def test_lowp_direct_lds_products(S, form):
     """Shapes that take the direct-to-LDS kernel for all three window products:
     right product 4096 x 1204 over K = 8320 (three K splits, the last one
     shorter; the last column tile ragged), left product 401 x 8320 over
-    K = 4096 (two 256-row tiles, column scales for fp8), core 803 x 803
+    K = 4096 (one 448-row tile for fp8 with column scales, two 256-row tiles for bf16), core 803 x 803
     over K = 4096; the second, 1000-row window takes the other kernels.  Besides the normwise bound, every row and
     column of X and Y is checked on its own, so one wrong tile cannot hide
     in the norm."""
