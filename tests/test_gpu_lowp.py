@@ -136,3 +136,39 @@ def test_lowp_direct_lds_products(S, form):
     wy = max(worst(Y, osk.Y, 0), worst(Y, osk.Y, 1))
     print(f"{form} worst row/column rel err X {wx:.3e} Y {wy:.3e}", flush=True)
     assert wx <= 2 * tol and wy <= 2 * tol, (wx, wy)
+
+
+@pytest.mark.parametrize("form", ["fp8", "bf16", "bf16x3", "fp32"])
+@pytest.mark.parametrize("m,n,window", [(5096, 8320, 4096), (1000, 2100, 256)])
+def test_lowp_exact_on_representable_data(S, form, m, n, window):
+    """Inputs every reduced form encodes exactly: window entries in {-2, ..., 2}
+    times 2^(a + b), a constant over each 32-row block and b over each
+    32-column block, and maps with entries in {-2, ..., 2}.  For fp8 each
+    block's E8M0 scale then leaves its e4m3 values exact (and the maps' row
+    scales theirs); BF16 and FP32 hold these values as they are.  Every
+    product and FP32 partial sum is an integer below 2^24, so X and Y of
+    every reduced form equal the FP64 oracle's up to the rounding of the FP64
+    epilogue; an fp8 scale taken from the wrong block, a dropped scale or a
+    misplaced tile is off by a power of two or worse.  The first shape takes
+    the direct-to-LDS products, the second (256-row windows) the other
+    kernels.  The core product of fp8 and bf16 rounds its right operand G to
+    BF16, so their Z keeps the BF16 bound; split-BF16 and FP32 carry G
+    exactly, so their Z is exact too."""
+    print(f"case {form} exact {m}x{n} w={window}", flush=True)
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 3, size=-(-m // 32))
+    b = rng.integers(0, 3, size=-(-n // 32))
+    scale = np.exp2(np.repeat(a, 32)[:m, None] + np.repeat(b, 32)[None, :n])
+    A = rng.integers(-2, 3, size=(m, n)).astype(np.float64) * scale
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=100, window=window, sketch_nu=0.9, sketch_eta=1.1)
+    maps = O.make_maps(p)
+    for mp in maps.values():
+        mp.data = rng.integers(-2, 3, size=mp.data.shape).astype(np.float64)
+    osk = O.SketchySVD(p, maps=maps)
+    osk.linear_update(A)
+    g = _run(S, A, p, osk, form)
+    X, Y, Z = g.state()
+    ex, ey, ez = O.rel_err(X, osk.X), O.rel_err(Y, osk.Y), O.rel_err(Z, osk.Z)
+    print(f"{form} exact rel err X {ex:.3e} Y {ey:.3e} Z {ez:.3e}", flush=True)
+    assert ex <= 1e-12 and ey <= 1e-12, (ex, ey)
+    assert ez <= (1e-12 if form in ("bf16x3", "fp32") else 2 * TOLS["bf16"]), ez
