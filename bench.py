@@ -103,22 +103,44 @@ def make_dense(cfg, torch):
 
 
 def optimal_tail(A, r, torch, iters=8, over=100):
-    """sqrt(sum_{i>r} sigma_i^2) / ||A||_F by block subspace iteration on the
-    device (FP64 torch products; a measurement aid, not the sketch path):
-    ||A||^2 minus the top-r Ritz values squared of an (r + over)-dimensional
-    subspace refined `iters` times."""
+    """sqrt(sum_{i>r} sigma_i^2) / ||A||_F by block subspace iteration (a
+    measurement aid, not the sketch path): the products with A run on the
+    device (torch FP64), the orthonormalisations and the final singular
+    values on the host (numpy LAPACK); ||A||^2 minus the top-r Ritz values
+    squared of an (r + over)-dimensional subspace refined `iters` times."""
     n = A.shape[1]
-    g = torch.Generator(device="cuda")
-    g.manual_seed(12345)
-    Q = torch.randn((n, r + over), dtype=torch.float64, device="cuda", generator=g)
-    Q, _ = torch.linalg.qr(A @ Q)
+
+    def orth(M):
+        Q, _ = np.linalg.qr(M.cpu().numpy())
+        return torch.as_tensor(Q, device="cuda")
+
+    G = np.random.default_rng(12345).standard_normal((n, r + over))
+    Q = orth(A @ torch.as_tensor(G, device="cuda"))
     for _ in range(iters):
-        Z, _ = torch.linalg.qr(A.t() @ Q)
-        Q, _ = torch.linalg.qr(A @ Z)
-    sv = torch.linalg.svdvals(Q.t() @ A)
+        Z = orth(A.t() @ Q)
+        Q = orth(A @ Z)
+    sv = np.linalg.svd((Q.t() @ A).cpu().numpy(), compute_uv=False)
     a2 = float((A * A).sum())
     top = float((sv[:r] ** 2).sum())
     return math.sqrt(max(a2 - top, 0.0) / a2)
+
+
+def host_cpu():
+    """CPU model and the cores this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"cpu_model": model, "host_cores_available": avail, "host_cpus_online": os.cpu_count()}
 
 
 def dominant_kernel(cfg):
@@ -132,7 +154,10 @@ def flops_per_row(cfg):
     return 2.0 * n * (2 * k + s) + 2.0 * s * s
 
 
-def run_form(S, ctx, cfg, A, maps, form, steps, warmup, torch, error=True):
+def run_form(S, ctx, cfg, A, maps, form, steps, warmup, torch, error=True, ref_state=None, keep_state=False):
+    """Times `steps` linear_update calls of one form (after `warmup`), then
+    (error=True) the finalize and the streamed rank-r error.  ref_state: the
+    FP64 form's (X, Y, Z), against which this form's sketches are compared."""
     p = S.AlgParams(matrix_m=cfg["m"], matrix_n=cfg["n"], rank=cfg["rank"], sketch_range=cfg["k"],
                     sketch_core=cfg["s"], window=cfg["window"], precision=form)
     sk = S.SketchySVD(p, maps=maps, ctx=ctx)
@@ -160,17 +185,95 @@ def run_form(S, ctx, cfg, A, maps, form, steps, warmup, torch, error=True):
     sk.linear_update(A)
     torch.cuda.synchronize()
     ctx.set_profiling(False)
-    stats = {k: ctx.kernel_stats(k) for k in ("sketch_left", "sketch_right", "sketch_core")}
-    # Rank-r quality: ||A - U S V^T||_F / ||A||_F by the streamed residual pass.
-    err = None
+    stats = {k: ctx.kernel_stats(k) for k in ("window", "sketch_left", "sketch_right", "sketch_core")}
+    out = dict(form=form, ms_per_step=ms_per_step, rows_per_s=cfg["m"] / (ms_per_step * 1e-3),
+               kernel_avg_ms=(tot_ms / cnt if cnt else None), kernel_launches=cnt, rel_fro_err=None,
+               finalize_ms=None, state_err_vs_fp64=None,
+               kernel_ms={k: (v[1] / v[0] if v[0] else None) for k, v in stats.items()})
     if error:
+        X, Y, Z = sk.state()
+        if ref_state is not None:
+            out["state_err_vs_fp64"] = {nm: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                                        for nm, a, b in zip("XYZ", (X, Y, Z), ref_state)}
+        if keep_state:
+            out["state"] = (X, Y, Z)
+        # finalize (QR of Y and X^T, core solve, k x k SVD; FP64 on every form)
+        sk.low_rank_approx(device=True)
+        torch.cuda.synchronize()
+        before = dict(sk.timings()["timings"].get("approx", {}))
+        t0 = time.perf_counter()
         U, Sv, V = sk.low_rank_approx(device=True)
+        torch.cuda.synchronize()
+        out["finalize_ms"] = (time.perf_counter() - t0) * 1e3
+        after = sk.timings()["timings"].get("approx", {})
+        out["finalize_phases_ms"] = {key: (after[key] - before.get(key, 0.0)) * 1e3 for key in after}
+        # rank-r quality: ||A - U S V^T||_F / ||A||_F by the streamed residual pass
         e2, a2 = S.residual_fro(A, U, Sv, V, window=cfg["window"], ctx=ctx)
-        err = math.sqrt(e2 / a2)
+        out["rel_fro_err"] = math.sqrt(e2 / a2)
     sk.close()
-    return dict(form=form, ms_per_step=ms_per_step, rows_per_s=cfg["m"] / (ms_per_step * 1e-3),
-                kernel_avg_ms=(tot_ms / cnt if cnt else None), kernel_launches=cnt, rel_fro_err=err,
-                kernel_ms={k: (v[1] / v[0] if v[0] else None) for k, v in stats.items()})
+    return out
+
+
+def h2d_inclusive(S, ctx, cfg, A, maps, form, torch, reps=2):
+    """rows/s when A starts in pinned host memory: one host-to-device copy of
+    the whole matrix followed by linear_update (sequential, no overlap)."""
+    m, n = cfg["m"], cfg["n"]
+    Ah = torch.empty((n, m), dtype=torch.float64, pin_memory=True)
+    Ah.copy_(A.t())
+    Ad = torch.empty((n, m), dtype=torch.float64, device="cuda")
+    p = S.AlgParams(matrix_m=m, matrix_n=n, rank=cfg["rank"], sketch_range=cfg["k"], sketch_core=cfg["s"],
+                    window=cfg["window"], precision=form)
+    sk = S.SketchySVD(p, maps=maps, ctx=ctx)
+    sk.linear_update(Ad.t())
+    torch.cuda.synchronize()
+    t_copy = t_all = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        Ad.copy_(Ah, non_blocking=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sk.linear_update(Ad.t())
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_copy += t1 - t0
+        t_all += t2 - t0
+    sk.close()
+    del Ad, Ah
+    return {"form": form, "rows_per_s": m * reps / t_all, "ms_per_step": t_all * 1e3 / reps,
+            "h2d_ms": t_copy * 1e3 / reps, "h2d_gbs": 8.0 * m * n * reps / t_copy / 1e9,
+            "note": "pinned host A copied whole, then linear_update; no overlap"}
+
+
+def forms_block(results, base, tail):
+    return {f: {"rows_per_s": r["rows_per_s"], "ms_per_step": r["ms_per_step"], "finalize_ms": r["finalize_ms"],
+                "finalize_phases_ms": r.get("finalize_phases_ms"),
+                "rel_fro_err": r["rel_fro_err"],
+                "err_vs_fp64": (r["rel_fro_err"] / base["rel_fro_err"])
+                if (r["rel_fro_err"] and base.get("rel_fro_err")) else None,
+                "err_vs_optimal": (r["rel_fro_err"] / tail) if (tail and r["rel_fro_err"]) else None,
+                "state_err_vs_fp64": r["state_err_vs_fp64"],
+                "speedup_vs_fp64": r["rows_per_s"] / base["rows_per_s"], "kernel_ms": r["kernel_ms"]}
+            for f, r in results.items()}
+
+
+def time_forms(S, ctx, cfg, A, maps, forms, args, torch):
+    results = {}
+    ref_state = None
+    for form in forms:
+        try:
+            print(f"[bench] {cfg['label']}: timing form {form}", file=sys.stderr, flush=True)
+            r = run_form(S, ctx, cfg, A, maps, form, args.steps, args.warmup, torch, error=not args.no_error,
+                         ref_state=ref_state, keep_state=(form == "fp64"))
+            if form == "fp64" and "state" in r:
+                ref_state = r.pop("state")
+            results[form] = r
+            print(f"[bench] {form}: {r['ms_per_step']:.2f} ms/step, err {r['rel_fro_err']}, "
+                  f"finalize {r['finalize_ms']} ms, state err {r['state_err_vs_fp64']}", file=sys.stderr, flush=True)
+        except S.SkemaError as e:
+            if e.code != 7:
+                raise
+            print(f"[bench] form {form} not available: {e}", file=sys.stderr, flush=True)
+    return results
 
 
 def cpu_baseline(S, cfg, A, maps, budget_s):
@@ -202,7 +305,8 @@ def cpu_baseline(S, cfg, A, maps, budget_s):
             break
     return dict(value=rows / spent, unit="rows/s", cores=int(threads), kind="port",
                 sample=f"oracle (numpy/OpenBLAS, {threads} threads) window update over the first {rows} of "
-                       f"{m} rows ({rows // w} windows of {w}), same A and maps; {spent:.1f} s")
+                       f"{m} rows ({rows // w} windows of {w}) of {cfg.get('label', 'the matrix')}, same maps; "
+                       f"{spent:.1f} s", **host_cpu())
 
 
 def cpu_baseline_windows(op, H_of, m, w, budget_s, what, sparse_convention="rows"):
@@ -230,7 +334,8 @@ def cpu_baseline_windows(op, H_of, m, w, budget_s, what, sparse_convention="rows
             break
     return dict(value=rows / spent, unit="rows/s", cores=threads, kind="port",
                 sample=f"oracle (numpy/scipy/OpenBLAS, {threads} threads) {what} over the first {rows} of {m} rows "
-                       f"(windows of {w}, the oracle's own maps of the same kind and sizes); {spent:.1f} s")
+                       f"(windows of {w}, the oracle's own maps of the same kind and sizes); {spent:.1f} s",
+                **host_cpu())
 
 
 def make_sparse(cfg):
@@ -354,7 +459,7 @@ def bench_rbf(args, S, torch):
     gamma = 1.0 / d
     p64 = S.AlgParams(matrix_m=n, matrix_n=n, rank=cfg["rank"], sketch_range=k, sketch_core=s, window=w,
                       kernel_func="gaussrbf", kernel_gamma=gamma, nfeat=d)
-    maps = S.make_maps(p64, ctx)
+    maps = S.make_maps(p64, ctx, dist="normal")
     forms = [f for f in args.forms.split(",") if f in ("fp64", "bf16")] or ["fp64"]
     if "fp64" not in forms:
         forms.insert(0, "fp64")
@@ -383,10 +488,15 @@ def bench_rbf(args, S, torch):
         ctx.set_profiling(False)
         stats = {kname: ctx.kernel_stats(kname) for kname in ("window", "rbf_left", "rbf_right", "sketch_core")}
         stats["rbf_right"] = right_stats
+        sk.low_rank_approx(device=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         U, Sv, V = sk.low_rank_approx(device=True)
+        torch.cuda.synchronize()
+        fin_ms = (time.perf_counter() - t0) * 1e3
         err = rbf_residual(S, Xd, gamma, U, Sv, V, w, ctx)
         sk.close()
-        res[form] = dict(form=form, ms=dt * 1e3, rows_per_s=n / dt, err=err,
+        res[form] = dict(form=form, ms=dt * 1e3, rows_per_s=n / dt, err=err, finalize_ms=fin_ms,
                          kernel_ms={kn: (v[1] / v[0] if v[0] else None) for kn, v in stats.items()},
                          right_avg_ms=stats["rbf_right"][1] / stats["rbf_right"][0],
                          svals_head=[float(x) for x in Sv[:5].cpu()])
@@ -412,7 +522,9 @@ def bench_rbf(args, S, torch):
         "roofline": {"bound": "mfma", "kernel": "rbf_right", "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                      "flops_per_launch": flops_alg, "avg_launch_ms": best["right_avg_ms"]},
+        "finalize_ms": best["finalize_ms"],
         "forms": {f: {"rows_per_s": r["rows_per_s"], "ms_per_step": r["ms"], "rel_fro_err": r["err"],
+                      "finalize_ms": r["finalize_ms"],
                       "err_vs_fp64": r["err"] / base["err"], "speedup_vs_fp64": r["rows_per_s"] / base["rows_per_s"],
                       "kernel_ms": r["kernel_ms"], "svals_head": r["svals_head"]} for f, r in res.items()},
         "cpu_baseline": None,
@@ -440,9 +552,9 @@ def bench_adjoint(args, S, torch):
     ctx = S.Context(0)
     n, d, k, s = cfg["m"], cfg["d"], cfg["k"], cfg["s"]
     gamma = 0.5 / d
-    om = S.GaussDimRedux(k, n, seed=0, ctx=ctx)
-    ph = S.GaussDimRedux(s, n, seed=1, ctx=ctx)
-    ps = S.GaussDimRedux(s, n, seed=2, ctx=ctx)
+    om = S.GaussDimRedux(k, n, seed=0, dist="normal", ctx=ctx)
+    ph = S.GaussDimRedux(s, n, seed=1, dist="normal", ctx=ctx)
+    ps = S.GaussDimRedux(s, n, seed=2, dist="normal", ctx=ctx)
     forms = [f for f in args.forms.split(",") if f in ("fp64", "bf16")] or ["fp64", "bf16"]
     h = k + s
     # tile pass: (F_I R_J^T) with inner 2h, the Gram over d, and H X_J over d
@@ -521,6 +633,8 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
+    ap.add_argument("--no-lownoise", action="store_true",
+                    help="skip the low-noise companion matrix of the default dense workload")
     ap.add_argument("--no-error", action="store_true",
                     help="time the update only: no finalize, residual or optimal-tail passes (counter runs); "
                          "the forms are then ranked by speed alone")
@@ -550,43 +664,64 @@ def main():
         return bench_adjoint(args, S, torch)
     if args.workload.startswith("rbf-"):
         return bench_rbf(args, S, torch)
-    cfg = WORKLOADS[args.workload]
+    return bench_dense(args, S, torch)
+
+
+def bench_dense(args, S, torch):
+    """Every form on the dense workload and, for the default workload, on its
+    low-noise companion (noise standard deviation 1e-4 instead of variance
+    1e-4).  `value` is the fastest form whose rank-r error is within 1.05x of
+    the FP64 form's on every matrix timed."""
+    cfg = dict(WORKLOADS[args.workload], label=args.workload)
+    forms = args.forms.split(",")
+    if not args.no_error and "fp64" not in forms:
+        forms.insert(0, "fp64")
     print(f"[bench] building {args.workload} on cuda:0", file=sys.stderr, flush=True)
     A = make_dense(cfg, torch)
     ctx = S.Context(0)
     p64 = S.AlgParams(matrix_m=cfg["m"], matrix_n=cfg["n"], rank=cfg["rank"], sketch_range=cfg["k"],
                       sketch_core=cfg["s"], window=cfg["window"])
-    maps = S.make_maps(p64, ctx)
-    results = {}
-    for form in args.forms.split(","):
-        try:
-            print(f"[bench] timing form {form}", file=sys.stderr, flush=True)
-            results[form] = run_form(S, ctx, cfg, A, maps, form, args.steps, args.warmup, torch,
-                                     error=not args.no_error)
-            print(f"[bench] {form}: {results[form]['ms_per_step']:.2f} ms/step, err {results[form]['rel_fro_err']}",
-                  file=sys.stderr, flush=True)
-        except S.SkemaError as e:
-            if e.code != 7:
-                raise
-            print(f"[bench] form {form} not available: {e}", file=sys.stderr, flush=True)
-    if "fp64" in results and not args.no_error:
-        base = results["fp64"]
-        best = base
-        for form, r in results.items():
-            if form != "fp64" and r["rel_fro_err"] <= 1.05 * base["rel_fro_err"] and r["rows_per_s"] > best["rows_per_s"]:
-                best = r
-    else:  # no FP64 reference run (profiling a single form): report the fastest form
-        best = max(results.values(), key=lambda r: r["rows_per_s"])
-        base = best
+    maps = S.make_maps(p64, ctx, dist="normal")
+    results = time_forms(S, ctx, cfg, A, maps, forms, args, torch)
     tail = None
     if not args.no_error:
         print("[bench] optimal rank-r tail (subspace iteration)", file=sys.stderr, flush=True)
         tail = optimal_tail(A, cfg["rank"], torch)
+    # the low-noise companion matrix (same spectrum and seeds)
+    low = None
+    low_name = args.workload + "-lownoise"
+    if not args.no_error and not args.no_lownoise and low_name in WORKLOADS:
+        cfg_low = dict(WORKLOADS[low_name], label=low_name)
+        print(f"[bench] building {low_name} on cuda:0", file=sys.stderr, flush=True)
+        A_keep = A
+        A = None
+        del A_keep
+        torch.cuda.empty_cache()
+        A_low = make_dense(cfg_low, torch)
+        res_low = time_forms(S, ctx, cfg_low, A_low, maps, forms, args, torch)
+        print("[bench] optimal rank-r tail of the low-noise matrix", file=sys.stderr, flush=True)
+        tail_low = optimal_tail(A_low, cfg_low["rank"], torch)
+        low = dict(cfg=cfg_low, results=res_low, tail=tail_low)
+        A = A_low
+    # headline: fastest form within 1.05x of FP64's rank-r error on every matrix timed
+    if "fp64" in results and not args.no_error:
+        base = results["fp64"]
+        ok = [f for f, r in results.items() if r["rel_fro_err"] <= 1.05 * base["rel_fro_err"]
+              and (low is None or (f in low["results"] and
+                                   low["results"][f]["rel_fro_err"] <= 1.05 * low["results"]["fp64"]["rel_fro_err"]))]
+        best = max((results[f] for f in ok), key=lambda r: r["rows_per_s"])
+    else:  # no FP64 reference run (profiling a single form): report the fastest form
+        best = max(results.values(), key=lambda r: r["rows_per_s"])
+        base = best
     name, flops = dominant_kernel(cfg)
     form = best["form"]
     mfma_flops = flops * MFMA_PER_FLOP[form]
     achieved = mfma_flops / (best["kernel_avg_ms"] * 1e-3) / 1e12 if best["kernel_avg_ms"] else None
     peak = PEAKS_TFLOPS[form]
+    h2d = None
+    if not args.no_error:
+        print(f"[bench] host-buffer rate of {form}", file=sys.stderr, flush=True)
+        h2d = h2d_inclusive(S, ctx, cfg, A, maps, form, torch)
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
@@ -610,6 +745,8 @@ def main():
                         f"Gaussian maps seeds 0-3, A resident in HBM",
             "m": cfg["m"], "n": cfg["n"], "rank": cfg["rank"], "range": cfg["k"], "core": cfg["s"],
             "window": cfg["window"], "form": form, "parallelism": "single GPU",
+            "selection": "fastest form whose rank-r error is within 1.05x of the FP64 form's on this matrix"
+                         + (" and on the low-noise matrix" if low else ""),
         },
         "roofline": {
             "bound": "mfma", "kernel": name, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -617,15 +754,18 @@ def main():
             "flops_per_launch": mfma_flops, "avg_launch_ms": best["kernel_avg_ms"],
         },
         "cpu_baseline": cpu,
-        "forms": {f: {"rows_per_s": r["rows_per_s"], "ms_per_step": r["ms_per_step"],
-                      "rel_fro_err": r["rel_fro_err"],
-                      "err_vs_fp64": (r["rel_fro_err"] / base["rel_fro_err"]) if r["rel_fro_err"] else None,
-                      "err_vs_optimal": (r["rel_fro_err"] / tail) if (tail and r["rel_fro_err"]) else None,
-                      "speedup_vs_fp64": r["rows_per_s"] / base["rows_per_s"], "kernel_ms": r["kernel_ms"]}
-                  for f, r in results.items()},
+        "finalize_ms": best["finalize_ms"],
+        "h2d_inclusive": h2d,
+        "forms": forms_block(results, base, tail),
         "flops_per_row": flops_per_row(cfg),
         "optimal_rank_r_err": tail,
     }
+    if low is not None:
+        out["lownoise"] = {
+            "workload": f"{low_name}: as above with N(0, {low['cfg']['noise_var']:g}) noise (variance)",
+            "optimal_rank_r_err": low["tail"],
+            "forms": forms_block(low["results"], low["results"]["fp64"], low["tail"]),
+        }
     print(json.dumps(out))
 
 

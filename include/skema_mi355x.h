@@ -267,6 +267,11 @@ int skema_spd_get_state(const skema_spd* sp, double* Y);
  * shift_removal SKEMA_SPD_SHIFT_EPS subtracts the shift eps ||Y||_2 (Tropp et
  * al. 2017, Alg. 3); SKEMA_SPD_SHIFT_REFERENCE subtracts ||Y||_2 as the
  * reference does (:1141-1145), which clips every eigenvalue below ||Y||_2. */
+/* Timings with the reference's SketchySPD keys ({"init":{"omega"},
+ * "update":{"omega","window","daxpy"}, "approx":{"norm2","daxpy","omega",
+ * "update","dpotrf","dgels","dgesvd"}}, src/Skema_SketchySVD.cpp:776-786) as
+ * JSON text; the fused window update's wall time is under update/omega. */
+int skema_spd_timings_json(const skema_spd* sp, char* buf, size_t len);
 #define SKEMA_SPD_SHIFT_EPS 0
 #define SKEMA_SPD_SHIFT_REFERENCE 1
 int skema_spd_low_rank_approx(skema_spd* sp, int shift_removal, double* U, double* S);

@@ -11,7 +11,11 @@
 
 #include <array>
 #include <cstdint>
+#include <cstdio>
+#include <iostream>
+#include <map>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -128,6 +132,18 @@ struct AlgParams {
   double kernel_gamma = 1.0;
   int64_t nfeat = 0;
   int precision = SKEMA_PREC_FP64;
+  // Seeded Gauss maps: U(-a, a) like the reference's fill
+  // (src/Skema_DimRedux.hpp:112-115); SKEMA_DIST_NORMAL selects N(0, 1).
+  int gauss_dist = SKEMA_DIST_UNIFORM;
+  // --sketch-compute-svals-iters: rank-r singular values after every window
+  // into the history's traces (src/Skema_SketchySVD.cpp:208-216).
+  bool sketch_compute_svals_iters = false;
+  // --debug-file: text dumps of the maps, the state after every window and
+  // the finalize intermediates, named <stem>_<name>.txt in the working
+  // directory (src/Skema_SketchySVD.cpp:192-206, :235-245, :368-490).
+  std::string debug_filename;
+  bool debug = false;    // --debug: print the window and X, Y, Z (:218-232)
+  int print_level = 0;   // --print-level: phase banners (:351-544, :587-638)
 
   skema_params to_c() const {
     skema_params p;
@@ -203,7 +219,7 @@ class GaussDimRedux : public DimRedux<GaussDimRedux> {
  public:
   static constexpr bool sparse = false;
   GaussDimRedux(const Context& ctx, int64_t nrow, int64_t ncol, uint64_t seed, bool init_transposed = false,
-                int dist = SKEMA_DIST_NORMAL) {
+                int dist = SKEMA_DIST_UNIFORM) {
     skema_map* m = nullptr;
     check(skema_map_gauss_seeded(ctx.get(), nrow, ncol, seed, dist, init_transposed, &m));
     adopt(m, nrow, ncol, init_transposed);
@@ -228,6 +244,72 @@ class SparseSignDimRedux : public DimRedux<SparseSignDimRedux> {
   }
 };
 
+// ---- text dumps (--debug-file / --debug) ---------------------------------------------
+namespace detail {
+// <stem of the debug file>_<name> in the working directory, as the reference
+// names its dumps (debug_filename.filename().stem() + "_" + name).
+inline std::string debug_name(const std::string& debug_file, const std::string& name) {
+  std::string base = debug_file.substr(debug_file.find_last_of('/') + 1);
+  const size_t dot = base.find_last_of('.');
+  if (dot != std::string::npos && dot > 0) base = base.substr(0, dot);
+  return base + "_" + name;
+}
+// Impl::write (src/Skema_Common.hpp:165-186): "%.16f " per entry, a row per line.
+inline void write_matrix(const std::string& fname, const double* colmajor, int64_t rows, int64_t cols) {
+  FILE* f = std::fopen(fname.c_str(), "w");
+  if (!f) throw Error(SKEMA_ERR_ARG, "cannot write " + fname);
+  for (int64_t i = 0; i < rows; ++i) {
+    for (int64_t j = 0; j < cols; ++j) std::fprintf(f, "%.16f ", colmajor[i + j * rows]);
+    std::fprintf(f, "\n");
+  }
+  std::fclose(f);
+}
+inline void write_vector(const std::string& fname, const std::vector<double>& v) {
+  FILE* f = std::fopen(fname.c_str(), "w");
+  if (!f) throw Error(SKEMA_ERR_ARG, "cannot write " + fname);
+  for (double x : v) std::fprintf(f, "%.16f\n", x);
+  std::fclose(f);
+}
+// MatrixMarket coordinate text of a stored CSR map (1-based indices).
+inline void write_csr_mtx(const std::string& fname, int64_t nrow, int64_t ncol, const std::vector<int64_t>& rp,
+                          const std::vector<int32_t>& ci, const std::vector<double>& v) {
+  FILE* f = std::fopen(fname.c_str(), "w");
+  if (!f) throw Error(SKEMA_ERR_ARG, "cannot write " + fname);
+  std::fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n%lld %lld %lld\n", (long long)nrow,
+               (long long)ncol, (long long)ci.size());
+  for (int64_t r = 0; r < nrow; ++r)
+    for (int64_t e = rp[r]; e < rp[r + 1]; ++e)
+      std::fprintf(f, "%lld %d %.16g\n", (long long)(r + 1), ci[e] + 1, v[e]);
+  std::fclose(f);
+}
+inline void print_matrix(const char* name, const double* colmajor, int64_t rows, int64_t cols) {
+  std::cout << name << " = \n";
+  for (int64_t i = 0; i < rows; ++i) {
+    for (int64_t j = 0; j < cols; ++j) std::cout << " " << colmajor[i + j * rows];
+    std::cout << "\n";
+  }
+  std::cout.flush();
+}
+inline void dump_map(skema_map* h, const std::string& debug_file, const std::string& label) {
+  int64_t nrow = 0, ncol = 0, nnz = 0;
+  int sparse = 0, tr = 0;
+  check(skema_map_info(h, &nrow, &ncol, &nnz, &sparse, &tr));
+  if (sparse) {
+    std::vector<int64_t> rp(nrow + 1);
+    std::vector<int32_t> ci(std::max<int64_t>(nnz, 1));
+    std::vector<double> v(std::max<int64_t>(nnz, 1));
+    check(skema_map_download(h, v.data(), rp.data(), ci.data()));
+    ci.resize(nnz);
+    v.resize(nnz);
+    write_csr_mtx(debug_name(debug_file, label + ".mtx"), nrow, ncol, rp, ci, v);
+  } else {
+    std::vector<double> d((size_t)(nrow * ncol));
+    check(skema_map_download(h, d.data(), nullptr, nullptr));
+    write_matrix(debug_name(debug_file, label + ".txt"), d.data(), nrow, ncol);
+  }
+}
+}  // namespace detail
+
 // ---- SketchySVD ------------------------------------------------------------------
 // MatrixType: DeviceMatrix (dense A), CrsMatrix (CSR A) or PointSet (RBF).
 template <class MatrixType, class DimReduxT>
@@ -248,17 +330,50 @@ class SketchySVD {
     check(skema_sketch_create(ctx_.get(), &cp, maps_[0].handle(), maps_[1].handle(), maps_[2].handle(),
                               maps_[3].handle(), &h));
     h_.reset(h, [](skema_sketch* q) { (void)skema_sketch_destroy(q); });
+    if (!p_.debug_filename.empty()) {  // the maps as stored (src/Skema_DimRedux.hpp:124-126, :264-266)
+      const char* labels[4] = {"Upsilon", "Omega", "Phi", "Psi"};
+      for (int i = 0; i < 4; ++i) detail::dump_map(maps_[i].handle(), p_.debug_filename, labels[i]);
+    }
   }
 
-  // linear_update (src/Skema_SketchySVD.cpp:119-246)
-  void linear_update(const MatrixType& A) { update_all(A); }
+  // linear_update (src/Skema_SketchySVD.cpp:119-246).  The per-window hooks
+  // (--sketch-compute-svals-iters, --debug-file, --debug) run the window
+  // loop here, one library window update at a time; otherwise the library
+  // runs the whole loop.  As in the reference, the single-window case
+  // (window == m, :137-158) has no hooks.
+  void linear_update(const MatrixType& A) {
+    const int64_t m = p_.matrix_m, w = p_.window;
+    const bool hooks = p_.sketch_compute_svals_iters || !p_.debug_filename.empty() || p_.debug;
+    if (!hooks || w >= m) {
+      update_all(A);
+      return;
+    }
+    check(skema_sketch_reset(h_.get()));
+    int64_t ucnt = 0;
+    for (int64_t r0 = 0; r0 < m; r0 += w, ++ucnt) {
+      const int64_t rows = std::min(w, m - r0);
+      update_window(A, r0, rows);
+      if (!p_.debug_filename.empty()) dump_state("." + std::to_string(ucnt));
+      if (p_.sketch_compute_svals_iters) {
+        // the finalize leaves X, Y, Z intact here (the reference's Q = Y
+        // alias would fold this finalize into the later windows, :381)
+        auto usv = low_rank_approx(false);
+        traces_[std::to_string(ucnt)]["svals"] = std::get<1>(usv);
+      }
+      if (p_.debug) {
+        print_window(A, r0, rows);
+        std::vector<double> X, Y, Z;
+        state(X, Y, Z);
+        detail::print_matrix("X", X.data(), k(), p_.matrix_n);
+        detail::print_matrix("Y", Y.data(), p_.matrix_m, k());
+        detail::print_matrix("Z", Z.data(), s(), s());
+      }
+    }
+    if (!p_.debug_filename.empty()) dump_state("");
+  }
 
   // low_rank_approx (src/Skema_SketchySVD.cpp:568-664): U (m x r), S (r), V (n x r)
-  std::tuple<DeviceMatrix, std::vector<double>, DeviceMatrix> low_rank_approx() {
-    DeviceMatrix U(p_.matrix_m, p_.rank), V(p_.matrix_n, p_.rank), S(p_.rank, 1);
-    check(skema_low_rank_approx_device(h_.get(), U.data(), U.ld(), S.data(), V.data(), V.ld()));
-    return {U, S.to_host(), V};
-  }
+  std::tuple<DeviceMatrix, std::vector<double>, DeviceMatrix> low_rank_approx() { return low_rank_approx(true); }
 
   // compute_residuals (src/Skema_SketchySVD.cpp:718-727): the per-triplet
   // residual norms rnrms of the factors (U, S, V) returned by low_rank_approx.
@@ -276,10 +391,122 @@ class SketchySVD {
     check(skema_sketch_timings_json(h_.get(), buf.data(), buf.size()));
     return std::string(buf.data());
   }
+  // The history's traces (src/Skema_SketchySVD.cpp:50, :214-215) as JSON:
+  // {"<window>": {"svals": [...]}, ...}; {} without --sketch-compute-svals-iters.
+  std::string traces_json() const {
+    std::ostringstream os;
+    os.precision(17);
+    os << "{";
+    bool first = true;
+    for (auto& w : traces_) {
+      os << (first ? "" : ",") << "\"" << w.first << "\":{";
+      first = false;
+      bool f2 = true;
+      for (auto& kv : w.second) {
+        os << (f2 ? "" : ",") << "\"" << kv.first << "\":[";
+        f2 = false;
+        for (size_t i = 0; i < kv.second.size(); ++i) os << (i ? "," : "") << kv.second[i];
+        os << "]";
+      }
+      os << "}";
+    }
+    os << "}";
+    return os.str();
+  }
+  // X (k x n), Y (m x k), Z (s x s), column-major, on the host.
+  void state(std::vector<double>& X, std::vector<double>& Y, std::vector<double>& Z) const {
+    X.resize((size_t)(k() * p_.matrix_n));
+    Y.resize((size_t)(p_.matrix_m * k()));
+    Z.resize((size_t)(s() * s()));
+    check(skema_get_state(h_.get(), X.data(), Y.data(), Z.data()));
+  }
+  int64_t k() const { return p_.range(); }
+  int64_t s() const { return p_.core(); }
   skema_sketch* handle() const { return h_.get(); }
 
  private:
-  DimReduxT make(int64_t r, int64_t c, uint64_t seed, bool t) { return DimReduxT(ctx_, r, c, seed, t); }
+  DimReduxT make(int64_t r, int64_t c, uint64_t seed, bool t) {
+    if constexpr (DimReduxT::sparse) return DimReduxT(ctx_, r, c, seed, t);
+    else return DimReduxT(ctx_, r, c, seed, t, p_.gauss_dist);
+  }
+  std::tuple<DeviceMatrix, std::vector<double>, DeviceMatrix> low_rank_approx(bool final_call) {
+    const bool talk = final_call && p_.print_level > 0;
+    if (talk) {
+      std::cout << "\nComputing fixed-rank approximation\n"
+                << "SketchSVD::initial_approx: [P,~] = qr(X^T,0)\n"
+                << "SketchSVD::initial_approx: [Q,~] = qr(Y,0);\n"
+                << "SketchSVD::initial_approx: Phi*Q\n"
+                << "SketchSVD::initial_approx: Psi*P\n"
+                << "SketchSVD::initial_approx: [U1,T1] = qr(Phi*Q,0)\n"
+                << "SketchSVD::initial_approx: [U2,T2] = qr(Psi*P,0)\n"
+                << "SketchSVD::initial_approx: W = T1 \\ (U1^T*Z*U2) / T2^T\n"
+                << "SketchSVD::fixed_rank_approx: [uu,ss,vv] = svd(Z)\n"
+                << "SketchSVD::fixed_rank_approx: U = Q*U\n"
+                << "SketchSVD::fixed_rank_approx: V = P*Vt'" << std::endl;
+    }
+    DeviceMatrix U(p_.matrix_m, p_.rank), V(p_.matrix_n, p_.rank), S(p_.rank, 1);
+    check(skema_low_rank_approx_device(h_.get(), U.data(), U.ld(), S.data(), V.data(), V.ld()));
+    if (final_call && !p_.debug_filename.empty()) {  // the finalize intermediates (:368-490)
+      for (const char* nm : {"P", "Q", "PhiQ", "PsiP", "T1", "T2"}) {
+        int64_t r = 0, c = 0;
+        check(skema_get_checkpoint(h_.get(), nm, nullptr, &r, &c));
+        std::vector<double> buf((size_t)(r * c));
+        check(skema_get_checkpoint(h_.get(), nm, buf.data(), &r, &c));
+        detail::write_matrix(detail::debug_name(p_.debug_filename, std::string(nm) + ".txt"), buf.data(), r, c);
+      }
+    }
+    return {U, S.to_host(), V};
+  }
+  void dump_state(const std::string& suffix) const {
+    std::vector<double> X, Y, Z;
+    state(X, Y, Z);
+    detail::write_matrix(detail::debug_name(p_.debug_filename, "X.txt" + suffix), X.data(), k(), p_.matrix_n);
+    detail::write_matrix(detail::debug_name(p_.debug_filename, "Y.txt" + suffix), Y.data(), p_.matrix_m, k());
+    detail::write_matrix(detail::debug_name(p_.debug_filename, "Z.txt" + suffix), Z.data(), s(), s());
+  }
+  void update_window(const DeviceMatrix& A, int64_t r0, int64_t rows) {
+    check(skema_update_dense(h_.get(), A.data() + r0, A.ld(), r0, rows));
+  }
+  void update_window(const CrsMatrix& A, int64_t r0, int64_t rows) {
+    check(skema_update_csr(h_.get(), A.rowptr() + r0, A.col(), A.val(), r0, rows));
+  }
+  void update_window(const PointSet& P, int64_t r0, int64_t rows) {
+    check(skema_update_rbf(h_.get(), P.X.data(), P.X.ld(), r0, rows));
+  }
+  // the window H (src/Skema_SketchySVD.cpp:220-221)
+  void print_window(const DeviceMatrix& A, int64_t r0, int64_t rows) const {
+    DeviceMatrix H(rows, p_.matrix_n);
+    if (hipMemcpy2D(H.data(), sizeof(double) * rows, A.data() + r0, sizeof(double) * A.ld(), sizeof(double) * rows,
+                    p_.matrix_n, hipMemcpyDeviceToDevice) != hipSuccess)
+      throw Error(SKEMA_ERR_HIP, "print_window: copy failed");
+    auto h = H.to_host();
+    detail::print_matrix("H", h.data(), rows, p_.matrix_n);
+  }
+  void print_window(const CrsMatrix& A, int64_t r0, int64_t rows) const {
+    std::vector<int64_t> rp(rows + 1);
+    if (hipMemcpy(rp.data(), A.rowptr() + r0, sizeof(int64_t) * (rows + 1), hipMemcpyDeviceToHost) != hipSuccess)
+      throw Error(SKEMA_ERR_HIP, "print_window: copy failed");
+    const int64_t nnz = rp[rows] - rp[0];
+    std::vector<int32_t> ci(std::max<int64_t>(nnz, 1));
+    std::vector<double> v(std::max<int64_t>(nnz, 1));
+    if (nnz > 0 && (hipMemcpy(ci.data(), A.col() + rp[0], sizeof(int32_t) * nnz, hipMemcpyDeviceToHost) !=
+                        hipSuccess ||
+                    hipMemcpy(v.data(), A.val() + rp[0], sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess))
+      throw Error(SKEMA_ERR_HIP, "print_window: copy failed");
+    std::cout << "H = \n";
+    for (int64_t r = 0; r < rows; ++r)
+      for (int64_t e = rp[r] - rp[0]; e < rp[r + 1] - rp[0]; ++e)
+        std::cout << " (" << r << ", " << ci[e] << ") " << v[e] << "\n";
+    std::cout.flush();
+  }
+  void print_window(const PointSet& P, int64_t r0, int64_t rows) const {
+    const int64_t n = P.X.extent(0);
+    DeviceMatrix K(rows, n);
+    check(skema_rbf_window(ctx_.get(), P.X.data(), n, P.X.extent(1), P.X.ld(), p_.kernel_gamma, r0, rows, K.data(),
+                           K.ld()));
+    auto h = K.to_host();
+    detail::print_matrix("H", h.data(), rows, n);
+  }
   void update_all(const DeviceMatrix& A) { check(skema_linear_update_dense(h_.get(), A.data(), A.ld())); }
   void update_all(const CrsMatrix& A) {
     check(skema_linear_update_csr(h_.get(), A.rowptr(), A.col(), A.val()));
@@ -304,6 +531,7 @@ class SketchySVD {
   std::vector<DimReduxT> maps_;
   std::shared_ptr<skema_sketch> h_;
   std::vector<double> rnrms_;
+  std::map<std::string, std::map<std::string, std::vector<double>>> traces_;
 };
 
 // ---- SketchySPD ------------------------------------------------------------------
@@ -314,15 +542,30 @@ class SketchySPD {
  public:
   // ctor (src/Skema_SketchySVD.cpp:757-789): Omega = DimReduxT(ncol, range, seeds[0])
   SketchySPD(const AlgParams& p, const Context& ctx = Context(0))
-      : p_(p), ctx_(ctx), omega_(ctx, p.matrix_n, p.range(), p.seeds[0], false) {
+      : p_(p), ctx_(ctx), omega_(make_omega(p, ctx)) {
     skema_params cp = p.to_c();
     skema_spd* h = nullptr;
     check(skema_spd_create(ctx_.get(), &cp, omega_.handle(), &h));
     h_.reset(h, [](skema_spd* q) { (void)skema_spd_destroy(q); });
+    if (!p_.debug_filename.empty()) detail::dump_map(omega_.handle(), p_.debug_filename, "Omega");
   }
 
-  // linear_update (src/Skema_SketchySVD.cpp:790-880), every window sketched
-  void linear_update(const MatrixType& A) { update_all(A); }
+  // linear_update (src/Skema_SketchySVD.cpp:790-880), every window sketched;
+  // --debug-file writes Y at the end (:881-885)
+  void linear_update(const MatrixType& A) {
+    update_all(A);
+    if (!p_.debug_filename.empty()) {
+      std::vector<double> Y((size_t)(p_.matrix_m * p_.range()));
+      check(skema_spd_get_state(h_.get(), Y.data()));
+      detail::write_matrix(detail::debug_name(p_.debug_filename, "Y.txt"), Y.data(), p_.matrix_m, p_.range());
+    }
+  }
+
+  std::string timings_json() const {
+    std::vector<char> buf(8192);
+    check(skema_spd_timings_json(h_.get(), buf.data(), buf.size()));
+    return std::string(buf.data());
+  }
 
   // low_rank_approx (src/Skema_SketchySVD.cpp:934-1151): U (n x r), eigenvalues (r)
   std::tuple<DeviceMatrix, std::vector<double>> low_rank_approx(int shift_removal = SKEMA_SPD_SHIFT_EPS) {
@@ -343,6 +586,10 @@ class SketchySPD {
   skema_spd* handle() const { return h_.get(); }
 
  private:
+  static DimReduxT make_omega(const AlgParams& p, const Context& ctx) {
+    if constexpr (DimReduxT::sparse) return DimReduxT(ctx, p.matrix_n, p.range(), p.seeds[0], false);
+    else return DimReduxT(ctx, p.matrix_n, p.range(), p.seeds[0], false, p.gauss_dist);
+  }
   void update_all(const DeviceMatrix& A) { check(skema_spd_linear_update_dense(h_.get(), A.data(), A.ld())); }
   void update_all(const CrsMatrix& A) {
     check(skema_spd_linear_update_csr(h_.get(), A.rowptr(), A.col(), A.val()));

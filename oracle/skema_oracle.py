@@ -370,9 +370,11 @@ def map_shapes(p: AlgParams):
     }
 
 
-def make_maps(p: AlgParams, sparse_convention: str = "rows", dist: str = "normal"):
+def make_maps(p: AlgParams, sparse_convention: str = "rows", dist: str = "uniform"):
     """Materialise the four maps from p.seeds with the stored orientation.
-    Gauss: gauss_map on the stored shape.  Sparse-sign: 'rows' = reference
+    Gauss: gauss_map on the stored shape, U(-a, a) by default like the
+    reference's fill (src/Skema_DimRedux.hpp:112-115), N(0,1) with
+    dist="normal".  Sparse-sign: 'rows' = reference
     per-stored-row convention; 'cols' = per-column of the LOGICAL map."""
     shapes = map_shapes(p)
     out = {}
@@ -400,7 +402,7 @@ def logical_map(mp) -> np.ndarray:
 class SketchySVD:
     """SketchySVD<M,D> (src/Skema_SketchySVD.hpp:11-64)."""
 
-    def __init__(self, p: AlgParams, maps=None, sparse_convention: str = "rows", dist: str = "normal"):
+    def __init__(self, p: AlgParams, maps=None, sparse_convention: str = "rows", dist: str = "uniform"):
         self.p = p
         self.nrow, self.ncol = p.matrix_m, p.matrix_n
         self.rank = p.rank

@@ -117,6 +117,7 @@ SIGNATURES = {
     "skema_spd_linear_update_csr": (_i32, [_vp, _vp, _vp, _vp]),
     "skema_spd_linear_update_rbf": (_i32, [_vp, _vp, _i64]),
     "skema_spd_get_state": (_i32, [_vp, _vp]),
+    "skema_spd_timings_json": (_i32, [_vp, C.c_char_p, C.c_size_t]),
     "skema_spd_low_rank_approx": (_i32, [_vp, _i32, _vp, _vp]),
     "skema_spd_low_rank_approx_device": (_i32, [_vp, _i32, _vp, _i64, _vp]),
     "skema_residuals_dense": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),

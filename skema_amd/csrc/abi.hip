@@ -2,8 +2,10 @@
 // Every entry point validates its arguments on the host (shapes, leading
 // dimensions, device-pointer extents) before any kernel sees them, converts
 // library errors into status codes, and never throws across the ABI.
+#include <algorithm>
 #include <cstring>
 #include <memory>
+#include <vector>
 #include <sstream>
 
 #include "adjoint.hpp"
@@ -600,8 +602,9 @@ int skema_linear_update_csr(skema_sketch* h, const int64_t* rowptr, const int32_
     set_device(sk.ctx);
     SKEMA_REQUIRE(sk.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "linear_update_csr on an RBF sketch");
     check_csr_window(rowptr, col, val, sk.p.m, sk.ctx->stream);
-    zero_state(sk);
-    // the window loop from a zero state in one pass over the whole CSR
+    // the window loop from a zero state in one pass over the whole CSR; the
+    // pass writes every entry of X, Y and Z, so the state is not cleared first
+    sk.finalized = false;
     update_csr_all(sk, rowptr, col, val);
     sk.ctx->sync_if_needed();
   });
@@ -753,40 +756,39 @@ int skema_sketch_timings_json(const skema_sketch* h, char* buf, size_t len) {
   });
 }
 
+namespace {
+void check_residual_fro_args(int64_t m, int64_t n, int64_t r, const double* A, int64_t lda, const double* U,
+                             int64_t ldu, const double* S, const double* V, int64_t ldv, int64_t window,
+                             const double* err2, const double* norm2) {
+  SKEMA_REQUIRE(err2 && norm2, kErrArg, "residual: null argument");
+  SKEMA_REQUIRE(m > 0 && n > 0 && r >= 0 && window > 0, kErrShape, "residual: bad sizes");
+  check_mat(A, m, n, lda, "residual A");
+  check_mat(U, m, r, ldu, "residual U");
+  check_mat(V, n, r, ldv, "residual V");
+  if (r) check_dev(S, r * sizeof(double), "residual S");
+}
+}  // namespace
+
 int skema_residual_fro_dense(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, const double* A, int64_t lda,
                              const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
                              int64_t window, double* err2, double* norm2) {
   return guarded([&] {
-    SKEMA_REQUIRE(ctx && err2 && norm2, kErrArg, "residual: null argument");
-    SKEMA_REQUIRE(m > 0 && n > 0 && r >= 0 && window > 0, kErrShape, "residual: bad sizes");
+    SKEMA_REQUIRE(ctx, kErrArg, "residual: null argument");
     Ctx* c = &ctx->c;
     set_device(c);
-    check_mat(A, m, n, lda, "residual A");
-    check_mat(U, m, r, ldu, "residual U");
-    check_mat(V, n, r, ldv, "residual V");
-    if (r) check_dev(S, r * sizeof(double), "residual S");
+    check_residual_fro_args(m, n, r, A, lda, U, ldu, S, V, ldv, window, err2, norm2);
     hipStream_t st = c->stream;
     const int64_t w = std::min(window, m);
     DevBuf R((size_t)w * n * sizeof(double)), US((size_t)w * std::max<int64_t>(r, 1) * sizeof(double)),
-        acc(2 * sizeof(double)), red;
-    double e2 = 0.0, a2 = 0.0;
-    for (int64_t r0 = 0; r0 < m; r0 += w) {
-      const int64_t rows = std::min(w, m - r0);
-      copy_matrix(st, rows, n, A + r0, lda, R.as<double>(), rows);
-      sum_squares(st, rows, n, R.as<double>(), rows, acc.as<double>() + 1, &red);
-      if (r > 0) {
-        scale_columns(st, rows, r, U + r0, ldu, S, US.as<double>(), rows);
-        gemm_f64(st, 'N', 'T', rows, n, r, -1.0, US.as<double>(), rows, V, ldv, 1.0, R.as<double>(), rows, &c->gws);
-      }
-      sum_squares(st, rows, n, R.as<double>(), rows, acc.as<double>(), &red);
-      double h[2];
-      SKEMA_HIP(hipMemcpyAsync(h, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-      SKEMA_HIP(hipStreamSynchronize(st));
-      e2 += h[0];
-      a2 += h[1];
-    }
-    *err2 = e2;
-    *norm2 = a2;
+        acc(2 * sizeof(double)), part(kSumSqPartials * sizeof(double));
+    fill_f64(st, acc.as<double>(), 2, 0.0);
+    residual_fro_stream(st, m, n, r, A, lda, U, ldu, S, V, ldv, w, R.as<double>(), US.as<double>(),
+                        part.as<double>(), acc.as<double>());
+    double h[2];
+    SKEMA_HIP(hipMemcpyAsync(h, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    SKEMA_HIP(hipStreamSynchronize(st));
+    *err2 = h[0];
+    *norm2 = h[1];
   });
 }
 
@@ -926,10 +928,20 @@ namespace {
 void spd_check_window(const Spd& sp, int64_t row0, int64_t rows) {
   SKEMA_REQUIRE(rows > 0 && row0 >= 0 && row0 + rows <= sp.p.m, kErrShape, "spd update: window outside [0, m)");
 }
+// The window loop of linear_update; its wall time (synchronous calls) goes
+// to update/omega, the key of the reference's fused-here rmap (the window
+// generator and the axpy are folded into the same kernels).
 template <class Fn>
-void spd_windows(const Spd& sp, Fn&& fn) {
+void spd_windows(Spd& sp, Fn&& fn) {
   const int64_t m = sp.p.m, w = std::min(sp.p.window, m);
+  auto& tu = sp.timings["update"];
+  for (const char* key : {"omega", "window", "daxpy"}) tu.emplace(key, 0.0);
+  const double t0 = now_seconds();
   for (int64_t r0 = 0; r0 < m; r0 += w) fn(r0, std::min(w, m - r0));
+  if (!sp.ctx->async_calls) {
+    SKEMA_HIP(hipStreamSynchronize(sp.ctx->stream));
+    tu["omega"] += now_seconds() - t0;
+  }
 }
 void spd_zero(Spd& sp) { fill_f64(sp.ctx->stream, sp.Y.as<double>(), sp.p.m * sp.k, 0.0); }
 }  // namespace
@@ -1043,6 +1055,30 @@ int skema_spd_low_rank_approx_device(skema_spd* h, int shift_removal, double* U,
     check_dev(S, (size_t)sp.p.rank * sizeof(double), "spd S");
     spd_low_rank_approx(sp, shift_removal, U, ldu, S);
     SKEMA_HIP(hipStreamSynchronize(sp.ctx->stream));
+  });
+}
+
+int skema_spd_timings_json(const skema_spd* h, char* buf, size_t len) {
+  return guarded([&] {
+    SKEMA_REQUIRE(h && buf && len > 0, kErrArg, "spd_timings_json: null argument");
+    std::ostringstream os;
+    os.precision(17);
+    os << "{\"timings\":{";
+    bool f1 = true;
+    for (auto& a : h->s.timings) {
+      os << (f1 ? "" : ",") << "\"" << a.first << "\":{";
+      f1 = false;
+      bool f2 = true;
+      for (auto& b : a.second) {
+        os << (f2 ? "" : ",") << "\"" << b.first << "\":" << b.second;
+        f2 = false;
+      }
+      os << "}";
+    }
+    os << "}}";
+    const std::string js = os.str();
+    SKEMA_REQUIRE(js.size() + 1 <= len, kErrArg, "spd_timings_json: buffer too small");
+    std::memcpy(buf, js.c_str(), js.size() + 1);
   });
 }
 

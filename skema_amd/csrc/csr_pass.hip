@@ -9,178 +9,341 @@
 // pass runs as a handful of large launches instead of W rounds of small
 // ones, with no host synchronisation between windows.
 //
-//   left  (X and W = Phi diag(c) A): sparse maps -> one 64-thread block per
-//         column j of A (CSC), lanes over its nonzeros, FP64 LDS atomics
-//         into the column's accumulators; dense maps -> one wave per column,
-//         lanes own map rows, nonzeros in ascending row order
-//         (deterministic).
-//   right (Y): sparse maps -> one block per 64 rows of A, lanes over their
-//         nonzeros, LDS atomics; dense maps -> one wave per row, lanes own
-//         map rows (deterministic).
-// The sparse-map products are exact to rounding but, through the atomics,
-// not bitwise reproducible from run to run.
+// Sparse (sparse-sign) maps:
+//   CSC of A in two bucketing levels (column tiles, then columns) with LDS
+//   cursors, so the entries of a column come in no fixed order;
+//   left  (X and W = Phi diag(c) A in one pass): one wave per column of A,
+//         lanes over its nonzeros, each nonzero's term c_i a added into the
+//         rows p of the maps' column i;
+//   right (Y): one block per 32 rows of A, threads over their nonzeros.
+//   Both accumulate in LDS in exact two-word fixed point (fixedpoint.hpp),
+//   so every sum is the same whatever order the atomics land in: X, Y, Z
+//   are bitwise reproducible.
+// Dense maps: one wave per column (left) or row (right), lanes own map rows,
+//   nonzeros in ascending row order over a sorted CSC (deterministic).
 //   core  Z = W Psi^T: dense x CSR^T gather (sparse Psi) or the FP64 GEMM.
 #include <vector>
 
 #include "csr_pass.hpp"
+#include "fixedpoint.hpp"
 #include "sparse.hpp"
 #include "util.hpp"
 
 namespace skema {
 namespace {
 
-constexpr int kPad = 65;  // LDS accumulator stride (doubles) per map row: lanes spread over banks
+// ---- CSC of A by two bucketing levels (sparse-map path) --------------------
+// Level 1: blocks of kL1Rows rows count their nonzeros per column tile (nt
+// tiles of tc columns) in LDS; an exclusive scan over (tile, block) gives
+// each block a contiguous run in every tile's bucket; the scatter pass walks
+// the nonzeros again (coalesced) and writes (row, column, value) into the
+// runs.  Level 2: one block per tile counts the tile's nonzeros per column,
+// scans the counts in LDS into the column pointers and places each nonzero.
+// Runs are filled through LDS cursors, so the entries of a column come in no
+// fixed order; the products below do not depend on it.
+constexpr int kL1Rows = 256;
+constexpr int kMaxTiles = 256;
+constexpr int kMaxTileCols = 4096;
 
-// out(p, j) for p in [p0, p0 + P): sum over column j's nonzeros (i, a) of
-// a * cw[i / window] * Mt(i, p), Mt = CSR of the map's transpose.  One
-// 256-thread block per column: threads take the column's nonzeros, walk their
-// map entries, and add into the column's accumulators in LDS (FP64 LDS
-// atomics: the order of the additions, and so the last bits, may vary from
-// run to run); the column is then written contiguously.
-__global__ __launch_bounds__(256) void k_left_all_sparse(const int64_t* cptr, const int32_t* crow, const double* cval,
-                                                        const int64_t* mrp, const int32_t* mci, const double* mv,
-                                                        const double* cw, int64_t window, int64_t p0, int64_t P,
-                                                        double* out, int64_t ldo) {
-  extern __shared__ double acc[];  // P doubles
-  const int tid = threadIdx.x;
-  const int64_t j = blockIdx.x;
-  for (int64_t p = tid; p < P; p += 256) acc[p] = 0.0;
+__global__ __launch_bounds__(256) void k_l1_count(const int64_t* rowptr, int64_t rows, const int32_t* col, int tc,
+                                                 int nt, int nb, int* cnt) {
+  __shared__ int hist[kMaxTiles];
+  for (int t = threadIdx.x; t < nt; t += 256) hist[t] = 0;
   __syncthreads();
-  for (int64_t e = cptr[j] + tid; e < cptr[j + 1]; e += 256) {
-    const int64_t i = crow[e];
-    const double a = cval[e] * cw[(uint32_t)i / (uint32_t)window];
-    for (int64_t f = mrp[i]; f < mrp[i + 1]; ++f) {
-      const int64_t p = (int64_t)mci[f] - p0;
-      if (p >= 0 && p < P) atomicAdd(&acc[p], a * mv[f]);
+  const int64_t r0 = (int64_t)blockIdx.x * kL1Rows;
+  const int64_t r1 = (r0 + kL1Rows < rows) ? r0 + kL1Rows : rows;
+  for (int64_t e = rowptr[r0] + threadIdx.x; e < rowptr[r1]; e += 256) atomicAdd(&hist[col[e] / tc], 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nt; t += 256) cnt[(int64_t)t * nb + blockIdx.x] = hist[t];
+}
+
+// Per tile t: exclusive scan of its counts over the row blocks (contiguous,
+// cnt[t nb .. t nb + nb)) into off, and the tile's total into tot[t].
+__global__ __launch_bounds__(1024) void k_tile_scan(const int* cnt, int nb, int* off, int* tot) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int* c = cnt + (int64_t)t * nb;
+  int* o = off + (int64_t)t * nb;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += 1024) {
+    const int v = (b0 + tid < nb) ? c[b0 + tid] : 0;
+    int x = v;  // inclusive scan within the wave
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(x, d);
+      if (lane >= d) x += y;
     }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int before = carry;
+    for (int i = 0; i < w; ++i) before += wsum[i];
+    if (b0 + tid < nb) o[b0 + tid] = before + x - v;
+    __syncthreads();
+    if (tid == 1023) carry = before + x;
+    __syncthreads();
   }
+  if (tid == 0) tot[t] = carry;
+}
+
+// base[t] = sum of tot[0 .. t) (nt <= 1024, one block); base[nt] = nnz.
+__global__ __launch_bounds__(1024) void k_tile_base(const int* tot, int nt, int* base) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  part[tid] = tid < nt ? tot[tid] : 0;
   __syncthreads();
-  for (int64_t p = tid; p < P; p += 256) out[(p0 + p) + j * ldo] = acc[p];
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  if (tid <= nt) base[tid] = tid ? part[tid - 1] : 0;
 }
 
-// The z entries of ELL row c: with ZC = 8 (the sparse-sign default) two
-// 16-byte loads issued together, otherwise one dword per entry.
-template <int ZC>
-__device__ __forceinline__ void ell_row(const uint32_t* ell, int64_t c, int z, uint32_t (&u)[16]) {
-  if constexpr (ZC == 8) {
-    const uint4 a = *reinterpret_cast<const uint4*>(ell + c * 8);
-    const uint4 b = *reinterpret_cast<const uint4*>(ell + c * 8 + 4);
-    u[0] = a.x; u[1] = a.y; u[2] = a.z; u[3] = a.w;
-    u[4] = b.x; u[5] = b.y; u[6] = b.z; u[7] = b.w;
-  } else {
-    for (int f = 0; f < z; ++f) u[f] = ell[c * z + f];
+__global__ __launch_bounds__(256) void k_l1_scatter(const int64_t* rowptr, int64_t rows, const int32_t* col,
+                                                   const double* val, int tc, int nt, int nb, const int* off,
+                                                   const int* base, int32_t* brow, int32_t* bcol, double* bval) {
+  __shared__ int cur[kMaxTiles];
+  __shared__ int64_t rp[kL1Rows + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * kL1Rows;
+  const int nr = (int)((r0 + kL1Rows < rows) ? kL1Rows : rows - r0);
+  for (int t = threadIdx.x; t < nt; t += 256) cur[t] = base[t] + off[(int64_t)t * nb + blockIdx.x];
+  for (int i = threadIdx.x; i <= nr; i += 256) rp[i] = rowptr[r0 + i];
+  __syncthreads();
+  int r = 0;  // row of the thread's current nonzero (advances monotonically)
+  for (int64_t e = rp[0] + threadIdx.x; e < rp[nr]; e += 256) {
+    while (rp[r + 1] <= e) ++r;
+    const int32_t j = col[e];
+    const int pos = atomicAdd(&cur[j / tc], 1);
+    brow[pos] = (int32_t)(r0 + r);
+    bcol[pos] = j;
+    bval[pos] = val[e];
   }
 }
 
-// The same with the map's transpose in ELL form (z entries per row,
-// u = row | sign << 31, magnitude mag): one contiguous 4z-byte read per
-// nonzero.  One wave per column (four per block, each with its own LDS
-// accumulator of P doubles): a column holds a few hundred nonzeros, so a
-// block per column would leave most of its lanes idle.
-template <int ZC>
-__global__ __launch_bounds__(256) void k_left_all_ell(const int64_t* cptr, const int32_t* crow, const double* cval,
-                                                      int64_t ncols, const uint32_t* ell, int z, double mag,
-                                                      const double* cw, int64_t window, int64_t p0, int64_t P,
-                                                      double* out, int64_t ldo) {
-  extern __shared__ double acc_all[];  // 4 x P doubles
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t j = (int64_t)blockIdx.x * 4 + wave;
-  double* acc = acc_all + wave * P;
-  for (int64_t p = lane; p < P; p += 64) acc[p] = 0.0;
-  if (j < ncols) {
-    for (int64_t e = cptr[j] + lane; e < cptr[j + 1]; e += 64) {
-      const int64_t i = crow[e];
-      const double a = cval[e] * cw[(uint32_t)i / (uint32_t)window] * mag;
-      uint32_t uu[16];
-      ell_row<ZC>(ell, i, z, uu);
-      const int zz = ZC ? ZC : z;
-#pragma unroll
-      for (int f = 0; f < (ZC ? ZC : 16); ++f) {
-        if (f >= zz) break;
-        const uint32_t u = uu[f];
-        const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
-        if (p >= 0 && p < P) atomicAdd(&acc[p], (u >> 31) ? -a : a);
+__global__ __launch_bounds__(1024) void k_l2_place(const int* base, int nt, int tc, int64_t cols, int64_t nnz,
+                                                  const int32_t* brow, const int32_t* bcol, const double* bval,
+                                                  int64_t* cptr, int32_t* crow, double* cval) {
+  __shared__ int cnt[kMaxTileCols];
+  __shared__ int part[1024];
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int64_t j0 = (int64_t)t * tc;
+  const int w = (int)((j0 + tc < cols) ? tc : cols - j0);
+  const int64_t b0 = base[t];
+  const int64_t b1 = (t + 1 < nt) ? base[t + 1] : nnz;
+  for (int i = tid; i < w; i += 1024) cnt[i] = 0;
+  __syncthreads();
+  for (int64_t e = b0 + tid; e < b1; e += 1024) atomicAdd(&cnt[bcol[e] - j0], 1);
+  __syncthreads();
+  // exclusive scan of cnt[0 .. w) into cursors (chunk sums, then a scan of those)
+  const int chunk = (w + 1023) / 1024;
+  const int cb = tid * chunk, ce = (cb + chunk < w) ? cb + chunk : w;
+  int sum = 0;
+  for (int i = cb; i < ce; ++i) sum += cnt[i];
+  part[tid] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = tid >= o ? part[tid - o] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = (int)b0 + (tid ? part[tid - 1] : 0);
+  for (int i = cb; i < ce; ++i) {
+    const int v = cnt[i];
+    cnt[i] = run;  // now the cursor of column j0 + i
+    cptr[j0 + i] = run;
+    run += v;
+  }
+  if (t == nt - 1 && tid == 0) cptr[cols] = nnz;
+  __syncthreads();
+  for (int64_t e = b0 + tid; e < b1; e += 1024) {
+    const int pos = atomicAdd(&cnt[bcol[e] - j0], 1);
+    crow[pos] = brow[e];
+    cval[pos] = bval[e];
+  }
+}
+
+// ---- sparse-map products in exact fixed point --------------------------------
+// A sparse logical map seen by its columns: the ELL form of its transpose
+// (z entries of one magnitude per column: ell[c z + f] = row | sign << 31),
+// or the CSR of its transpose (rp, ci, v).  Either way at most zmax entries
+// per column (rows may repeat) and |v| <= vmax.
+struct SpMapCols {
+  const uint32_t* ell = nullptr;
+  int z = 0;
+  double mag = 1.0;
+  const int64_t* rp = nullptr;
+  const int32_t* ci = nullptr;
+  const double* v = nullptr;
+  double vmax = 1.0;
+  int64_t zmax = 1;
+};
+
+__device__ inline double wave_max(double x) {
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+  return x;
+}
+
+// Adds term t (ELL: the unsigned term, scaled by the entry's sign; CSR: t
+// times the entry's value) of column c of map M into the accumulator pairs
+// acc[2 (p - p0)] for rows p of M in [p0, p0 + pn).
+template <bool ELL>
+__device__ inline void fx_scatter(const SpMapCols& M, int64_t c, double t, long long H, long long L,
+                                  const FxScale& fs, int64_t p0, int64_t pn, unsigned long long* acc) {
+  if constexpr (ELL) {
+    for (int f = 0; f < M.z; ++f) {
+      const uint32_t u = M.ell[c * M.z + f];
+      const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
+      if (p >= 0 && p < pn) {
+        if (u >> 31) fx_add_lds(acc + 2 * p, -H, -L);
+        else fx_add_lds(acc + 2 * p, H, L);
+      }
+    }
+  } else {
+    for (int64_t f = M.rp[c]; f < M.rp[c + 1]; ++f) {
+      const int64_t p = (int64_t)M.ci[f] - p0;
+      if (p >= 0 && p < pn) {
+        long long h, l;
+        fx_split(t * M.v[f], fs, h, l);
+        fx_add_lds(acc + 2 * p, h, l);
       }
     }
   }
-  __syncthreads();  // the wave's LDS atomics are complete before it reads back
-  if (j < ncols)
-    for (int64_t p = lane; p < P; p += 64) out[(p0 + p) + j * ldo] = acc[p];
 }
 
-// Y(i, p) = nu Y(i, p) + eta * sum over row i's nonzeros (j, a) of a * Mt(j, p)
-// for p in [p0, p0 + P), P <= PCH.  One 256-thread block per 64 rows:
-// threads take the block's nonzeros and add into acc[p][row] (LDS atomics),
-// then the first wave writes the rows, consecutive lanes consecutive rows.
-template <int PCH>
-__global__ __launch_bounds__(256) void k_right_all_sparse(const int64_t* rowptr, const int32_t* col, const double* val,
-                                                         int64_t rows, const int64_t* mrp, const int32_t* mci,
-                                                         const double* mv, int64_t p0, int64_t P, double eta,
-                                                         double nu, double* Y, int64_t ldy) {
-  __shared__ double acc[PCH * kPad];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int64_t r0 = (int64_t)blockIdx.x * 64;
-  const int64_t r1 = (r0 + 64 < rows) ? r0 + 64 : rows;
-  for (int e = tid; e < PCH * kPad; e += 256) acc[e] = 0.0;
+// Left products over the CSC of A: out1 rows [p1, p1 + n1) of M1 diag(c) A
+// and out2 rows [p2, p2 + n2) of M2 diag(c) A (n2 may be 0), with c_i =
+// cw[i / window].  One wave per column j: a first pass takes the column's
+// largest term |c_i a| (the fixed-point scale of its accumulators), a second
+// adds every term into the n1 + n2 accumulator pairs of the wave in LDS;
+// ELL terms carry the map magnitude only at the end.  Four waves per block,
+// columns in a block-strided loop with a uniform trip count (barriers).
+template <bool ELL>
+__global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int32_t* crow, const double* cval,
+                                                int64_t ncols, const double* cw, int64_t window, SpMapCols M1,
+                                                int64_t p1, int64_t n1, double* out1, int64_t ld1, SpMapCols M2,
+                                                int64_t p2, int64_t n2, double* out2, int64_t ld2) {
+  extern __shared__ unsigned long long fxacc[];  // 4 waves x 2 (n1 + n2)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t na = n1 + n2;
+  unsigned long long* acc = fxacc + (int64_t)wave * 2 * na;
+  const int64_t groups = (ncols + 3) / 4;
+  for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+    const int64_t j = 4 * g + wave;
+    for (int64_t q = lane; q < 2 * na; q += 64) acc[q] = 0ull;
+    FxScale fs;
+    if (j < ncols) {
+      const int64_t e0 = cptr[j], e1 = cptr[j + 1];
+      double mx = 0.0;
+      for (int64_t e = e0 + lane; e < e1; e += 64) {
+        const int64_t i = crow[e];
+        mx = fmax(mx, fabs(cval[e] * cw[(uint32_t)i / (uint32_t)window]));
+        if (isnan(cval[e] * cw[(uint32_t)i / (uint32_t)window])) mx = __builtin_inf();
+      }
+      mx = wave_max(mx);
+      // a target receives at most zmax terms per nonzero (entries of one map
+      // column may repeat a row)
+      const int64_t terms = (e1 - e0) * (M1.zmax > M2.zmax ? M1.zmax : M2.zmax);
+      fs = fx_scale(ELL ? mx : mx * (M1.vmax > M2.vmax ? M1.vmax : M2.vmax), terms);
+      if (fs.finite) {
+        for (int64_t e = e0 + lane; e < e1; e += 64) {
+          const int64_t i = crow[e];
+          const double t = cval[e] * cw[(uint32_t)i / (uint32_t)window];
+          long long H = 0, L = 0;
+          if (ELL) fx_split(t, fs, H, L);
+          fx_scatter<ELL>(M1, i, t, H, L, fs, p1, n1, acc);
+          if (n2) fx_scatter<ELL>(M2, i, t, H, L, fs, p2, n2, acc + 2 * n1);
+        }
+      }
+    }
+    __syncthreads();  // the wave's LDS atomics are complete before it reads back
+    if (j < ncols) {
+      for (int64_t q = lane; q < n1; q += 64)
+        out1[(p1 + q) + j * ld1] = (ELL ? M1.mag : 1.0) * fx_value((long long)acc[2 * q], (long long)acc[2 * q + 1], fs);
+      for (int64_t q = lane; q < n2; q += 64)
+        out2[(p2 + q) + j * ld2] =
+            (ELL ? M2.mag : 1.0) * fx_value((long long)acc[2 * (n1 + q)], (long long)acc[2 * (n1 + q) + 1], fs);
+    }
+    __syncthreads();  // read back before the next column clears the accumulators
+  }
+}
+
+// Y(i, p) = eta * sum over row i's nonzeros (j, a) of a * M(p, j) for p in
+// [p0, p0 + pn), pn <= kRightSlots.  One block per kRightRows rows: a first
+// pass (one thread per row) takes the row's largest |a| (its scale), then
+// the threads take the block's nonzeros and add into acc[slot][row] pairs;
+// the rows are written out coalesced (consecutive threads, consecutive rows).
+constexpr int kRightRows = 32;
+constexpr int kRightSlots = 128;
+
+template <bool ELL>
+__global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const int32_t* col, const double* val,
+                                                 int64_t rows, SpMapCols M, int64_t p0, int64_t pn, double eta,
+                                                 double* Y, int64_t ldy) {
+  extern __shared__ unsigned long long fxacc[];  // 2 x pn x kRightRows
+  __shared__ int64_t rp[kRightRows + 1];
+  __shared__ unsigned long long rmax[kRightRows];  // bits of the row's largest |a| (non-negative doubles order as integers)
+  __shared__ FxScale rs[kRightRows];
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * kRightRows;
+  const int nr = (int)((r0 + kRightRows < rows) ? kRightRows : rows - r0);
+  for (int q = tid; q < 2 * pn * kRightRows; q += 256) fxacc[q] = 0ull;
+  for (int i = tid; i <= nr; i += 256) rp[i] = rowptr[r0 + i];
+  if (tid < kRightRows) rmax[tid] = 0ull;
   __syncthreads();
-  const int64_t e0 = rowptr[r0], e1 = rowptr[r1];
-  int64_t r = r0;  // row of the thread's current nonzero (rows advance monotonically)
-  for (int64_t e = e0 + tid; e < e1; e += 256) {
-    while (rowptr[r + 1] <= e) ++r;
-    const int64_t jc = col[e];
-    const double a = val[e];
-    for (int64_t f = mrp[jc]; f < mrp[jc + 1]; ++f) {
-      const int64_t p = (int64_t)mci[f] - p0;
-      if (p >= 0 && p < P) atomicAdd(&acc[p * kPad + (r - r0)], a * mv[f]);
+  {
+    int r = 0;
+    for (int64_t e = rp[0] + tid; e < rp[nr]; e += 256) {
+      while (rp[r + 1] <= e) ++r;
+      atomicMax(&rmax[r], (unsigned long long)__double_as_longlong(fabs(val[e])));  // NaN orders above inf
     }
   }
   __syncthreads();
-  const int64_t i = r0 + lane;
-  if (tid < 64 && i < rows) {
-    for (int64_t p = 0; p < P; ++p) {
-      double* y = Y + i + (p0 + p) * ldy;
-      *y = (nu != 0.0) ? nu * *y + eta * acc[p * kPad + lane] : eta * acc[p * kPad + lane];
-    }
+  if (tid < nr) {
+    const double mx = __longlong_as_double((long long)rmax[tid]);
+    const int64_t cnt = rp[tid + 1] - rp[tid];
+    rs[tid] = fx_scale(ELL ? mx : mx * M.vmax, cnt * M.zmax);
   }
-}
-
-template <int PCH, int ZC>
-__global__ __launch_bounds__(256) void k_right_all_ell(const int64_t* rowptr, const int32_t* col, const double* val,
-                                                       int64_t rows, const uint32_t* ell, int z, double mag,
-                                                       int64_t p0, int64_t P, double eta, double nu, double* Y,
-                                                       int64_t ldy) {
-  __shared__ double acc[PCH * kPad];
-  __shared__ int64_t rp[65];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t r0 = (int64_t)blockIdx.x * 64;
-  const int64_t r1 = (r0 + 64 < rows) ? r0 + 64 : rows;
-  for (int e = tid; e < PCH * kPad; e += 256) acc[e] = 0.0;
-  if (tid <= r1 - r0) rp[tid] = rowptr[r0 + tid];
   __syncthreads();
-  const int64_t e0 = rp[0], e1 = rp[r1 - r0];
-  int r = 0;  // block-relative row of the thread's current nonzero (advances monotonically, LDS reads)
-  for (int64_t e = e0 + tid; e < e1; e += 256) {
+  int r = 0;
+  for (int64_t e = rp[0] + tid; e < rp[nr]; e += 256) {
     while (rp[r + 1] <= e) ++r;
-    const double a = val[e] * mag;
-    uint32_t uu[16];
-    ell_row<ZC>(ell, (int64_t)col[e], z, uu);
-    const int zz = ZC ? ZC : z;
-#pragma unroll
-    for (int f = 0; f < (ZC ? ZC : 16); ++f) {
-      if (f >= zz) break;
-      const uint32_t u = uu[f];
-      const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
-      if (p >= 0 && p < P) atomicAdd(&acc[p * kPad + r], (u >> 31) ? -a : a);
+    const FxScale fs = rs[r];
+    if (!fs.finite) continue;
+    const int64_t j = col[e];
+    const double t = val[e];
+    if constexpr (ELL) {
+      long long H, L;
+      fx_split(t, fs, H, L);
+      for (int f = 0; f < M.z; ++f) {
+        const uint32_t u = M.ell[j * M.z + f];
+        const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
+        if (p >= 0 && p < pn) {
+          unsigned long long* a = fxacc + 2 * (p * kRightRows + r);
+          if (u >> 31) fx_add_lds(a, -H, -L);
+          else fx_add_lds(a, H, L);
+        }
+      }
+    } else {
+      for (int64_t f = M.rp[j]; f < M.rp[j + 1]; ++f) {
+        const int64_t p = (int64_t)M.ci[f] - p0;
+        if (p >= 0 && p < pn) {
+          long long h, l;
+          fx_split(t * M.v[f], fs, h, l);
+          fx_add_lds(fxacc + 2 * (p * kRightRows + r), h, l);
+        }
+      }
     }
   }
   __syncthreads();
-  // wave w writes columns p = w, w + 4, ...: 64 consecutive rows per store
-  const int64_t i = r0 + lane;
-  if (i < rows) {
-    for (int64_t p = wave; p < P; p += 4) {
-      double* y = Y + i + (p0 + p) * ldy;
-      *y = (nu != 0.0) ? nu * *y + eta * acc[p * kPad + lane] : eta * acc[p * kPad + lane];
-    }
+  const double scale = eta * (ELL ? M.mag : 1.0);
+  for (int q = tid; q < pn * kRightRows; q += 256) {
+    const int rr = q % kRightRows;
+    const int64_t p = q / kRightRows;
+    if (rr < nr)
+      Y[(r0 + rr) + (p0 + p) * ldy] = scale * fx_value((long long)fxacc[2 * q], (long long)fxacc[2 * q + 1], rs[rr]);
   }
 }
 
@@ -247,28 +410,120 @@ __global__ __launch_bounds__(256) void k_right_all_dense(const int64_t* rowptr, 
 
 unsigned nblocks(int64_t work, int64_t per) { return (unsigned)std::max<int64_t>(1, cdiv(work, per)); }
 
-// Sparse left product over map rows [0, P) in LDS chunks of up to 4096 rows
-// (ELL form of the map's transpose when it has one).
-void left_sparse(hipStream_t st, const CscWindow& H, const Map& M, int64_t P, const double* cw, int64_t window,
-                 double* out, int64_t ldo) {
-  if (H.cols == 0) return;
-  const Csr& Mt = M.logical_csr_t();
-  const int64_t chunk = M.ell_zeta > 0 ? 1024 : 4096;
-  for (int64_t p0 = 0; p0 < P; p0 += chunk) {
-    const int64_t pc = std::min<int64_t>(chunk, P - p0);
-    if (M.ell_zeta == 8)
-      hipLaunchKernelGGL(k_left_all_ell<8>, dim3(nblocks(H.cols, 4)), dim3(256), (size_t)4 * pc * sizeof(double), st,
-                         H.ptr, H.row, H.val, H.cols, M.ell.as<uint32_t>(), M.ell_zeta, M.ell_mag, cw, window, p0, pc,
-                         out, ldo);
-    else if (M.ell_zeta > 0)
-      hipLaunchKernelGGL(k_left_all_ell<0>, dim3(nblocks(H.cols, 4)), dim3(256), (size_t)4 * pc * sizeof(double), st,
-                         H.ptr, H.row, H.val, H.cols, M.ell.as<uint32_t>(), M.ell_zeta, M.ell_mag, cw, window, p0, pc,
-                         out, ldo);
-    else
-      hipLaunchKernelGGL(k_left_all_sparse, dim3((unsigned)H.cols), dim3(256), (size_t)pc * sizeof(double), st,
-                         H.ptr, H.row, H.val, Mt.rp(), Mt.ci(), Mt.v(), cw, window, p0, pc, out, ldo);
-    check_launch("k_left_all_sparse");
+SpMapCols map_cols(const Map& M) {
+  SpMapCols c;
+  if (M.ell_zeta > 0) {
+    c.ell = M.ell.as<uint32_t>();
+    c.z = M.ell_zeta;
+    c.mag = M.ell_mag;
+    c.vmax = M.ell_mag;
+    c.zmax = M.ell_zeta;
+  } else {
+    const Csr& t = M.logical_csr_t();
+    c.rp = t.rp();
+    c.ci = t.ci();
+    c.v = t.v();
+    c.vmax = M.sp_vmax;
+    c.zmax = M.sp_zmax > 0 ? M.sp_zmax : 1;
   }
+  return c;
+}
+
+// Accumulator pairs per wave of k_left_fx (4 waves: 64 KB of LDS).
+constexpr int64_t kLeftSlots = 512;
+
+void launch_left_fx(hipStream_t st, bool ell, const CscWindow& H, const double* cw, int64_t window,
+                    const SpMapCols& M1, int64_t p1, int64_t n1, double* o1, int64_t ld1, const SpMapCols& M2,
+                    int64_t p2, int64_t n2, double* o2, int64_t ld2) {
+  const int64_t groups = cdiv(H.cols, 4);
+  const unsigned grid = (unsigned)std::min<int64_t>(groups, 8192);
+  const size_t lds = (size_t)4 * 2 * (n1 + n2) * sizeof(unsigned long long);
+  if (ell)
+    hipLaunchKernelGGL(k_left_fx<true>, dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, M1,
+                       p1, n1, o1, ld1, M2, p2, n2, o2, ld2);
+  else
+    hipLaunchKernelGGL(k_left_fx<false>, dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, M1,
+                       p1, n1, o1, ld1, M2, p2, n2, o2, ld2);
+  check_launch("k_left_fx");
+}
+
+// X = M1 diag(c) A (k1 rows) and W = M2 diag(c) A (k2 rows): one pass over the
+// CSC when both maps have the same form and their rows fit one wave's
+// accumulators, otherwise one pass per map and chunk of rows.
+void left_sparse_fx(hipStream_t st, const CscWindow& H, const Map& Ma, int64_t ka, double* oa, int64_t lda,
+                    const Map& Mb, int64_t kb, double* ob, int64_t ldb, const double* cw, int64_t window) {
+  if (H.cols == 0) return;
+  const SpMapCols ca = map_cols(Ma), cb = map_cols(Mb);
+  const bool ea = Ma.ell_zeta > 0, eb = Mb.ell_zeta > 0;
+  if (ea == eb && ka + kb <= kLeftSlots) {
+    launch_left_fx(st, ea, H, cw, window, ca, 0, ka, oa, lda, cb, 0, kb, ob, ldb);
+    return;
+  }
+  for (int64_t p0 = 0; p0 < ka; p0 += kLeftSlots)
+    launch_left_fx(st, ea, H, cw, window, ca, p0, std::min(kLeftSlots, ka - p0), oa, lda, ca, 0, 0, nullptr, 1);
+  for (int64_t p0 = 0; p0 < kb; p0 += kLeftSlots)
+    launch_left_fx(st, eb, H, cw, window, cb, p0, std::min(kLeftSlots, kb - p0), ob, ldb, cb, 0, 0, nullptr, 1);
+}
+
+// Y = eta A M^T by row blocks, in chunks of kRightSlots map rows.
+void right_sparse_fx(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val, int64_t m,
+                     const Map& M, int64_t k, double eta, double* Y, int64_t ldy) {
+  const SpMapCols c = map_cols(M);
+  const dim3 grid((unsigned)cdiv(m, kRightRows));
+  for (int64_t p0 = 0; p0 < k; p0 += kRightSlots) {
+    const int64_t pn = std::min<int64_t>(kRightSlots, k - p0);
+    const size_t lds = (size_t)2 * pn * kRightRows * sizeof(unsigned long long);
+    if (M.ell_zeta > 0)
+      hipLaunchKernelGGL(k_right_fx<true>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
+    else
+      hipLaunchKernelGGL(k_right_fx<false>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
+    check_launch("k_right_fx");
+  }
+}
+
+// CSC of the whole CSR by the two bucketing levels (entries of a column in no
+// fixed order); false when a column tile would exceed kMaxTileCols.
+bool csc_by_buckets(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val, int64_t m,
+                    int64_t n, int64_t nnz, DevBuf& scratch, CscWindow& H) {
+  int64_t nt = std::min<int64_t>(kMaxTiles, n);
+  const int64_t tc = cdiv(n, nt);
+  nt = cdiv(n, tc);
+  if (tc > kMaxTileCols || nnz <= 0) return false;
+  const int64_t nb = cdiv(m, kL1Rows);
+  SKEMA_REQUIRE(nt * nb < ((int64_t)1 << 31), kErrShape, "linear_update_csr: too many row blocks");
+  const size_t o_off = align_up((size_t)nt * nb * sizeof(int), 256);
+  const size_t o_tot = o_off + align_up((size_t)nt * nb * sizeof(int), 256);
+  const size_t o_base = o_tot + align_up((size_t)nt * sizeof(int), 256);
+  const size_t o_brow = o_base + align_up((size_t)(nt + 1) * sizeof(int), 256);
+  const size_t o_bcol = o_brow + align_up((size_t)nnz * sizeof(int32_t), 256);
+  const size_t o_bval = o_bcol + align_up((size_t)nnz * sizeof(int32_t), 256);
+  const size_t o_ptr = o_bval + align_up((size_t)nnz * sizeof(double), 256);
+  const size_t o_row = o_ptr + align_up((size_t)(n + 1) * sizeof(int64_t), 256);
+  const size_t o_val = o_row + align_up((size_t)nnz * sizeof(int32_t), 256);
+  scratch.reserve(o_val + (size_t)nnz * sizeof(double));
+  char* b = scratch.as<char>();
+  int* cnt = reinterpret_cast<int*>(b);
+  int* off = reinterpret_cast<int*>(b + o_off);
+  int* tot = reinterpret_cast<int*>(b + o_tot);
+  int* base = reinterpret_cast<int*>(b + o_base);
+  int32_t* brow = reinterpret_cast<int32_t*>(b + o_brow);
+  int32_t* bcol = reinterpret_cast<int32_t*>(b + o_bcol);
+  double* bval = reinterpret_cast<double*>(b + o_bval);
+  H.rows = m;
+  H.cols = n;
+  H.nnz = nnz;
+  H.ptr = reinterpret_cast<int64_t*>(b + o_ptr);
+  H.row = reinterpret_cast<int32_t*>(b + o_row);
+  H.val = reinterpret_cast<double*>(b + o_val);
+  hipLaunchKernelGGL(k_l1_count, dim3((unsigned)nb), dim3(256), 0, st, rowptr, m, col, (int)tc, (int)nt, (int)nb, cnt);
+  hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)nt), dim3(1024), 0, st, cnt, (int)nb, off, tot);
+  hipLaunchKernelGGL(k_tile_base, dim3(1), dim3(1024), 0, st, tot, (int)nt, base);
+  hipLaunchKernelGGL(k_l1_scatter, dim3((unsigned)nb), dim3(256), 0, st, rowptr, m, col, val, (int)tc, (int)nt,
+                     (int)nb, off, base, brow, bcol, bval);
+  hipLaunchKernelGGL(k_l2_place, dim3((unsigned)nt), dim3(1024), 0, st, base, (int)nt, (int)tc, n, nnz, brow, bcol,
+                     bval, H.ptr, H.row, H.val);
+  check_launch("csc_by_buckets");
+  return true;
 }
 
 void left_dense(hipStream_t st, const CscWindow& H, const double* M, int64_t ldm, int64_t P, const double* cw,
@@ -318,12 +573,19 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
   double* Wm = nullptr;
   {
     Timed t(c, "window");
-    // the sparse-map left products accumulate with atomics: no column sort
-    H = csr_window_to_csc(st, rowptr, col, val, m, n, sk.win, /*sorted=*/!sk.sparse_maps);
-    scr.reserve(align_up((size_t)W * sizeof(double), 256) + (size_t)s * n * sizeof(double));
+    scr.reserve(align_up((size_t)W * sizeof(double), 256) + align_up((size_t)s * n * sizeof(double), 256) + 256);
     cw = scr.as<double>();
     Wm = reinterpret_cast<double*>(scr.as<char>() + align_up((size_t)W * sizeof(double), 256));
+    int* flag = reinterpret_cast<int*>(scr.as<char>() + align_up((size_t)W * sizeof(double), 256) +
+                                       align_up((size_t)s * n * sizeof(double), 256));
     SKEMA_HIP(hipMemcpyAsync(cw, hw.data(), (size_t)W * sizeof(double), hipMemcpyHostToDevice, st));
+    bool done = false;
+    if (sk.sparse_maps) {
+      // the fixed-point products take the column entries in any order
+      const int64_t nnz = validate_csr_window(st, rowptr, col, m, n, flag);
+      done = csc_by_buckets(st, rowptr, col, val, m, n, nnz, sk.win, H);
+    }
+    if (!done) H = csr_window_to_csc(st, rowptr, col, val, m, n, sk.win, /*sorted=*/true);
   }
   double* X = sk.X.as<double>();
   double* Y = sk.Y.as<double>();
@@ -331,36 +593,11 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
   if (sk.sparse_maps) {
     {
       Timed t(c, "sketch_left");
-      left_sparse(st, H, *sk.maps[0], k, cw, window, X, k);
-      left_sparse(st, H, *sk.maps[2], s, cw, window, Wm, s);
+      left_sparse_fx(st, H, *sk.maps[0], k, X, k, *sk.maps[2], s, Wm, s, cw, window);
     }
     {
       Timed t(c, "sketch_right");
-      const Map& Om = *sk.maps[1];
-      const Csr& Ot = Om.logical_csr_t();
-      for (int64_t p0 = 0; p0 < k; p0 += 128) {
-        const int64_t pc = std::min<int64_t>(128, k - p0);
-        const dim3 grid(nblocks(m, 64));
-        if (Om.ell_zeta == 8 && pc <= 64)
-          hipLaunchKernelGGL((k_right_all_ell<64, 8>), grid, dim3(256), 0, st, rowptr, col, val, m,
-                             Om.ell.as<uint32_t>(), Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
-        else if (Om.ell_zeta == 8)
-          hipLaunchKernelGGL((k_right_all_ell<128, 8>), grid, dim3(256), 0, st, rowptr, col, val, m,
-                             Om.ell.as<uint32_t>(), Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
-        else if (Om.ell_zeta > 0 && pc <= 64)
-          hipLaunchKernelGGL((k_right_all_ell<64, 0>), grid, dim3(256), 0, st, rowptr, col, val, m,
-                             Om.ell.as<uint32_t>(), Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
-        else if (Om.ell_zeta > 0)
-          hipLaunchKernelGGL((k_right_all_ell<128, 0>), grid, dim3(256), 0, st, rowptr, col, val, m,
-                             Om.ell.as<uint32_t>(), Om.ell_zeta, Om.ell_mag, p0, pc, eta, 0.0, Y, m);
-        else if (pc <= 64)
-          hipLaunchKernelGGL(k_right_all_sparse<64>, grid, dim3(256), 0, st, rowptr, col, val, m, Ot.rp(), Ot.ci(),
-                             Ot.v(), p0, pc, eta, 0.0, Y, m);
-        else
-          hipLaunchKernelGGL(k_right_all_sparse<128>, grid, dim3(256), 0, st, rowptr, col, val, m, Ot.rp(), Ot.ci(),
-                             Ot.v(), p0, pc, eta, 0.0, Y, m);
-        check_launch("k_right_all_sparse");
-      }
+      right_sparse_fx(st, rowptr, col, val, m, *sk.maps[1], k, eta, Y, m);
     }
     {
       Timed t(c, "sketch_core");

@@ -5,6 +5,7 @@
 // for dgesvd (:71-97).
 #include <algorithm>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "linalg.hpp"
@@ -240,27 +241,135 @@ __global__ void k_set_identity(int64_t n, double* V, int64_t ldv) {
   V[i + c * ldv] = (i == c) ? 1.0 : 0.0;
 }
 
+// V (rows x jb, ld rows): the reflectors of a panel whose top-left element is
+// A[0]: unit diagonal, zeros above it, the stored vectors below.
+__global__ void k_build_v(const double* A, int64_t lda, int64_t rows, int64_t jb, double* V) {
+  const int64_t total = rows * jb;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % rows, c = e / rows;
+    V[e] = (i < c) ? 0.0 : (i == c ? 1.0 : A[i + c * lda]);
+  }
+}
+
+// dlarft (forward, columnwise) from G = V^T V: T upper triangular (jb x jb,
+// jb <= 64) with H(0) H(1) ... H(jb-1) = I - V T V^T, i.e. T(i, i) = tau_i and
+// T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).  One 64-thread block.
+__global__ __launch_bounds__(64) void k_build_t(const double* G, int64_t jb, const double* tau, double* T) {
+  const int t = threadIdx.x;
+  for (int64_t i = 0; i < jb; ++i) {
+    if (t < i) {
+      double w = 0.0;
+      for (int64_t l = t; l < i; ++l) w += T[t + l * jb] * G[l + i * jb];
+      T[t + i * jb] = -tau[i] * w;
+    } else if (t == i) {
+      T[i + i * jb] = tau[i];
+    } else if (t < jb) {
+      T[t + i * jb] = 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+// Q (rows x cols) = the first cols columns of the identity.
+__global__ void k_eye_rect(int64_t rows, int64_t cols, double* Q, int64_t ldq) {
+  const int64_t total = rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % rows, c = e / rows;
+    Q[i + c * ldq] = (i == c) ? 1.0 : 0.0;
+  }
+}
+
+constexpr int64_t kQrPanel = 32;
+
+unsigned grid_for_elems(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, 256), 65536)); }
+
+// Workspace of the blocked QR: V (m x nb), G and T (nb x nb), two nb x n.
+struct QrWork {
+  double *V, *G, *T, *W1, *W2;
+};
+QrWork qr_work(DevBuf* work, int64_t m, int64_t n) {
+  const int64_t nb = kQrPanel;
+  const size_t oV = 0, oG = align_up((size_t)m * nb * sizeof(double), 256);
+  const size_t oT = oG + align_up((size_t)nb * nb * sizeof(double), 256);
+  const size_t o1 = oT + align_up((size_t)nb * nb * sizeof(double), 256);
+  const size_t o2 = o1 + align_up((size_t)nb * n * sizeof(double), 256);
+  work->reserve(o2 + (size_t)nb * n * sizeof(double));
+  char* b = work->as<char>();
+  return QrWork{reinterpret_cast<double*>(b + oV), reinterpret_cast<double*>(b + oG),
+                reinterpret_cast<double*>(b + oT), reinterpret_cast<double*>(b + o1),
+                reinterpret_cast<double*>(b + o2)};
+}
+
+// V and T of the panel at (j0, j0) with jb columns (rows j0 .. m-1).
+void panel_vt(hipStream_t st, const double* A, int64_t lda, int64_t m, int64_t j0, int64_t jb, const double* tau,
+              const QrWork& w, GemmWorkspace* gws) {
+  const int64_t mp = m - j0;
+  hipLaunchKernelGGL(k_build_v, dim3(grid_for_elems(mp * jb)), dim3(256), 0, st, A + j0 + j0 * lda, lda, mp, jb, w.V);
+  check_launch("k_build_v");
+  gemm_f64(st, 'T', 'N', jb, jb, mp, 1.0, w.V, mp, w.V, mp, 0.0, w.G, jb, gws);
+  hipLaunchKernelGGL(k_build_t, dim3(1), dim3(64), 0, st, w.G, jb, tau + j0, w.T);
+  check_launch("k_build_t");
+}
+
 }  // namespace
 
-void householder_qr(hipStream_t st, int64_t m, int64_t n, double* A, int64_t lda, double* tau) {
+void householder_qr(hipStream_t st, int64_t m, int64_t n, double* A, int64_t lda, double* tau, DevBuf* work,
+                    GemmWorkspace* gws) {
   SKEMA_REQUIRE(m >= n && n >= 0, kErrShape, "householder_qr: needs rows >= cols");
   SKEMA_REQUIRE(lda >= m, kErrShape, "householder_qr: lda < rows");
   if (n == 0) return;
-  hipLaunchKernelGGL(k_qr_first, dim3(1), dim3(kQrThreads), 0, st, A, m, tau);
-  check_launch("k_qr_first");
-  for (int64_t j = 0; j + 1 < n; ++j) {
-    hipLaunchKernelGGL(k_qr_step, dim3((unsigned)(n - j - 1)), dim3(kQrThreads), 0, st, A, lda, m, n, j, tau);
+  if (!work || n <= kQrPanel) {
+    hipLaunchKernelGGL(k_qr_first, dim3(1), dim3(kQrThreads), 0, st, A, m, tau);
+    check_launch("k_qr_first");
+    for (int64_t j = 0; j + 1 < n; ++j) {
+      hipLaunchKernelGGL(k_qr_step, dim3((unsigned)(n - j - 1)), dim3(kQrThreads), 0, st, A, lda, m, n, j, tau);
+    }
+    check_launch("k_qr_step");
+    return;
   }
-  check_launch("k_qr_step");
+  const QrWork w = qr_work(work, m, n);
+  for (int64_t j0 = 0; j0 < n; j0 += kQrPanel) {
+    const int64_t jb = std::min(kQrPanel, n - j0), mp = m - j0;
+    // the panel, column by column (dgeqr2 on rows j0..m-1 of columns j0..j0+jb-1)
+    hipLaunchKernelGGL(k_qr_first, dim3(1), dim3(kQrThreads), 0, st, A + j0 + j0 * lda, mp, tau + j0);
+    for (int64_t j = j0; j + 1 < j0 + jb; ++j)
+      hipLaunchKernelGGL(k_qr_step, dim3((unsigned)(j0 + jb - j - 1)), dim3(kQrThreads), 0, st, A, lda, m, n, j, tau);
+    check_launch("k_qr_step");
+    if (j0 + jb >= n) break;
+    // trailing columns: A2 <- (I - V T V^T)^T A2 = A2 - V (T^T (V^T A2))  (dlarfb 'L','T','F','C')
+    const int64_t n2 = n - j0 - jb;
+    double* A2 = A + j0 + (j0 + jb) * lda;
+    panel_vt(st, A, lda, m, j0, jb, tau, w, gws);
+    gemm_f64(st, 'T', 'N', jb, n2, mp, 1.0, w.V, mp, A2, lda, 0.0, w.W1, jb, gws);
+    gemm_f64(st, 'T', 'N', jb, n2, jb, 1.0, w.T, jb, w.W1, jb, 0.0, w.W2, jb, gws);
+    gemm_f64(st, 'N', 'N', mp, n2, jb, -1.0, w.V, mp, w.W2, jb, 1.0, A2, lda, gws);
+  }
 }
 
 void householder_q(hipStream_t st, int64_t m, int64_t n, const double* A, int64_t lda, const double* tau,
-                   double* Q, int64_t ldq) {
+                   double* Q, int64_t ldq, DevBuf* work, GemmWorkspace* gws) {
   SKEMA_REQUIRE(m >= n && n >= 0 && lda >= m && ldq >= m, kErrShape, "householder_q: bad shape");
-  for (int64_t j = n - 1; j >= 0; --j) {
-    hipLaunchKernelGGL(k_org_step, dim3((unsigned)(n - j)), dim3(kQrThreads), 0, st, A, lda, tau, Q, ldq, m, j);
+  if (n == 0) return;
+  if (!work || n <= kQrPanel) {
+    for (int64_t j = n - 1; j >= 0; --j) {
+      hipLaunchKernelGGL(k_org_step, dim3((unsigned)(n - j)), dim3(kQrThreads), 0, st, A, lda, tau, Q, ldq, m, j);
+    }
+    check_launch("k_org_step");
+    return;
   }
-  check_launch("k_org_step");
+  const QrWork w = qr_work(work, m, n);
+  hipLaunchKernelGGL(k_eye_rect, dim3(grid_for_elems(m * n)), dim3(256), 0, st, m, n, Q, ldq);
+  check_launch("k_eye_rect");
+  // Q = H(0) ... H(n-1) [I; 0], one panel at a time from the last: panel p
+  // acts on rows >= j0 only, where the columns before j0 are still zero
+  for (int64_t j0 = ((n - 1) / kQrPanel) * kQrPanel; j0 >= 0; j0 -= kQrPanel) {
+    const int64_t jb = std::min(kQrPanel, n - j0), mp = m - j0, nq = n - j0;
+    double* Q2 = Q + j0 + j0 * ldq;
+    panel_vt(st, A, lda, m, j0, jb, tau, w, gws);
+    gemm_f64(st, 'T', 'N', jb, nq, mp, 1.0, w.V, mp, Q2, ldq, 0.0, w.W1, jb, gws);
+    gemm_f64(st, 'N', 'N', jb, nq, jb, 1.0, w.T, jb, w.W1, jb, 0.0, w.W2, jb, gws);
+    gemm_f64(st, 'N', 'N', mp, nq, jb, -1.0, w.V, mp, w.W2, jb, 1.0, Q2, ldq, gws);
+  }
 }
 
 void extract_r(hipStream_t st, int64_t n, const double* A, int64_t lda, double* R, int64_t ldr) {
@@ -323,8 +432,10 @@ int jacobi_svd(hipStream_t st, int64_t n, double* A, int64_t lda, double* U, int
   check_launch("k_set_identity");
 
   const double tol = 1.0e-15;
+  constexpr int kMaxSweeps = 100;
   int sweeps = 0;
-  for (; sweeps < 60; ++sweeps) {
+  bool converged = false;
+  for (; sweeps < kMaxSweeps; ++sweeps) {
     fill_i32(st, d_cnt, 1, 0);
     for (int r = 0; r + 1 < (int)round_start.size(); ++r) {
       const int cnt = round_start[r + 1] - round_start[r];
@@ -336,8 +447,14 @@ int jacobi_svd(hipStream_t st, int64_t n, double* A, int64_t lda, double* U, int
     int h_cnt = 0;
     SKEMA_HIP(hipMemcpyAsync(&h_cnt, d_cnt, sizeof(int), hipMemcpyDeviceToHost, st));
     SKEMA_HIP(hipStreamSynchronize(st));
-    if (h_cnt == 0) break;
+    if (h_cnt == 0) {
+      converged = true;
+      break;
+    }
   }
+  SKEMA_REQUIRE(converged, kErrNumeric,
+                "jacobi_svd: still rotating after " + std::to_string(kMaxSweeps) + " sweeps (order " + std::to_string(n) +
+                    "); the factors would not be orthogonal");
   hipLaunchKernelGGL(k_col_norms, dim3((unsigned)n), dim3(256), 0, st, A, lda, n, Sraw);
   check_launch("k_col_norms");
   std::vector<double> hs(n);

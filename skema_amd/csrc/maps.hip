@@ -134,6 +134,8 @@ Map* make_sparse_map(Ctx* ctx, int64_t nrow, int64_t ncol, int64_t nnz, const in
     const int64_t* p = st ? rp : trp.data();
     const int32_t* c = st ? ci.data() : tci.data();
     const double* val = st ? v.data() : tv.data();
+    for (int64_t r = 0; r < lc; ++r) m->sp_zmax = std::max<int64_t>(m->sp_zmax, p[r + 1] - p[r]);
+    for (int64_t e = 0; e < nnz; ++e) m->sp_vmax = std::max(m->sp_vmax, std::fabs(val[e]));
     const int64_t z = lc > 0 ? p[1] - p[0] : 0;
     bool ok = z > 0 && z <= 16 && nnz == z * lc && val[0] != 0.0;
     const double mag = ok ? std::fabs(val[0]) : 0.0;

@@ -247,6 +247,172 @@ __global__ __launch_bounds__(kThreadsR) void k_rbf_right_bf16(RbfLpArgs g) {
     }
 }
 
+// ---- right, tall tile ------------------------------------------------------------
+// 512 threads own 256 window rows x 208 columns (13 column tiles) of
+// [y | G]; k + s columns take ceil((k+s)/208) column blocks, each forming its
+// own Gram tiles.  Per 32-point step the L2 traffic of a block is the
+// [Omega; Psi] slice (208 x 32) and the points (32 x DP) for 2 x 256 x 32 x
+// (208 + DP) MFMA flops: a quarter of a 64-row tile's bytes per flop, at
+// the price of the Gram formed once per column block.  Wave w owns rows
+// 32w .. 32w+31: it forms their Gram tiles with the 32 points of the step
+// (window-row features in registers, the points' from LDS), exponentiates
+// and writes K (bf16) to LDS, and runs the sketch MFMAs of its 2 x 13 tiles
+// with the [Omega; Psi] slice staged in LDS and shared by all waves.
+// Pipelined one step apart (sketch of t, Gram + exp of t+1, [Omega; Psi] of
+// t+1 and points of t+2 staged, [Omega; Psi] of t+2 and points of t+3
+// loaded), double-buffered LDS, one barrier per step.
+constexpr int kTallRows = 256, kTallCols = 208;
+
+template <int DP>
+__global__ __launch_bounds__(512, 1) void k_rbf_right_bf16_tall(RbfLpArgs g) {
+  constexpr int BM = kTallRows, BN = kTallCols, BK = 32, TNC = BN / 16, PS = DP + 8, KS = BK + 8;
+  constexpr int PCH = BK * DP / 8;  // 16-byte chunks of a step's points
+  constexpr int OCH = BN * BK / 8;  // 16-byte chunks of a step's [Omega; Psi] slice
+  constexpr int OR = (OCH + 511) / 512;
+  static_assert(PCH <= 512, "one points chunk per thread");
+  __shared__ __attribute__((aligned(16))) __bf16 Xs[2][BK][PS];
+  __shared__ __attribute__((aligned(16))) __bf16 Ks[2][BM][KS];
+  __shared__ __attribute__((aligned(16))) __bf16 Os[2][BN][KS];
+  __shared__ float NJ[4][BK];
+  __shared__ float NI[BM];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
+  const int split = t / g.tiles_m;
+  const int64_t c0 = (int64_t)blockIdx.y * BN;
+  const int64_t kbeg = (int64_t)split * g.k_per_split;
+  const int64_t kend = (g.K < kbeg + g.k_per_split) ? g.K : kbeg + g.k_per_split;
+  const int64_t steps = (kend - kbeg + BK - 1) / BK;
+
+  // Gram A operands: rows m0 + 32 wave + 16 x + l16, features 32 c + 8 lq ..
+  bf16x8 xa[2][DP / 32];
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    const int64_t i = m0 + 32 * wave + 16 * x + l16;
+#pragma unroll
+    for (int c = 0; c < DP / 32; ++c)
+      xa[x][c] = (i < g.M) ? load8(g.P + (g.row0 + i) * g.ldpb, 32 * c + 8 * lq, g.ldpb) : bf16x8{};
+  }
+  if (tid < BM) NI[tid] = (m0 + tid < g.M) ? g.nrm[g.row0 + m0 + tid] : 0.f;
+
+  bf16x8 xr = {};
+  float njr = 0.f;
+  auto load_points = [&](int64_t u) {
+    const int64_t j0 = kbeg + u * BK;
+    if (tid < PCH) {
+      const int jj = tid / (DP / 8), fc = tid % (DP / 8);
+      const int64_t j = j0 + jj;
+      xr = (j < kend) ? load8(g.P + j * g.ldpb, 8 * fc, g.ldpb) : bf16x8{};
+    }
+    njr = (tid < BK && j0 + tid < kend) ? g.nrm[j0 + tid] : 0.f;
+  };
+  auto store_points = [&](int b, int64_t u) {
+    if (tid < PCH) *reinterpret_cast<bf16x8*>(&Xs[b][tid / (DP / 8)][8 * (tid % (DP / 8))]) = xr;
+    if (tid < BK) NJ[u & 3][tid] = njr;
+  };
+  bf16x8 orr[OR];
+  auto load_op = [&](int64_t u) {
+    const int64_t j0 = kbeg + u * BK;
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+      const int ch = tid + 512 * q;
+      const int cc = ch / (BK / 8), pc = ch % (BK / 8);
+      const int64_t col = c0 + cc, j = j0 + 8 * pc;
+      orr[q] = (ch < OCH && col < g.N && j < kend) ? load8(g.OP + col * g.ldop, j, kend) : bf16x8{};
+    }
+  };
+  auto store_op = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < OR; ++q) {
+      const int ch = tid + 512 * q;
+      if (ch < OCH) *reinterpret_cast<bf16x8*>(&Os[b][ch / (BK / 8)][8 * (ch % (BK / 8))]) = orr[q];
+    }
+  };
+  auto gram_exp = [&](int b, int64_t u) {  // Xs[b] (step u) -> Ks[b]
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt) {
+      f4v gr[2] = {f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+      for (int c = 0; c < DP / 32; ++c) {
+        const bf16x8 bx = *reinterpret_cast<const bf16x8*>(&Xs[b][16 * pt + l16][32 * c + 8 * lq]);
+#pragma unroll
+        for (int x = 0; x < 2; ++x) gr[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa[x][c], bx, gr[x], 0, 0, 0);
+      }
+      const int jl = 16 * pt + l16;
+      const int64_t j = kbeg + u * BK + jl;
+      const float nj = NJ[u & 3][jl];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int il = 32 * wave + 16 * x + 4 * lq + v;
+          const int64_t i = m0 + il;
+          float kv = 0.f;
+          if (i < g.M && j < kend) kv = rbf_value_f(NI[il], nj, gr[x][v], g.gamma, g.row0 + i == j);
+          Ks[b][il][jl] = (__bf16)kv;
+        }
+    }
+  };
+
+  f4v acc[2][TNC];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < TNC; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  load_points(0); store_points(0, 0);
+  load_points(1); store_points(1, 1);
+  load_op(0); store_op(0);
+  __syncthreads();
+  gram_exp(0, 0);
+  load_points(2);
+  load_op(1);
+  __syncthreads();
+  for (int64_t u = 0; u < steps; ++u) {
+    const int b = (int)(u & 1), nb = b ^ 1;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&Ks[b][32 * wave + 16 * x + l16][8 * lq]);
+#pragma unroll
+      for (int y = 0; y < TNC; ++y) {
+        const bf16x8 o = *reinterpret_cast<const bf16x8*>(&Os[b][16 * y + l16][8 * lq]);
+        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, o, acc[x][y], 0, 0, 0);
+      }
+    }
+    if (u + 1 < steps) gram_exp(nb, u + 1);  // points of step u+1 sit in Xs[nb]
+    store_op(nb);                            // [Omega; Psi] of step u+1
+    store_points(b, u + 2);                  // points of step u+2 -> Xs[b]
+    load_op(u + 2);
+    load_points(u + 3);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < TNC; ++y) {
+      const int64_t i0 = m0 + 32 * wave + 16 * x + 4 * lq;
+      const int64_t j = c0 + 16 * y + l16;
+      if (i0 >= g.M || j >= g.N) continue;
+      if (g.part) {
+        float* p = g.part + (int64_t)split * g.M * g.N + j * g.M;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (i0 + v < g.M) p[i0 + v] = acc[x][y][v];
+      } else if (i0 + 4 <= g.M && j >= g.out.nsplit) {
+        __bf16* pg = g.G + i0 + (j - g.out.nsplit) * g.ldg;
+        const bf16x4 h = bf16x4{(__bf16)acc[x][y][0], (__bf16)acc[x][y][1], (__bf16)acc[x][y][2],
+                                (__bf16)acc[x][y][3]};
+        if ((((uintptr_t)pg) & 7) == 0) *reinterpret_cast<bf16x4*>(pg) = h;
+        else for (int v = 0; v < 4; ++v) pg[v] = h[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (i0 + v < g.M) store_right(g, i0 + v, j, (double)acc[x][y][v]);
+      }
+    }
+}
+
 // Sum of the FP32 partials of the point splits (fixed order), then the
 // epilogue; consecutive threads own consecutive rows, so the stores coalesce.
 __global__ void k_rbf_right_bf16_reduce(RbfLpArgs g, int splits) {
@@ -426,9 +592,12 @@ void rbf_right_bf16(Ctx* c, const __bf16* P, int dp, const float* nrm, float gam
   a.OP = OP; a.ldop = ldop;
   a.out = yout; a.out.nsplit = k;
   a.G = G; a.ldg = ldg;
+  // tall tiles (256 rows x 208 columns) once the window fills them;
+  // otherwise 64 rows x all columns
+  const bool tall = a.M >= 128;
   const int tn = pick(kTnChoices, cdiv(cdiv(a.N, 16), 8));
-  const int64_t col_blocks = cdiv(a.N, 128 * tn);
-  a.tiles_m = (int)cdiv(a.M, 64);
+  const int64_t col_blocks = tall ? cdiv(a.N, kTallCols) : cdiv(a.N, 128 * tn);
+  a.tiles_m = (int)cdiv(a.M, tall ? kTallRows : 64);
   // one 512-thread block per CU: choose the split of the points that
   // minimises rounds x points per split plus the partial traffic
   const int64_t per_split = a.tiles_m * col_blocks;
@@ -446,7 +615,11 @@ void rbf_right_bf16(Ctx* c, const __bf16* P, int dp, const float* nrm, float gam
     a.part = c->gws.buf.as<float>();
   }
   const dim3 grid((unsigned)(a.tiles_m * splits), (unsigned)col_blocks);
-  if (dp == 32) launch_right<32>(st, a, grid, tn);
+  if (tall) {
+    if (dp == 32) hipLaunchKernelGGL(k_rbf_right_bf16_tall<32>, grid, dim3(512), 0, st, a);
+    else if (dp == 64) hipLaunchKernelGGL(k_rbf_right_bf16_tall<64>, grid, dim3(512), 0, st, a);
+    else hipLaunchKernelGGL(k_rbf_right_bf16_tall<128>, grid, dim3(512), 0, st, a);
+  } else if (dp == 32) launch_right<32>(st, a, grid, tn);
   else if (dp == 64) launch_right<64>(st, a, grid, tn);
   else launch_right<128>(st, a, grid, tn);
   check_launch("k_rbf_right_bf16");

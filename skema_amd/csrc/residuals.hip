@@ -10,6 +10,7 @@
 // so A^T U = K U comes out of the same pass as K V, and K is never
 // written).  The norms are one block per column with a fixed-order
 // reduction, so repeated calls give identical bits.
+#include <algorithm>
 #include <vector>
 
 #include "gemm.hpp"
@@ -85,6 +86,23 @@ void rbf_products(Ctx* c, const double* X, int64_t n, int64_t d, int64_t ldp, do
 }
 
 }  // namespace
+
+void residual_fro_stream(hipStream_t st, int64_t m, int64_t n, int64_t r, const double* A, int64_t lda,
+                         const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv, int64_t w,
+                         double* R, double* US, double* part, double* acc) {
+  SKEMA_REQUIRE(w > 0, kErrShape, "residual: window must be positive");
+  for (int64_t r0 = 0; r0 < m; r0 += w) {
+    const int64_t rows = std::min(w, m - r0);
+    copy_matrix(st, rows, n, A + r0, lda, R, rows);
+    sum_squares_acc(st, rows, n, R, rows, part, acc + 1);
+    if (r > 0) {
+      scale_columns(st, rows, r, U + r0, ldu, S, US, rows);
+      // no workspace: the product is never split along K (K = r)
+      gemm_f64(st, 'N', 'T', rows, n, r, -1.0, US, rows, V, ldv, 1.0, R, rows, nullptr);
+    }
+    sum_squares_acc(st, rows, n, R, rows, part, acc);
+  }
+}
 
 void residuals_dense(Ctx* c, int64_t m, int64_t n, int64_t r, const double* A, int64_t lda, const double* U,
                      int64_t ldu, const double* S, const double* V, int64_t ldv, double* rnorms) {

@@ -34,6 +34,7 @@ struct Ctx {
   std::string profile_only;  // when set, only timing scopes of this name record events
   GemmWorkspace gws;
   DevBuf scratch;  // general grow-only scratch
+  DevBuf qr;       // blocked QR workspace (panel reflectors, T, products)
   KernelStats stats;
   void sync_if_needed() {
     if (!async_calls) SKEMA_HIP(hipStreamSynchronize(stream));
@@ -97,6 +98,10 @@ struct Map {
   DevBuf ell;
   int ell_zeta = 0;
   double ell_mag = 0.0;
+  // Sparse maps: the largest |entry| and the most entries of one logical
+  // column (bounds for exact fixed-point accumulation).
+  double sp_vmax = 0.0;
+  int64_t sp_zmax = 0;
 };
 
 struct Sketch {

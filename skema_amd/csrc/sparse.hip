@@ -413,27 +413,34 @@ void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, co
   check_launch("k_spmm_dense_csrT");
 }
 
-CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val,
-                            int64_t rows, int64_t cols, DevBuf& scratch, bool sorted) {
+int64_t validate_csr_window(hipStream_t st, const int64_t* rowptr, const int32_t* col, int64_t rows, int64_t cols,
+                            int* flag) {
   int64_t ends[2] = {0, 0};
   SKEMA_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SKEMA_HIP(hipMemcpyAsync(&ends[1], rowptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   SKEMA_HIP(hipStreamSynchronize(st));
   const int64_t nnz = ends[1] - ends[0];
   SKEMA_REQUIRE(nnz >= 0 && nnz < ((int64_t)1 << 31), kErrArg, "CSR window: bad rowptr range");
+  // Reject malformed windows before any kernel indexes with them.
+  fill_i32(st, flag, 1, 0);
+  const int64_t work = nnz > rows ? nnz : rows;
+  const unsigned nb = blocks_for(work) > 65536 ? 65536 : blocks_for(work);
+  hipLaunchKernelGGL(k_validate_csr, dim3(nb), dim3(256), 0, st, rowptr, rows, col, cols, flag);
+  check_launch("k_validate_csr");
+  int h = 0;
+  SKEMA_HIP(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  SKEMA_HIP(hipStreamSynchronize(st));
+  SKEMA_REQUIRE(!(h & 1), kErrArg, "CSR window: rowptr is not non-decreasing");
+  SKEMA_REQUIRE(!(h & 2), kErrArg, "CSR window: column index outside [0, n)");
+  return nnz;
+}
+
+CscWindow csr_window_to_csc(hipStream_t st, const int64_t* rowptr, const int32_t* col, const double* val,
+                            int64_t rows, int64_t cols, DevBuf& scratch, bool sorted) {
+  int64_t nnz = 0;
   {
-    // Reject malformed windows before any kernel indexes with them.
     DevBuf flag(sizeof(int));
-    fill_i32(st, flag.as<int>(), 1, 0);
-    const int64_t work = nnz > rows ? nnz : rows;
-    const unsigned nb = blocks_for(work) > 65536 ? 65536 : blocks_for(work);
-    hipLaunchKernelGGL(k_validate_csr, dim3(nb), dim3(256), 0, st, rowptr, rows, col, cols, flag.as<int>());
-    check_launch("k_validate_csr");
-    int h = 0;
-    SKEMA_HIP(hipMemcpyAsync(&h, flag.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
-    SKEMA_HIP(hipStreamSynchronize(st));
-    SKEMA_REQUIRE(!(h & 1), kErrArg, "CSR window: rowptr is not non-decreasing");
-    SKEMA_REQUIRE(!(h & 2), kErrArg, "CSR window: column index outside [0, n)");
+    nnz = validate_csr_window(st, rowptr, col, rows, cols, flag.as<int>());
   }
   const size_t o_cnt = 0;
   const size_t o_ptr = align_up((size_t)cols * sizeof(int) * 2, 256);

@@ -19,6 +19,12 @@ void spmm_csr_rows(hipStream_t st, const Csr& S, int64_t r0, int64_t r1, int64_t
 void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, const Csr& S, const GemmOut& out,
                      DevBuf* ws = nullptr);
 
+// Checks a device CSR window (rowptr non-decreasing, column indices in
+// [0, cols)) and returns its nonzero count; flag is one device int of
+// scratch.  Throws on a malformed window before any kernel indexes with it.
+int64_t validate_csr_window(hipStream_t st, const int64_t* rowptr, const int32_t* col, int64_t rows, int64_t cols,
+                            int* flag);
+
 // Column-compressed copy of a CSR window (rowptr may start at an offset).
 struct CscWindow {
   int64_t rows = 0, cols = 0, nnz = 0;

@@ -184,6 +184,17 @@ void spd_low_rank_approx(Spd& sp, int shift_removal, double* U, int64_t ldu, dou
       Ri((size_t)k * k * sizeof(double)), E((size_t)n * k * sizeof(double)), Q((size_t)n * k * sizeof(double)),
       tau((size_t)k * sizeof(double)), Re((size_t)k * k * sizeof(double)), Ur((size_t)k * k * sizeof(double)),
       Vr((size_t)k * k * sizeof(double)), Se((size_t)k * sizeof(double)), info(sizeof(int)), scratch;
+  // phase times under the reference's keys (src/Skema_SketchySVD.cpp:776-786)
+  auto& tm = sp.timings["approx"];
+  for (const char* key : {"omega", "daxpy", "norm2", "update", "dpotrf", "dgels", "dgesvd"}) tm.emplace(key, 0.0);
+  SKEMA_HIP(hipStreamSynchronize(st));
+  double t0 = now_seconds();
+  auto lap = [&](const char* key) {
+    SKEMA_HIP(hipStreamSynchronize(st));
+    const double t = now_seconds();
+    tm[key] += t - t0;
+    t0 = t;
+  };
   double nrm = 0.0;
   {
     Timed t(c, "spd_norm2");
@@ -194,6 +205,7 @@ void spd_low_rank_approx(Spd& sp, int shift_removal, double* U, int64_t ldu, dou
     SKEMA_HIP(hipMemcpyAsync(&top, Sg.p, sizeof(double), hipMemcpyDeviceToHost, st));
     SKEMA_HIP(hipStreamSynchronize(st));
     nrm = std::sqrt(std::max(top, 0.0));
+    lap("norm2");
   }
   const double shift = std::numeric_limits<double>::epsilon() * nrm;
   {
@@ -201,9 +213,12 @@ void spd_low_rank_approx(Spd& sp, int shift_removal, double* U, int64_t ldu, dou
     // Y_shift = Y + shift Omega; B = Omega^T Y_shift; C = (B + B^T) / 2
     hipLaunchKernelGGL(k_axpy_mat, dim3(g256(n * k)), dim3(256), 0, st, n * k, Y, shift, sp.omL, Yv.as<double>());
     check_launch("k_axpy_mat");
+    lap("daxpy");
     gemm_f64(st, 'T', 'N', k, k, n, 1.0, sp.omL, n, Yv.as<double>(), n, 0.0, G.as<double>(), k, &c->gws);
+    lap("omega");
     hipLaunchKernelGGL(k_sym_half, dim3(g256(k * k)), dim3(256), 0, st, k, G.as<double>(), Cm.as<double>());
     check_launch("k_sym_half");
+    lap("update");
     hipLaunchKernelGGL(k_chol_upper, dim3(1), dim3(1024), 0, st, k, Cm.as<double>(), info.as<int>());
     check_launch("k_chol_upper");
     int h = 0;
@@ -212,23 +227,26 @@ void spd_low_rank_approx(Spd& sp, int shift_removal, double* U, int64_t ldu, dou
     SKEMA_REQUIRE(h == 0, kErrNumeric,
                   "spd low_rank_approx: (B + B^T)/2 is not positive definite (leading minor " + std::to_string(h) +
                       ")");
+    lap("dpotrf");
     // E = Y R^{-1}: R^{-1} by back-substitution on the identity, then one GEMM
     hipLaunchKernelGGL(k_identity, dim3(g256(k * k)), dim3(256), 0, st, k, Ri.as<double>());
     check_launch("k_identity");
     trsm_upper(st, k, k, Cm.as<double>(), k, Ri.as<double>(), k);
     gemm_f64(st, 'N', 'N', n, k, k, 1.0, Y, n, Ri.as<double>(), k, 0.0, E.as<double>(), n, &c->gws);
+    lap("dgels");
   }
   {
     Timed t(c, "spd_svd");
     // thin SVD of E: Householder QR, Jacobi SVD of R, U = Q U_R
-    householder_qr(st, n, k, E.as<double>(), n, tau.as<double>());
+    householder_qr(st, n, k, E.as<double>(), n, tau.as<double>(), &c->qr, &c->gws);
     extract_r(st, k, E.as<double>(), n, Re.as<double>(), k);
-    householder_q(st, n, k, E.as<double>(), n, tau.as<double>(), Q.as<double>(), n);
+    householder_q(st, n, k, E.as<double>(), n, tau.as<double>(), Q.as<double>(), n, &c->qr, &c->gws);
     jacobi_svd(st, k, Re.as<double>(), k, Ur.as<double>(), k, Se.as<double>(), Vr.as<double>(), k, &scratch);
     gemm_f64(st, 'N', 'N', n, r, k, 1.0, Q.as<double>(), n, Ur.as<double>(), k, 0.0, U, ldu, &c->gws);
     hipLaunchKernelGGL(k_remove_shift, dim3(g256(r)), dim3(256), 0, st, r, Se.as<double>(),
                        shift_removal == 1 ? nrm : shift, S);
     check_launch("k_remove_shift");
+    lap("dgesvd");
   }
 }
 

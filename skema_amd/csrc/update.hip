@@ -303,12 +303,12 @@ void finalize_sketch(Sketch& sk) {
 
   // [P,~] = qr(X^T,0)  (:355-357)
   transpose(st, k, n, sk.X.as<double>(), k, W, n);
-  householder_qr(st, n, k, W, n, tau);
-  householder_q(st, n, k, W, n, tau, P, n);
+  householder_qr(st, n, k, W, n, tau, &c->qr, &c->gws);
+  householder_q(st, n, k, W, n, tau, P, n, &c->qr, &c->gws);
   // [Q,~] = qr(Y,0)  (:381-383), on a copy: Y stays intact.
   copy_matrix(st, m, k, sk.Y.as<double>(), m, W, m);
-  householder_qr(st, m, k, W, m, tau);
-  householder_q(st, m, k, W, m, tau, Q, m);
+  householder_qr(st, m, k, W, m, tau, &c->qr, &c->gws);
+  householder_q(st, m, k, W, m, tau, Q, m, &c->qr, &c->gws);
   tm["dgeqrf"] += sw.lap();
 
   // U1 = Phi*Q, U2 = Psi*P  (:409, :430)
@@ -333,13 +333,13 @@ void finalize_sketch(Sketch& sk) {
   double* Z2t = Z2 + k * k; // k x k
   double* Cw = Z2t + k * k; // k x k (Jacobi works in place)
   copy_matrix(st, s, k, PhiQ, s, F, s);
-  householder_qr(st, s, k, F, s, tau);
+  householder_qr(st, s, k, F, s, tau, &c->qr, &c->gws);
   extract_r(st, k, F, s, T1, k);
-  householder_q(st, s, k, F, s, tau, U1, s);
+  householder_q(st, s, k, F, s, tau, U1, s, &c->qr, &c->gws);
   copy_matrix(st, s, k, PsiP, s, F, s);
-  householder_qr(st, s, k, F, s, tau);
+  householder_qr(st, s, k, F, s, tau, &c->qr, &c->gws);
   extract_r(st, k, F, s, T2, k);
-  householder_q(st, s, k, F, s, tau, U2, s);
+  householder_q(st, s, k, F, s, tau, U2, s, &c->qr, &c->gws);
   tm["dgeqrf"] += sw.lap();
 
   // Z2 = (U1^T * Z) * U2  (:500, :515)
@@ -354,9 +354,25 @@ void finalize_sketch(Sketch& sk) {
   transpose(st, k, k, Z2t, k, C, k);
   tm["dgels"] += sw.lap();
 
-  // [Uc, S, Vc] = svd(C)  (:609)
-  copy_matrix(st, k, k, C, k, Cw, k);
-  jacobi_svd(st, k, Cw, k, Uc, k, Sv, Vc, k, &c->scratch);
+  // [Uc, S, Vc] = svd(C)  (:609).  C = Qc R by Householder QR, then the
+  // one-sided Jacobi SVD of R^T = Ur S Vr^T, so C = (Qc Vr) S Ur^T: the QR
+  // leaves R graded (row norms decaying), which cuts the Jacobi sweeps about
+  // threefold on geometrically decaying spectra.  F, U1, U2 and Z1 (each at
+  // least k x k) are free by now.
+  {
+    double* Qc = F;
+    double* Rt = U1;
+    double* Ur = U2;
+    double* Vr = Z1;
+    copy_matrix(st, k, k, C, k, Cw, k);
+    householder_qr(st, k, k, Cw, k, tau, &c->qr, &c->gws);
+    extract_r(st, k, Cw, k, Z2, k);
+    transpose(st, k, k, Z2, k, Rt, k);
+    householder_q(st, k, k, Cw, k, tau, Qc, k, &c->qr, &c->gws);
+    jacobi_svd(st, k, Rt, k, Ur, k, Sv, Vr, k, &c->scratch);
+    gemm_f64(st, 'N', 'N', k, k, k, 1.0, Qc, k, Vr, k, 0.0, Uc, k, &c->gws);
+    copy_matrix(st, k, k, Ur, k, Vc, k);
+  }
   tm["dgesvd"] += sw.lap();
   sk.finalized = true;
 }

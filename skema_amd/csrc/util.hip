@@ -54,6 +54,13 @@ __global__ void k_sum_serial(const double* part, int n, double* out) {
     *out = s;
   }
 }
+__global__ void k_sum_serial_acc(const double* part, int n, double* acc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    *acc += s;
+  }
+}
 
 unsigned grid_for(int64_t n) {
   int64_t b = cdiv(n, 256);
@@ -101,6 +108,13 @@ void sum_squares(hipStream_t st, int64_t rows, int64_t cols, const double* A, in
   hipLaunchKernelGGL(k_sumsq_partial, dim3(nb), dim3(256), 0, st, rows, cols, A, lda, scratch->as<double>());
   hipLaunchKernelGGL(k_sum_serial, dim3(1), dim3(64), 0, st, scratch->as<double>(), nb, out);
   check_launch("sum_squares");
+}
+void sum_squares_acc(hipStream_t st, int64_t rows, int64_t cols, const double* A, int64_t lda, double* part,
+                     double* acc) {
+  if (rows <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL(k_sumsq_partial, dim3(kSumSqPartials), dim3(256), 0, st, rows, cols, A, lda, part);
+  hipLaunchKernelGGL(k_sum_serial_acc, dim3(1), dim3(64), 0, st, part, kSumSqPartials, acc);
+  check_launch("sum_squares_acc");
 }
 
 }  // namespace skema

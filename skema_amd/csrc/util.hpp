@@ -25,4 +25,11 @@ void scale_columns(hipStream_t st, int64_t rows, int64_t cols, const double* A, 
 void sum_squares(hipStream_t st, int64_t rows, int64_t cols, const double* A, int64_t lda, double* out,
                  DevBuf* scratch);
 
+// Same reduction with caller memory: `part` holds kSumSqPartials doubles
+// (overwritten), and the total is ADDED to *acc (device scalar), so a
+// sequence of calls accumulates in call order on the device.
+constexpr int kSumSqPartials = 1024;
+void sum_squares_acc(hipStream_t st, int64_t rows, int64_t cols, const double* A, int64_t lda, double* part,
+                     double* acc);
+
 }  // namespace skema

@@ -7,6 +7,15 @@
 //                [--eta e] [--nu v] [--kernel gaussrbf --gamma g]
 //                [--precision fp64|fp32|bf16|bf16x3|fp8] [--history-file h.json]
 //                [--symmetric | --asymmetric] [--force-three-sketch]
+//                [--normalize-matrix x] [--sketch-compute-svals-iters]
+//                [--debug-file f] [--debug] [--print-level p] [--map-dist uniform|normal]
+//
+// Flags are the reference's (src/Skema_AlgParams.cpp:242-330, driver/main.cpp:
+// 149-224), parsed against an explicit table: a flag that takes no value
+// never swallows the next token, the iSVD / PRIMME flags are accepted and
+// reported as ignored, and an unknown flag or one this build does not
+// provide (--rr, --solver other than sketch) is an error.  Like the
+// reference, a flag may be written without its leading "--".
 //
 // --symmetric (and kernel input unless --force-three-sketch) takes the
 // SketchySPD route (src/Skema_SketchySVD.cpp:1243-1277): eigenvalues as
@@ -134,55 +143,150 @@ std::vector<uint64_t> parse_seeds(std::string s) {
   return v;
 }
 
+void write_history(const std::string& history, const std::string& js) {
+  if (history.empty()) {
+    std::cout << js << std::endl;
+  } else {
+    std::ofstream o(history);
+    o << js << std::endl;
+  }
+}
+
+void json_array(std::ostringstream& js, const char* key, const std::vector<double>& v) {
+  js << ",\"" << key << "\":[";
+  for (size_t i = 0; i < v.size(); ++i) js << (i ? "," : "") << v[i];
+  js << "]";
+}
+
+// The final factors under --debug-file (src/Skema_SketchySVD.cpp:1387-1404).
+void dump_factors(const skema_mi355x::AlgParams& p, const skema_mi355x::DeviceMatrix* U, const std::vector<double>& S,
+                  const skema_mi355x::DeviceMatrix* V) {
+  using namespace skema_mi355x::detail;
+  if (p.debug_filename.empty()) return;
+  if (U) write_matrix(debug_name(p.debug_filename, "U.txt"), U->to_host().data(), U->extent(0), U->extent(1));
+  write_vector(debug_name(p.debug_filename, "S.txt"), S);
+  if (V) write_matrix(debug_name(p.debug_filename, "V.txt"), V->to_host().data(), V->extent(0), V->extent(1));
+}
+
 template <class M, class D>
 int run(const skema_mi355x::AlgParams& p, const M& A, const std::string& history, const skema_mi355x::Context& ctx) {
   skema_mi355x::SketchySVD<M, D> sk(p, ctx);
   sk.linear_update(A);
   auto usv = sk.low_rank_approx();
   const std::vector<double>& S = std::get<1>(usv);
-  // compute_residuals, then the history's svals and rnrms (src/Skema_SketchySVD.cpp:729-755)
+  // compute_residuals, then the history {timings, traces, svals, rnrms}
+  // (src/Skema_SketchySVD.cpp:729-755)
   const std::vector<double>& R = sk.compute_residuals(A, std::get<0>(usv), S, std::get<2>(usv));
+  dump_factors(p, &std::get<0>(usv), S, &std::get<2>(usv));
   std::ostringstream js;
   js.precision(17);
-  std::string t = sk.timings_json();
-  js << "{" << t.substr(1, t.size() - 2) << ",\"svals\":[";
-  for (size_t i = 0; i < S.size(); ++i) js << (i ? "," : "") << S[i];
-  js << "],\"rnrms\":[";
-  for (size_t i = 0; i < R.size(); ++i) js << (i ? "," : "") << R[i];
-  js << "]}";
-  if (history.empty()) {
-    std::cout << js.str() << std::endl;
-  } else {
-    std::ofstream o(history);
-    o << js.str() << std::endl;
-  }
+  const std::string t = sk.timings_json();
+  js << "{" << t.substr(1, t.size() - 2) << ",\"traces\":" << sk.traces_json();
+  json_array(js, "svals", S);
+  json_array(js, "rnrms", R);
+  js << "}";
+  write_history(history, js.str());
   return 0;
 }
 
 // The SketchySPD route (sketchyspd, src/Skema_SketchySVD.cpp:1243-1277):
-// history with the eigenvalues as svals and the eigenpair residuals.
+// history with its timings, the eigenvalues as svals and the eigenpair
+// residuals.
 template <class M, class D>
 int run_spd(const skema_mi355x::AlgParams& p, const M& A, const std::string& history,
             const skema_mi355x::Context& ctx) {
   skema_mi355x::SketchySPD<M, D> sk(p, ctx);
   sk.linear_update(A);
+  if (p.print_level > 0) std::cout << "\nComputing fixed-rank PSD approximation" << std::endl;
   auto us = sk.low_rank_approx();
   const std::vector<double>& S = std::get<1>(us);
   const std::vector<double>& R = sk.compute_residuals(A, std::get<0>(us), S);
+  dump_factors(p, &std::get<0>(us), S, nullptr);
   std::ostringstream js;
   js.precision(17);
-  js << "{\"svals\":[";
-  for (size_t i = 0; i < S.size(); ++i) js << (i ? "," : "") << S[i];
-  js << "],\"rnrms\":[";
-  for (size_t i = 0; i < R.size(); ++i) js << (i ? "," : "") << R[i];
-  js << "]}";
-  if (history.empty()) {
-    std::cout << js.str() << std::endl;
-  } else {
-    std::ofstream o(history);
-    o << js.str() << std::endl;
-  }
+  const std::string t = sk.timings_json();
+  js << "{" << t.substr(1, t.size() - 2) << ",\"traces\":{}";
+  json_array(js, "svals", S);
+  json_array(js, "rnrms", R);
+  js << "}";
+  write_history(history, js.str());
   return 0;
+}
+
+enum class Arg { Value, Flag };
+struct ArgSpec {
+  Arg kind;
+  bool used;  // read by the sketch route; otherwise accepted and ignored
+};
+
+// Every flag of AlgParams::parse (src/Skema_AlgParams.cpp:242-330) and of
+// main (driver/main.cpp:156-170), plus this build's --precision / --map-dist.
+const std::map<std::string, ArgSpec>& arg_table() {
+  static const std::map<std::string, ArgSpec> t = {
+      {"--input", {Arg::Value, true}}, {"--output", {Arg::Value, false}},
+      {"--debug-file", {Arg::Value, true}}, {"--history-file", {Arg::Value, true}},
+      {"--sparse", {Arg::Flag, true}}, {"--dense", {Arg::Flag, true}},
+      {"--symmetric", {Arg::Flag, true}}, {"--asymmetric", {Arg::Flag, true}},
+      {"--m", {Arg::Value, false}}, {"--n", {Arg::Value, false}},
+      {"--rank", {Arg::Value, true}}, {"--num-passes", {Arg::Value, false}},
+      {"--solver", {Arg::Value, true}}, {"--print-level", {Arg::Value, true}},
+      {"--debug", {Arg::Flag, true}}, {"--debug-off", {Arg::Flag, true}},
+      {"--rr", {Arg::Flag, true}}, {"--rr-off", {Arg::Flag, true}},
+      {"--normalize-matrix", {Arg::Value, true}}, {"--window", {Arg::Value, true}},
+      {"--model", {Arg::Value, true}}, {"--range", {Arg::Value, true}}, {"--core", {Arg::Value, true}},
+      {"--seeds", {Arg::Value, true}}, {"--eta", {Arg::Value, true}}, {"--nu", {Arg::Value, true}},
+      {"--force-three-sketch", {Arg::Flag, true}}, {"--force-three-sketch-off", {Arg::Flag, true}},
+      {"--sketch-compute-svals-iters", {Arg::Flag, true}}, {"--sketch-compute-svals-iters-off", {Arg::Flag, true}},
+      {"--svd", {Arg::Flag, false}}, {"--svds", {Arg::Flag, false}},
+      {"--isvd-compute-residual-iters", {Arg::Flag, false}}, {"--isvd-compute-residual-iters-off", {Arg::Flag, false}},
+      {"--isvd-sampler", {Arg::Value, false}}, {"--isvd-num-samples", {Arg::Value, false}},
+      {"--isvd-convtest-eps", {Arg::Value, false}}, {"--isvd-convtest-skip", {Arg::Value, false}},
+      {"--isvd-initial-guess", {Arg::Flag, false}}, {"--isvd-initial-guess-off", {Arg::Flag, false}},
+      {"--isvd-rank-add-factor", {Arg::Value, false}},
+      {"--primme_outputFile", {Arg::Value, false}}, {"--primme_printLevel", {Arg::Value, false}},
+      {"--primme_eps", {Arg::Value, false}}, {"--primme_initSize", {Arg::Value, false}},
+      {"--primme_maxBasisSize", {Arg::Value, false}}, {"--primme_minRestartSize", {Arg::Value, false}},
+      {"--primme_maxBlockSize", {Arg::Value, false}}, {"--primme_maxMatvecs", {Arg::Value, false}},
+      {"--primme_maxIters", {Arg::Value, false}}, {"--primme_locking", {Arg::Flag, false}},
+      {"--primme_locking-off", {Arg::Flag, false}}, {"--primme_method", {Arg::Value, false}},
+      {"--prime_methodStage2", {Arg::Value, false}},
+      {"--kernel", {Arg::Value, true}}, {"--gamma", {Arg::Value, true}},
+      {"--help", {Arg::Flag, true}}, {"--no-help", {Arg::Flag, true}},
+      {"--precision", {Arg::Value, true}}, {"--map-dist", {Arg::Value, true}},
+  };
+  return t;
+}
+
+void usage() {
+  std::cout << "Usage: skema_sketch --input A.mtx|A.bin [options]\n"
+               "  --sparse | --dense, --symmetric | --asymmetric, --rank r, --range k, --core s,\n"
+               "  --window w, --model gauss|sparse-sign, --seeds [a,b,c,d], --eta e, --nu v,\n"
+               "  --kernel gaussrbf, --gamma g, --force-three-sketch, --normalize-matrix x,\n"
+               "  --sketch-compute-svals-iters, --history-file f, --debug-file f, --debug,\n"
+               "  --print-level p, --precision fp64|fp32|bf16x3|bf16|fp8, --map-dist uniform|normal\n";
+}
+
+int64_t to_int(const std::string& flag, const std::string& v, int64_t lo) {
+  size_t pos = 0;
+  int64_t x = 0;
+  try {
+    x = std::stoll(v, &pos);
+  } catch (const std::exception&) {
+    fail("Bad input: " + flag + " " + v + ", must be an integer");
+  }
+  if (pos != v.size() || x < lo) fail("Bad input: " + flag + " " + v + ", must be an integer >= " + std::to_string(lo));
+  return x;
+}
+double to_real(const std::string& flag, const std::string& v) {
+  size_t pos = 0;
+  double x = 0.0;
+  try {
+    x = std::stod(v, &pos);
+  } catch (const std::exception&) {
+    fail("Bad input: " + flag + " " + v + ", must be a number");
+  }
+  if (pos != v.size() || !(x >= 0.0)) fail("Bad input: " + flag + " " + v + ", must be a number >= 0");
+  return x;
 }
 
 }  // namespace
@@ -190,52 +294,95 @@ int run_spd(const skema_mi355x::AlgParams& p, const M& A, const std::string& his
 int main(int argc, char** argv) {
   using namespace skema_mi355x;
   std::map<std::string, std::string> kv;
+  const auto& table = arg_table();
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
-    if (a.rfind("--", 0) != 0) fail("unexpected argument " + a);
-    const bool flag = (a == "--sparse" || a == "--dense" || a == "--force-three-sketch" || a == "--symmetric" ||
-                       a == "--asymmetric");
-    if (flag) kv[a] = "1";
-    else if (i + 1 < argc) kv[a] = argv[++i];
-    else fail("missing value for " + a);
+    if (a.rfind("--", 0) != 0 && table.count("--" + a)) a = "--" + a;  // "rank" for "--rank"
+    auto it = table.find(a);
+    if (it == table.end()) fail("unknown argument " + a);
+    std::string v = "1";
+    if (it->second.kind == Arg::Value) {
+      if (i + 1 >= argc) fail("missing value for " + a);
+      v = argv[++i];
+    }
+    if (!it->second.used) {
+      std::cerr << "skema_sketch: " << a << " applies to the isvd / primme solvers; ignored\n";
+      continue;
+    }
+    kv[a] = v;
+  }
+  if (kv.count("--help")) {
+    usage();
+    return 0;
   }
   if (!kv.count("--input")) fail("must provide --input");
-  if (kv.count("--solver") && kv["--solver"] != "sketch") fail("only --solver sketch is provided");
+  if (kv.count("--solver") && kv["--solver"] != "sketch") fail("only --solver sketch is provided (got " + kv["--solver"] + ")");
+  if (kv.count("--rr")) fail("--rr (the Rayleigh-Ritz pass) needs PRIMME and is not provided");
   AlgParams p;
+  // of an on/off pair the on form wins when both are given, as in
+  // parse_bool (src/Skema_AlgParams.cpp:333-360)
   const bool sparse = kv.count("--sparse") > 0;
-  p.rank = kv.count("--rank") ? std::stoll(kv["--rank"]) : 1;
-  p.sketch_range = kv.count("--range") ? std::stoll(kv["--range"]) : 1;
-  p.sketch_core = kv.count("--core") ? std::stoll(kv["--core"]) : 1;
-  p.window = kv.count("--window") ? std::stoll(kv["--window"]) : 1;
-  p.sketch_eta = kv.count("--eta") ? std::stod(kv["--eta"]) : 1.0;
-  p.sketch_nu = kv.count("--nu") ? std::stod(kv["--nu"]) : 1.0;
+  p.rank = kv.count("--rank") ? to_int("--rank", kv["--rank"], 1) : 1;
+  p.sketch_range = kv.count("--range") ? to_int("--range", kv["--range"], 0) : 1;
+  p.sketch_core = kv.count("--core") ? to_int("--core", kv["--core"], 0) : 1;
+  p.window = kv.count("--window") ? to_int("--window", kv["--window"], 1) : 1;
+  p.sketch_eta = kv.count("--eta") ? to_real("--eta", kv["--eta"]) : 1.0;
+  p.sketch_nu = kv.count("--nu") ? to_real("--nu", kv["--nu"]) : 1.0;
+  p.print_level = kv.count("--print-level") ? (int)to_int("--print-level", kv["--print-level"], 0) : 0;
+  if (p.print_level > 5) fail("Bad input: --print-level must be in [0, 5]");
+  p.debug = kv.count("--debug") > 0;
+  p.sketch_compute_svals_iters = kv.count("--sketch-compute-svals-iters") > 0;
+  if (kv.count("--debug-file")) p.debug_filename = kv["--debug-file"];
+  const double normalize = kv.count("--normalize-matrix") ? to_real("--normalize-matrix", kv["--normalize-matrix"]) : 0.0;
   if (kv.count("--seeds")) {
     auto s = parse_seeds(kv["--seeds"]);
     std::copy(s.begin(), s.end(), p.seeds.begin());
   }
   const std::string model = kv.count("--model") ? kv["--model"] : "gauss";
-  if (model != "gauss" && model != "sparse-sign") fail("--model must be gauss or sparse-sign");
+  if (model != "gauss" && model != "sparse-sign") fail("Bad input: --model " + model + ", must be one of: gauss, sparse-sign");
   p.dim_redux = model == "gauss" ? SKEMA_MAP_GAUSS : SKEMA_MAP_SPARSE_SIGN;
+  if (kv.count("--map-dist")) {
+    const std::string d = kv["--map-dist"];
+    if (d != "uniform" && d != "normal") fail("Bad input: --map-dist " + d + ", must be uniform or normal");
+    p.gauss_dist = d == "uniform" ? SKEMA_DIST_UNIFORM : SKEMA_DIST_NORMAL;
+  }
   const std::map<std::string, int> precs = {{"fp64", SKEMA_PREC_FP64}, {"fp32", SKEMA_PREC_FP32},
                                             {"bf16", SKEMA_PREC_BF16}, {"bf16x3", SKEMA_PREC_BF16X3},
                                             {"fp8", SKEMA_PREC_FP8}};
   if (kv.count("--precision")) {
-    if (!precs.count(kv["--precision"])) fail("unknown --precision");
+    if (!precs.count(kv["--precision"])) fail("unknown --precision " + kv["--precision"]);
     p.precision = precs.at(kv["--precision"]);
   }
+  if (kv.count("--kernel") && kv["--kernel"] != "gaussrbf" && kv["--kernel"] != "none")
+    fail("Bad input: --kernel " + kv["--kernel"] + ", must be one of: none, gaussrbf");
   const bool rbf = kv.count("--kernel") && kv["--kernel"] == "gaussrbf";
-  p.kernel_gamma = kv.count("--gamma") ? std::stod(kv["--gamma"]) : 1.0;
+  p.kernel_gamma = kv.count("--gamma") ? to_real("--gamma", kv["--gamma"]) : 1.0;
   const std::string in = kv["--input"];
   const bool is_bin = in.size() > 4 && in.substr(in.size() - 4) == ".bin";
   Matrix A = is_bin ? read_bin(in, sparse && !rbf) : read_mtx(in, sparse && !rbf);
+  // --normalize-matrix x divides A (for kernel input: the points) by x
+  // before sketching (driver/main.cpp:110-114, :139-141)
+  if (normalize > 0.0) {
+    for (auto& x : A.dense) x /= normalize;
+    for (auto& x : A.val) x /= normalize;
+  }
   const std::string history = kv.count("--history-file") ? kv["--history-file"] : "";
   // --symmetric selects SketchySPD; kernel input is symmetric unless
   // --force-three-sketch (driver/main.cpp:99-105, src/Skema_SketchySVD.cpp:1249)
   const bool force3 = kv.count("--force-three-sketch") > 0;
-  bool symmetric = kv.count("--symmetric") > 0 && !kv.count("--asymmetric");
+  bool symmetric = kv.count("--symmetric") > 0;
   if (rbf) symmetric = !force3;
   const bool spd = symmetric && !force3;
   const bool gauss = p.dim_redux == SKEMA_MAP_GAUSS;
+  if (p.print_level > 0) {
+    std::cout << "\n===============================================\n==================== Skema ===================="
+              << "\n===============================================\nOptions: \n  input = " << in
+              << "\n  rank = " << p.rank << "\n  window = " << p.window << "\n  model = " << model
+              << "\n  range = " << p.range() << "\n  core = " << p.core() << "\n  eta = " << p.sketch_eta
+              << "\n  nu = " << p.sketch_nu << "\n  precision = "
+              << (kv.count("--precision") ? kv["--precision"] : "fp64")
+              << "\n===============================================" << std::endl;
+  }
   try {
     Context ctx(0);
     if (rbf) {

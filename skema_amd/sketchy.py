@@ -178,13 +178,13 @@ class _DimRedux:
 
 class GaussDimRedux(_DimRedux):
     """GaussDimRedux (src/Skema_DimRedux.hpp:95-162).  Built from a seed on the
-    device (counter-based N(0,1), or the reference's U(-a,a) with
-    dist="uniform"), or from a materialised host array ``data``."""
+    device (counter-based; the reference's U(-a,a) by default, N(0,1) with
+    dist="normal"), or from a materialised host array ``data``."""
 
     issparse_ = False
 
     def __init__(self, nrow: int, ncol: int, seed: int = 0, init_transposed: bool = False, *,
-                 data: Optional[np.ndarray] = None, dist: str = "normal", ctx: Optional[Context] = None):
+                 data: Optional[np.ndarray] = None, dist: str = "uniform", ctx: Optional[Context] = None):
         ctx = ctx or default_context()
         h = C.c_void_p()
         if data is not None:
@@ -315,8 +315,9 @@ def map_shapes(p: AlgParams):
     }
 
 
-def make_maps(p: AlgParams, ctx: Optional[Context] = None, convention: str = "rows", dist: str = "normal"):
-    """Seeded maps (seeds[0..3]) with the reference's orientation."""
+def make_maps(p: AlgParams, ctx: Optional[Context] = None, convention: str = "rows", dist: str = "uniform"):
+    """Seeded maps (seeds[0..3]) with the reference's orientation; Gauss maps
+    U(-a, a) like the reference's fill, N(0, 1) with dist="normal"."""
     ctx = ctx or default_context()
     out = {}
     for name, seed in zip(("Upsilon", "Omega", "Phi", "Psi"), p.seeds):

@@ -1,5 +1,5 @@
 """Writes tests/golden/toy_sketch.npz: oracle X, Y, Z, U, S, V (and the
-materialised maps) for a 64 x 48 toy, Gauss maps from the counter stream,
+materialised maps) for a 64 x 48 toy, N(0,1) Gauss maps from the counter stream,
 windows of 10 rows, rank 4, nu = 0.9, eta = 1.1.
 
 Run from the repository root:  python tests/golden/make_golden.py
@@ -20,7 +20,7 @@ def build():
     rng = np.random.default_rng(2024)
     A = rng.standard_normal((m, n)) @ np.diag(0.8 ** np.arange(n)) + 1e-3 * rng.standard_normal((m, n))
     p = O.AlgParams(matrix_m=m, matrix_n=n, rank=4, window=10, sketch_nu=0.9, sketch_eta=1.1)
-    sk = O.SketchySVD(p)
+    sk = O.SketchySVD(p, dist="normal")
     sk.linear_update(A)
     X, Y, Z = sk.X.copy(), sk.Y.copy(), sk.Z.copy()
     U, S, V = sk.low_rank_approx()

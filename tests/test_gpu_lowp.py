@@ -15,7 +15,8 @@ from tests.gpu_helpers import dev, gpu_maps, to_gpu_params
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
-TOLS = {"bf16": 2e-2, "bf16x3": 5e-5, "fp32": 5e-5, "fp8": 1.5e-1}
+# About twice the largest normwise error measured on these cases.
+TOLS = {"bf16": 5e-3, "bf16x3": 1e-5, "fp32": 2e-6, "fp8": 8e-2}
 
 
 @pytest.fixture(scope="module")
@@ -55,9 +56,9 @@ def test_lowp_state_against_oracle(S, form, m, n, window):
     g = _run(S, A, p, osk, form)
     X, Y, Z = g.state()
     tol = TOLS[form]
-    assert O.rel_err(X, osk.X) <= tol, O.rel_err(X, osk.X)
-    assert O.rel_err(Y, osk.Y) <= tol, O.rel_err(Y, osk.Y)
-    assert O.rel_err(Z, osk.Z) <= 2 * tol, O.rel_err(Z, osk.Z)
+    ex, ey, ez = O.rel_err(X, osk.X), O.rel_err(Y, osk.Y), O.rel_err(Z, osk.Z)
+    print(f"{form} {m}x{n} w={window} rel err X {ex:.3e} Y {ey:.3e} Z {ez:.3e}", flush=True)
+    assert ex <= tol and ey <= tol and ez <= 2 * tol, (ex, ey, ez)
 
 
 @pytest.mark.parametrize("form", ["bf16", "bf16x3", "fp32", "fp8"])
@@ -82,8 +83,9 @@ def test_lowp_window_offsets_unaligned(S):
     for form in ["bf16x3", "bf16", "fp32", "fp8"]:
         g = _run(S, A, p, osk, form)
         X, Y, Z = g.state()
-        assert O.rel_err(X, osk.X) <= TOLS[form]
-        assert O.rel_err(Y, osk.Y) <= TOLS[form]
+        ex, ey = O.rel_err(X, osk.X), O.rel_err(Y, osk.Y)
+        print(f"{form} unaligned windows rel err X {ex:.3e} Y {ey:.3e}", flush=True)
+        assert ex <= TOLS[form] and ey <= TOLS[form], (form, ex, ey)
 
 
 @pytest.mark.parametrize("form", ["fp8", "bf16"])
@@ -101,9 +103,32 @@ def test_lowp_window_products_at_sketch_scale(S, form):
     g = _run(S, A, p, osk, form)
     X, Y, Z = g.state()
     tol = TOLS[form]
-    assert O.rel_err(X, osk.X) <= tol, O.rel_err(X, osk.X)
-    assert O.rel_err(Y, osk.Y) <= tol, O.rel_err(Y, osk.Y)
-    assert O.rel_err(Z, osk.Z) <= 2 * tol, O.rel_err(Z, osk.Z)
+    ex, ey, ez = O.rel_err(X, osk.X), O.rel_err(Y, osk.Y), O.rel_err(Z, osk.Z)
+    print(f"{form} sketch-scale rel err X {ex:.3e} Y {ey:.3e} Z {ez:.3e}", flush=True)
+    assert ex <= tol and ey <= tol and ez <= 2 * tol, (ex, ey, ez)
+
+
+@pytest.mark.parametrize("form", ["fp8", "bf16"])
+def test_lowp_left_product_second_tile_row(S, form):
+    """range 480 (between 449 and 512): the fp8 left product's 448-row tile
+    covers rows 0..447 and a second tile row holds rows 448..479; 4200 rows
+    give one full 4096-row window and a 104-row one."""
+    print(f"case lowp {form} range 480 second tile row", flush=True)
+    m, n = 4200, 4096
+    rng = np.random.default_rng(31)
+    A = rng.standard_normal((m, 40)) @ rng.standard_normal((40, n)) + 0.1 * rng.standard_normal((m, n))
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=100, sketch_range=480, sketch_core=961, window=4096,
+                    sketch_nu=0.9, sketch_eta=1.1)
+    osk = O.SketchySVD(p)
+    osk.linear_update(A)
+    g = _run(S, A, p, osk, form)
+    X, Y, Z = g.state()
+    tol = TOLS[form]
+    ex, ey, ez = O.rel_err(X, osk.X), O.rel_err(Y, osk.Y), O.rel_err(Z, osk.Z)
+    wx = float((np.linalg.norm(X - osk.X, axis=1) / np.linalg.norm(osk.X, axis=1)).max())
+    print(f"{form} range 480 rel err X {ex:.3e} Y {ey:.3e} Z {ez:.3e}; worst row of X {wx:.3e}", flush=True)
+    assert ex <= tol and ey <= tol and ez <= 2 * tol, (ex, ey, ez)
+    assert wx <= 2 * tol, wx
 
 
 @pytest.mark.parametrize("form", ["fp8", "bf16", "bf16x3", "fp32"])

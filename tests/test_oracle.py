@@ -347,7 +347,7 @@ def test_oracle_reproduces_committed_golden(golden_dir):
     g = np.load(f"{golden_dir}/toy_sketch.npz")
     m, n, r, w, nu, eta = g["params"]
     p = O.AlgParams(matrix_m=int(m), matrix_n=int(n), rank=int(r), window=int(w), sketch_nu=nu, sketch_eta=eta)
-    sk = O.SketchySVD(p)
+    sk = O.SketchySVD(p, dist="normal")
     for name in ("Upsilon", "Omega", "Phi", "Psi"):
         assert np.array_equal(getattr(sk, name).data, g[name])
     sk.linear_update(g["A"])
