@@ -284,6 +284,23 @@ int skema_residual_fro_dense(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, co
                              const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
                              int64_t window, double* err2, double* norm2);
 
+/* The same pass without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (the two sums, the partial sums, U diag(S) rows and the residual
+ * window) is one static 32 MiB device array per GPU, held until the process
+ * ends.  Windows are cut to the rows that fit, at most
+ * floor((32 MiB - 8448) / (8 (n + max(r, 1)))) rows; when one row does not
+ * fit (n + max(r, 1) > 4193248) the call fails with SKEMA_ERR_SHAPE before
+ * any kernel runs.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (two streams or two host threads) race.  Results equal
+ * skema_residual_fro_dense's, bit for bit when `window` rows fit.  A context
+ * must have been created on the GPU first (skema_ctx_create sets the array
+ * up). */
+int skema_residual_fro_dense_noalloc(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, const double* A, int64_t lda,
+                                     const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
+                                     int64_t window, double* err2, double* norm2);
+
 /* ---- residual norms of the final factors ---------------------------------- */
 /* rnorms[c] = ||[A^T u_c - s_c v_c ; A v_c - s_c u_c]||_2 for c < r: the
  * reference's residuals(A, U, S, V, rank, algParams, window)

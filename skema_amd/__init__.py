@@ -20,6 +20,7 @@ from .sketchy import (  # noqa: F401
     map_shapes,
     rbf_sketch_loss_grad,
     residual_fro,
+    residual_fro_noalloc,
     residual_norms,
     residual_norms_rbf,
     residual_norms_sym,

@@ -108,6 +108,8 @@ SIGNATURES = {
     "skema_sketch_timings_json": (_i32, [_vp, C.c_char_p, C.c_size_t]),
     "skema_residual_fro_dense": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
                                         C.POINTER(_d), C.POINTER(_d)]),
+    "skema_residual_fro_dense_noalloc": (_i32, [_vp, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
+                                                _i64, C.POINTER(_d), C.POINTER(_d)]),
     "skema_spd_create": (_i32, [_vp, C.POINTER(skema_params), _vp, _pp]),
     "skema_spd_destroy": (_i32, [_vp]),
     "skema_spd_update_dense": (_i32, [_vp, _vp, _i64, _i64, _i64]),

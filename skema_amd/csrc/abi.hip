@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 #include <sstream>
 
@@ -220,6 +221,10 @@ void skema_params_default(skema_params* p) {
 const char* skema_last_error(void) { return g_err.c_str(); }
 int skema_version(void) { return 10000; }
 
+namespace {
+void noalloc_lookup(int device);  // static scratch set-up (no-allocation residual pass, below)
+}  // namespace
+
 int skema_ctx_create(int device, skema_ctx** out) {
   return guarded([&] {
     SKEMA_REQUIRE(out != nullptr, kErrArg, "ctx_create: null out");
@@ -227,6 +232,7 @@ int skema_ctx_create(int device, skema_ctx** out) {
     SKEMA_HIP(hipGetDeviceCount(&count));
     SKEMA_REQUIRE(device >= 0 && device < count, kErrArg, "ctx_create: no such device");
     SKEMA_HIP(hipSetDevice(device));
+    noalloc_lookup(device);
     std::unique_ptr<skema_ctx_s> c(new skema_ctx_s);
     c->c.device = device;
     *out = c.release();
@@ -786,6 +792,75 @@ int skema_residual_fro_dense(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, co
                         part.as<double>(), acc.as<double>());
     double h[2];
     SKEMA_HIP(hipMemcpyAsync(h, acc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    SKEMA_HIP(hipStreamSynchronize(st));
+    *err2 = h[0];
+    *norm2 = h[1];
+  });
+}
+
+// ---- no-allocation residual pass -------------------------------------------
+// One static device array per GPU holds the pass's scratch: the two sums
+// (acc), the partial sums of the reduction, U diag(S) rows and the residual
+// window.  In host code `scratch` names a host-side stand-in, never the
+// device copy: its device address is looked up once per device, with that
+// device current, when a context is created on it (skema_ctx_create), and
+// only that address is handed to kernels and copies.
+namespace {
+constexpr size_t kNoAllocBytes = size_t(32) << 20;
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+constexpr size_t kNoAllocAccOff = 0;                                   // 2 doubles
+constexpr size_t kNoAllocPartOff = 256;                                // kSumSqPartials doubles
+constexpr size_t kNoAllocRowsOff = kNoAllocPartOff + kSumSqPartials * sizeof(double);  // R, then US
+
+std::mutex g_noalloc_mu;
+std::vector<void*> g_noalloc_ptr;  // per device; nullptr until looked up
+
+// Called with `device` current.
+void noalloc_lookup(int device) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if ((int)g_noalloc_ptr.size() <= device) g_noalloc_ptr.resize(device + 1, nullptr);
+  if (g_noalloc_ptr[device]) return;
+  void* ptr = nullptr;
+  SKEMA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  SKEMA_REQUIRE(ptr != nullptr, kErrHip, "static scratch: no device address");
+  g_noalloc_ptr[device] = ptr;
+}
+
+char* noalloc_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  SKEMA_REQUIRE(device < (int)g_noalloc_ptr.size() && g_noalloc_ptr[device], kErrState,
+                "static scratch: not set up on this device (create the context first)");
+  return static_cast<char*>(g_noalloc_ptr[device]);
+}
+}  // namespace
+
+int skema_residual_fro_dense_noalloc(skema_ctx* ctx, int64_t m, int64_t n, int64_t r, const double* A, int64_t lda,
+                                     const double* U, int64_t ldu, const double* S, const double* V, int64_t ldv,
+                                     int64_t window, double* err2, double* norm2) {
+  return guarded([&] {
+    SKEMA_REQUIRE(ctx, kErrArg, "residual: null argument");
+    Ctx* c = &ctx->c;
+    set_device(c);
+    check_residual_fro_args(m, n, r, A, lda, U, ldu, S, V, ldv, window, err2, norm2);
+    const int64_t rcols = std::max<int64_t>(r, 1);
+    const int64_t per_row = (int64_t)sizeof(double) * (n + rcols);
+    const int64_t fit = (int64_t)(kNoAllocBytes - kNoAllocRowsOff) / per_row;
+    SKEMA_REQUIRE(fit >= 1, kErrShape,
+                  "residual (no-alloc): one row needs " + std::to_string(per_row) + " bytes of the " +
+                      std::to_string(kNoAllocBytes - kNoAllocRowsOff) + "-byte static scratch (n + max(r,1) <= " +
+                      std::to_string((kNoAllocBytes - kNoAllocRowsOff) / sizeof(double)) + ")");
+    const int64_t w = std::min({window, m, fit});
+    char* base = noalloc_ptr(c->device);
+    double* acc = reinterpret_cast<double*>(base + kNoAllocAccOff);
+    double* part = reinterpret_cast<double*>(base + kNoAllocPartOff);
+    double* R = reinterpret_cast<double*>(base + kNoAllocRowsOff);
+    double* US = R + w * n;
+    hipStream_t st = c->stream;
+    fill_f64(st, acc, 2, 0.0);  // the array keeps the previous call's sums
+    residual_fro_stream(st, m, n, r, A, lda, U, ldu, S, V, ldv, w, R, US, part, acc);
+    double h[2];
+    SKEMA_HIP(hipMemcpyAsync(h, acc, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     SKEMA_HIP(hipStreamSynchronize(st));
     *err2 = h[0];
     *norm2 = h[1];

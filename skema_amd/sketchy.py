@@ -530,6 +530,19 @@ def residual_fro(A, U, S, V, window: int = 4096, ctx: Optional[Context] = None) 
     return e2.value, a2.value
 
 
+def residual_fro_noalloc(A, U, S, V, window: int = 4096, ctx: Optional[Context] = None) -> Tuple[float, float]:
+    """residual_fro through the entry point that allocates no device memory
+    (its scratch is the library's static per-GPU array; calls on one GPU must
+    not overlap)."""
+    ctx = ctx or default_context()
+    m, n = A.shape
+    r = U.shape[1]
+    e2, a2 = C.c_double(), C.c_double()
+    L.call("skema_residual_fro_dense_noalloc", ctx.h, m, n, r, _ptr(A), _ld(A), _ptr(U), _ld(U), _ptr(S), _ptr(V),
+           _ld(V), window, C.byref(e2), C.byref(a2))
+    return e2.value, a2.value
+
+
 def residual_norms(A, U, S, V, ctx: Optional[Context] = None) -> np.ndarray:
     """Per-triplet residual norms ||[A^T u_i - s_i v_i ; A v_i - s_i u_i]||_2
     (residuals, src/Skema_Residuals.hpp:160-217) for a dense column-major
