@@ -527,10 +527,10 @@ __global__ void k_pack_f_bf16(int64_t n, int64_t k, int64_t s, int64_t hp, const
 }
 // Logical map (rows x cols, column-major FP64, ld) -> BF16 row-major (ld ldo).
 __global__ void k_map_bf16_rows(const double* src, int64_t ld, int64_t rows, int64_t cols, __bf16* dst, int64_t ldo) {
-  const int64_t total = rows * cols;
+  const int64_t total = rows * ldo;  // the padding columns [cols, ldo) are zero
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e % rows, c = e / rows;
-    dst[r * ldo + c] = (__bf16)(float)src[r + c * ld];
+    const int64_t r = e / ldo, c = e % ldo;
+    dst[r * ldo + c] = (c < cols) ? (__bf16)(float)src[r + c * ld] : (__bf16)0.f;
   }
 }
 

@@ -101,30 +101,91 @@ __global__ __launch_bounds__(1024) void k_tile_base(const int* tot, int nt, int*
   if (tid <= nt) base[tid] = tid ? part[tid - 1] : 0;
 }
 
+// The block's nonzeros go out in batches of kL1Batch: each batch is
+// counting-sorted by column tile in LDS (entries of a tile contiguous), then
+// written as 16-byte records (row, column, value) with consecutive threads
+// on consecutive positions of each tile's run, so the writes arrive as runs
+// of whole cache lines instead of one scattered word per nonzero (which left
+// lines partly written and multiplied the HBM write traffic).
+constexpr int kL1Batch = 4096;
+
+struct CooRec {
+  int32_t row, col;
+  double val;
+};
+static_assert(sizeof(CooRec) == 16, "16-byte records");
+
 __global__ __launch_bounds__(256) void k_l1_scatter(const int64_t* rowptr, int64_t rows, const int32_t* col,
                                                    const double* val, int tc, int nt, int nb, const int* off,
-                                                   const int* base, int32_t* brow, int32_t* bcol, double* bval) {
-  __shared__ int cur[kMaxTiles];
+                                                   const int* base, CooRec* brec) {
+  constexpr int E = kL1Batch / 256;
+  __shared__ int cur[kMaxTiles];  // next global position of each tile's run
+  __shared__ int cnt[kMaxTiles];  // entries of the batch per tile
+  __shared__ int lst[kMaxTiles];  // LDS start of each tile in the batch
+  __shared__ int wsum[4];
   __shared__ int64_t rp[kL1Rows + 1];
+  __shared__ CooRec sb[kL1Batch];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int64_t r0 = (int64_t)blockIdx.x * kL1Rows;
   const int nr = (int)((r0 + kL1Rows < rows) ? kL1Rows : rows - r0);
-  for (int t = threadIdx.x; t < nt; t += 256) cur[t] = base[t] + off[(int64_t)t * nb + blockIdx.x];
-  for (int i = threadIdx.x; i <= nr; i += 256) rp[i] = rowptr[r0 + i];
+  static_assert(kMaxTiles == 256, "one thread per tile");
+  cur[tid] = (tid < nt) ? base[tid] + off[(int64_t)tid * nb + blockIdx.x] : 0;
+  for (int i = tid; i <= nr; i += 256) rp[i] = rowptr[r0 + i];
   __syncthreads();
+  const int64_t e_end = rp[nr];
   int r = 0;  // row of the thread's current nonzero (advances monotonically)
-  for (int64_t e = rp[0] + threadIdx.x; e < rp[nr]; e += 256) {
-    while (rp[r + 1] <= e) ++r;
-    const int32_t j = col[e];
-    const int pos = atomicAdd(&cur[j / tc], 1);
-    brow[pos] = (int32_t)(r0 + r);
-    bcol[pos] = j;
-    bval[pos] = val[e];
+  for (int64_t b0 = rp[0]; b0 < e_end; b0 += kL1Batch) {
+    const int nbat = (int)((e_end - b0 < kL1Batch) ? e_end - b0 : kL1Batch);
+    cnt[tid] = 0;  // nt <= 256 = blockDim
+    __syncthreads();
+    CooRec rec[E];
+    int slot[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int x = tid + 256 * q;
+      if (x < nbat) {
+        const int64_t e = b0 + x;
+        while (rp[r + 1] <= e) ++r;
+        rec[q].row = (int32_t)(r0 + r);
+        rec[q].col = col[e];
+        rec[q].val = val[e];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+      if (tid + 256 * q < nbat) slot[q] = atomicAdd(&cnt[rec[q].col / tc], 1);
+    __syncthreads();
+    {  // exclusive scan of cnt over the tiles, one thread per tile
+      const int v = cnt[tid];
+      int x = v;
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+      }
+      if (lane == 63) wsum[w] = x;
+      __syncthreads();
+      int before = 0;
+      for (int i = 0; i < w; ++i) before += wsum[i];
+      lst[tid] = before + x - v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+      if (tid + 256 * q < nbat) sb[lst[rec[q].col / tc] + slot[q]] = rec[q];
+    __syncthreads();
+    for (int x = tid; x < nbat; x += 256) {
+      const CooRec e = sb[x];
+      const int t = e.col / tc;
+      brec[cur[t] + (x - lst[t])] = e;
+    }
+    __syncthreads();
+    cur[tid] += cnt[tid];  // cur has kMaxTiles = blockDim entries; tiles >= nt unused
+    __syncthreads();
   }
 }
 
 __global__ __launch_bounds__(1024) void k_l2_place(const int* base, int nt, int tc, int64_t cols, int64_t nnz,
-                                                  const int32_t* brow, const int32_t* bcol, const double* bval,
-                                                  int64_t* cptr, int32_t* crow, double* cval) {
+                                                  const CooRec* brec, int64_t* cptr, int32_t* crow, double* cval) {
   __shared__ int cnt[kMaxTileCols];
   __shared__ int part[1024];
   const int t = blockIdx.x, tid = threadIdx.x;
@@ -134,7 +195,7 @@ __global__ __launch_bounds__(1024) void k_l2_place(const int* base, int nt, int 
   const int64_t b1 = (t + 1 < nt) ? base[t + 1] : nnz;
   for (int i = tid; i < w; i += 1024) cnt[i] = 0;
   __syncthreads();
-  for (int64_t e = b0 + tid; e < b1; e += 1024) atomicAdd(&cnt[bcol[e] - j0], 1);
+  for (int64_t e = b0 + tid; e < b1; e += 1024) atomicAdd(&cnt[brec[e].col - j0], 1);
   __syncthreads();
   // exclusive scan of cnt[0 .. w) into cursors (chunk sums, then a scan of those)
   const int chunk = (w + 1023) / 1024;
@@ -159,9 +220,10 @@ __global__ __launch_bounds__(1024) void k_l2_place(const int* base, int nt, int 
   if (t == nt - 1 && tid == 0) cptr[cols] = nnz;
   __syncthreads();
   for (int64_t e = b0 + tid; e < b1; e += 1024) {
-    const int pos = atomicAdd(&cnt[bcol[e] - j0], 1);
-    crow[pos] = brow[e];
-    cval[pos] = bval[e];
+    const CooRec x = brec[e];
+    const int pos = atomicAdd(&cnt[x.col - j0], 1);
+    crow[pos] = x.row;
+    cval[pos] = x.val;
   }
 }
 
@@ -186,29 +248,50 @@ __device__ inline double wave_max(double x) {
   return x;
 }
 
-// Adds term t (ELL: the unsigned term, scaled by the entry's sign; CSR: t
-// times the entry's value) of column c of map M into the accumulator pairs
-// acc[2 (p - p0)] for rows p of M in [p0, p0 + pn).
-template <bool ELL>
-__device__ inline void fx_scatter(const SpMapCols& M, int64_t c, double t, long long H, long long L,
-                                  const FxScale& fs, int64_t p0, int64_t pn, unsigned long long* acc) {
-  if constexpr (ELL) {
-    for (int f = 0; f < M.z; ++f) {
-      const uint32_t u = M.ell[c * M.z + f];
-      const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
-      if (p >= 0 && p < pn) {
-        if (u >> 31) fx_add_lds(acc + 2 * p, -H, -L);
-        else fx_add_lds(acc + 2 * p, H, L);
-      }
+// ELL column c of M (z <= ZM entries): the ZM loads are issued together so
+// one memory latency covers the whole column.
+template <int ZM>
+__device__ inline void ell_load(const SpMapCols& M, int64_t c, uint32_t (&u)[ZM]) {
+  if (M.z == ZM) {  // the column is ZM / 4 aligned 16-byte words: one request each
+    const uint4* src = reinterpret_cast<const uint4*>(M.ell + c * ZM);
+#pragma unroll
+    for (int w = 0; w < ZM / 4; ++w) {
+      const uint4 x = src[w];
+      u[4 * w] = x.x;
+      u[4 * w + 1] = x.y;
+      u[4 * w + 2] = x.z;
+      u[4 * w + 3] = x.w;
     }
   } else {
-    for (int64_t f = M.rp[c]; f < M.rp[c + 1]; ++f) {
-      const int64_t p = (int64_t)M.ci[f] - p0;
-      if (p >= 0 && p < pn) {
-        long long h, l;
-        fx_split(t * M.v[f], fs, h, l);
-        fx_add_lds(acc + 2 * p, h, l);
-      }
+#pragma unroll
+    for (int f = 0; f < ZM; ++f) u[f] = (f < M.z) ? M.ell[c * M.z + f] : 0u;
+  }
+}
+
+// Adds the fixed-point term (H, L), signed by each entry, into the pairs
+// acc[2 (p - p0)] of the entries' rows p in [p0, p0 + pn).
+template <int ZM>
+__device__ inline void ell_add(const uint32_t (&u)[ZM], int z, long long H, long long L, int64_t p0, int64_t pn,
+                               unsigned long long* acc) {
+#pragma unroll
+  for (int f = 0; f < ZM; ++f) {
+    const int64_t p = (int64_t)(u[f] & 0x7fffffffu) - p0;
+    if (f < z && p >= 0 && p < pn) {
+      if (u[f] >> 31) fx_add_lds(acc + 2 * p, -H, -L);
+      else fx_add_lds(acc + 2 * p, H, L);
+    }
+  }
+}
+
+// CSR form: term t times each entry of column c of M.
+__device__ inline void csr_add(const SpMapCols& M, int64_t c, double t, const FxScale& fs, int64_t p0, int64_t pn,
+                               unsigned long long* acc) {
+  for (int64_t f = M.rp[c]; f < M.rp[c + 1]; ++f) {
+    const int64_t p = (int64_t)M.ci[f] - p0;
+    if (p >= 0 && p < pn) {
+      long long h, l;
+      fx_split(t * M.v[f], fs, h, l);
+      fx_add_lds(acc + 2 * p, h, l);
     }
   }
 }
@@ -216,45 +299,112 @@ __device__ inline void fx_scatter(const SpMapCols& M, int64_t c, double t, long 
 // Left products over the CSC of A: out1 rows [p1, p1 + n1) of M1 diag(c) A
 // and out2 rows [p2, p2 + n2) of M2 diag(c) A (n2 may be 0), with c_i =
 // cw[i / window].  One wave per column j: a first pass takes the column's
-// largest term |c_i a| (the fixed-point scale of its accumulators), a second
-// adds every term into the n1 + n2 accumulator pairs of the wave in LDS;
-// ELL terms carry the map magnitude only at the end.  Four waves per block,
-// columns in a block-strided loop with a uniform trip count (barriers).
-template <bool ELL>
+// terms c_i a (up to kLeftCache per lane kept in registers) and their
+// largest magnitude (the fixed-point scale of the column's accumulators); a
+// second adds every term into the n1 + n2 accumulator pairs of the wave in
+// LDS, the map columns of all cached terms loaded before any is added.  ELL
+// terms carry the map magnitude only at the end.  ZM: ELL entries per map
+// column at most (both maps), 0 for maps given as CSR.  Four waves per
+// block, columns in a block-strided loop with a uniform trip count
+// (barriers).
+constexpr int kLeftCache = 4;
+
+template <int ZM>
 __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int32_t* crow, const double* cval,
                                                 int64_t ncols, const double* cw, int64_t window, SpMapCols M1,
                                                 int64_t p1, int64_t n1, double* out1, int64_t ld1, SpMapCols M2,
                                                 int64_t p2, int64_t n2, double* out2, int64_t ld2) {
   extern __shared__ unsigned long long fxacc[];  // 4 waves x 2 (n1 + n2)
+  constexpr bool ELL = ZM > 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t na = n1 + n2;
   unsigned long long* acc = fxacc + (int64_t)wave * 2 * na;
   const int64_t groups = (ncols + 3) / 4;
+  const uint32_t wdw = (uint32_t)window;
   for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
     const int64_t j = 4 * g + wave;
     for (int64_t q = lane; q < 2 * na; q += 64) acc[q] = 0ull;
     FxScale fs;
     if (j < ncols) {
       const int64_t e0 = cptr[j], e1 = cptr[j + 1];
+      double tq[kLeftCache];
+      int32_t iq[kLeftCache];
       double mx = 0.0;
-      for (int64_t e = e0 + lane; e < e1; e += 64) {
-        const int64_t i = crow[e];
-        mx = fmax(mx, fabs(cval[e] * cw[(uint32_t)i / (uint32_t)window]));
-        if (isnan(cval[e] * cw[(uint32_t)i / (uint32_t)window])) mx = __builtin_inf();
+      bool bad = false;
+#pragma unroll
+      for (int q = 0; q < kLeftCache; ++q) {
+        const int64_t e = e0 + lane + 64 * q;
+        iq[q] = (e < e1) ? crow[e] : -1;
+        tq[q] = (e < e1) ? cval[e] : 0.0;
       }
+      double cq[kLeftCache];  // window weights: unconditional loads (cw[0] always exists), all in flight
+#pragma unroll
+      for (int q = 0; q < kLeftCache; ++q) cq[q] = cw[(uint32_t)(iq[q] > 0 ? iq[q] : 0) / wdw];
+      // the map columns of the cached terms, in flight with the weights and
+      // across the scale reduction below
+      uint32_t u1[kLeftCache][ZM > 0 ? ZM : 1], u2[kLeftCache][ZM > 0 ? ZM : 1];
+      if constexpr (ELL) {
+#pragma unroll
+        for (int q = 0; q < kLeftCache; ++q) {
+          if (iq[q] >= 0) {
+            ell_load<ZM>(M1, iq[q], u1[q]);
+            if (n2) ell_load<ZM>(M2, iq[q], u2[q]);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kLeftCache; ++q) {
+        tq[q] *= cq[q];  // 0 for the lanes past the column's end
+        mx = fmax(mx, fabs(tq[q]));
+        bad |= isnan(tq[q]);
+      }
+      for (int64_t e = e0 + 64 * kLeftCache + lane; e < e1; e += 64) {
+        const double t = cval[e] * cw[(uint32_t)crow[e] / wdw];
+        mx = fmax(mx, fabs(t));
+        bad |= isnan(t);
+      }
+      if (bad) mx = __builtin_inf();
       mx = wave_max(mx);
       // a target receives at most zmax terms per nonzero (entries of one map
       // column may repeat a row)
       const int64_t terms = (e1 - e0) * (M1.zmax > M2.zmax ? M1.zmax : M2.zmax);
       fs = fx_scale(ELL ? mx : mx * (M1.vmax > M2.vmax ? M1.vmax : M2.vmax), terms);
       if (fs.finite) {
-        for (int64_t e = e0 + lane; e < e1; e += 64) {
-          const int64_t i = crow[e];
-          const double t = cval[e] * cw[(uint32_t)i / (uint32_t)window];
-          long long H = 0, L = 0;
-          if (ELL) fx_split(t, fs, H, L);
-          fx_scatter<ELL>(M1, i, t, H, L, fs, p1, n1, acc);
-          if (n2) fx_scatter<ELL>(M2, i, t, H, L, fs, p2, n2, acc + 2 * n1);
+        if constexpr (ELL) {
+#pragma unroll
+          for (int q = 0; q < kLeftCache; ++q) {
+            if (iq[q] >= 0) {
+              long long H, L;
+              fx_split(tq[q], fs, H, L);
+              ell_add<ZM>(u1[q], M1.z, H, L, p1, n1, acc);
+              if (n2) ell_add<ZM>(u2[q], M2.z, H, L, p2, n2, acc + 2 * n1);
+            }
+          }
+          for (int64_t e = e0 + 64 * kLeftCache + lane; e < e1; e += 64) {
+            const int32_t i = crow[e];
+            const double t = cval[e] * cw[(uint32_t)i / wdw];
+            uint32_t v1[ZM], v2[ZM];
+            ell_load<ZM>(M1, i, v1);
+            if (n2) ell_load<ZM>(M2, i, v2);
+            long long H, L;
+            fx_split(t, fs, H, L);
+            ell_add<ZM>(v1, M1.z, H, L, p1, n1, acc);
+            if (n2) ell_add<ZM>(v2, M2.z, H, L, p2, n2, acc + 2 * n1);
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < kLeftCache; ++q) {
+            if (iq[q] >= 0) {
+              csr_add(M1, iq[q], tq[q], fs, p1, n1, acc);
+              if (n2) csr_add(M2, iq[q], tq[q], fs, p2, n2, acc + 2 * n1);
+            }
+          }
+          for (int64_t e = e0 + 64 * kLeftCache + lane; e < e1; e += 64) {
+            const int32_t i = crow[e];
+            const double t = cval[e] * cw[(uint32_t)i / wdw];
+            csr_add(M1, i, t, fs, p1, n1, acc);
+            if (n2) csr_add(M2, i, t, fs, p2, n2, acc + 2 * n1);
+          }
         }
       }
     }
@@ -271,14 +421,18 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
 }
 
 // Y(i, p) = eta * sum over row i's nonzeros (j, a) of a * M(p, j) for p in
-// [p0, p0 + pn), pn <= kRightSlots.  One block per kRightRows rows: a first
-// pass (one thread per row) takes the row's largest |a| (its scale), then
-// the threads take the block's nonzeros and add into acc[slot][row] pairs;
-// the rows are written out coalesced (consecutive threads, consecutive rows).
+// [p0, p0 + pn), pn <= kRightSlots.  One block per kRightRows rows; each
+// thread takes the block's nonzeros tid, tid + 256, ... (kRightCache of them
+// held in registers, with their map columns loaded at once), the rows'
+// largest |a| (their fixed-point scales) by integer atomicMax on the IEEE
+// bits, then adds every term into acc[slot][row] pairs; the rows are written
+// out coalesced (consecutive threads, consecutive rows).  ZM as for
+// k_left_fx.
 constexpr int kRightRows = 32;
 constexpr int kRightSlots = 128;
+constexpr int kRightCache = 3;
 
-template <bool ELL>
+template <int ZM>
 __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const int32_t* col, const double* val,
                                                  int64_t rows, SpMapCols M, int64_t p0, int64_t pn, double eta,
                                                  double* Y, int64_t ldy) {
@@ -286,6 +440,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
   __shared__ int64_t rp[kRightRows + 1];
   __shared__ unsigned long long rmax[kRightRows];  // bits of the row's largest |a| (non-negative doubles order as integers)
   __shared__ FxScale rs[kRightRows];
+  constexpr bool ELL = ZM > 0;
   const int tid = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * kRightRows;
   const int nr = (int)((r0 + kRightRows < rows) ? kRightRows : rows - r0);
@@ -293,11 +448,39 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
   for (int i = tid; i <= nr; i += 256) rp[i] = rowptr[r0 + i];
   if (tid < kRightRows) rmax[tid] = 0ull;
   __syncthreads();
-  {
-    int r = 0;
-    for (int64_t e = rp[0] + tid; e < rp[nr]; e += 256) {
+  const int64_t e_end = rp[nr];
+  int64_t jq[kRightCache];
+  double tq[kRightCache];
+  int rq[kRightCache];
+  int r = 0;
+#pragma unroll
+  for (int q = 0; q < kRightCache; ++q) {
+    const int64_t e = rp[0] + tid + 256 * q;
+    if (e < e_end) {
       while (rp[r + 1] <= e) ++r;
-      atomicMax(&rmax[r], (unsigned long long)__double_as_longlong(fabs(val[e])));  // NaN orders above inf
+      jq[q] = col[e];
+      tq[q] = val[e];
+      rq[q] = r;
+    } else {
+      jq[q] = -1;
+      tq[q] = 0.0;
+      rq[q] = 0;
+    }
+  }
+  uint32_t u[kRightCache][ZM > 0 ? ZM : 1];
+  if constexpr (ELL) {
+#pragma unroll
+    for (int q = 0; q < kRightCache; ++q)
+      if (jq[q] >= 0) ell_load<ZM>(M, jq[q], u[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < kRightCache; ++q)
+    if (jq[q] >= 0) atomicMax(&rmax[rq[q]], (unsigned long long)__double_as_longlong(fabs(tq[q])));  // NaN orders above inf
+  {
+    int rr = r;
+    for (int64_t e = rp[0] + tid + 256 * kRightCache; e < e_end; e += 256) {
+      while (rp[rr + 1] <= e) ++rr;
+      atomicMax(&rmax[rr], (unsigned long long)__double_as_longlong(fabs(val[e])));
     }
   }
   __syncthreads();
@@ -307,34 +490,48 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
     rs[tid] = fx_scale(ELL ? mx : mx * M.vmax, cnt * M.zmax);
   }
   __syncthreads();
-  int r = 0;
-  for (int64_t e = rp[0] + tid; e < rp[nr]; e += 256) {
+  // adds the term t of row rr (nonzero in column j, map column entries uu)
+  auto add_ell = [&](const uint32_t (&uu)[ZM > 0 ? ZM : 1], double t, int rr) {
+    const FxScale fs = rs[rr];
+    if (!fs.finite) return;
+    long long H, L;
+    fx_split(t, fs, H, L);
+#pragma unroll
+    for (int f = 0; f < (ZM > 0 ? ZM : 1); ++f) {
+      const int64_t p = (int64_t)(uu[f] & 0x7fffffffu) - p0;
+      if (f < M.z && p >= 0 && p < pn) {
+        unsigned long long* a = fxacc + 2 * (p * kRightRows + rr);
+        if (uu[f] >> 31) fx_add_lds(a, -H, -L);
+        else fx_add_lds(a, H, L);
+      }
+    }
+  };
+  auto add_csr = [&](int64_t j, double t, int rr) {
+    const FxScale fs = rs[rr];
+    if (!fs.finite) return;
+    for (int64_t f = M.rp[j]; f < M.rp[j + 1]; ++f) {
+      const int64_t p = (int64_t)M.ci[f] - p0;
+      if (p >= 0 && p < pn) {
+        long long h, l;
+        fx_split(t * M.v[f], fs, h, l);
+        fx_add_lds(fxacc + 2 * (p * kRightRows + rr), h, l);
+      }
+    }
+  };
+#pragma unroll
+  for (int q = 0; q < kRightCache; ++q) {
+    if (jq[q] < 0) continue;
+    if constexpr (ELL) add_ell(u[q], tq[q], rq[q]);
+    else add_csr(jq[q], tq[q], rq[q]);
+  }
+  for (int64_t e = rp[0] + tid + 256 * kRightCache; e < e_end; e += 256) {  // rows denser than the cache
     while (rp[r + 1] <= e) ++r;
-    const FxScale fs = rs[r];
-    if (!fs.finite) continue;
-    const int64_t j = col[e];
-    const double t = val[e];
     if constexpr (ELL) {
-      long long H, L;
-      fx_split(t, fs, H, L);
-      for (int f = 0; f < M.z; ++f) {
-        const uint32_t u = M.ell[j * M.z + f];
-        const int64_t p = (int64_t)(u & 0x7fffffffu) - p0;
-        if (p >= 0 && p < pn) {
-          unsigned long long* a = fxacc + 2 * (p * kRightRows + r);
-          if (u >> 31) fx_add_lds(a, -H, -L);
-          else fx_add_lds(a, H, L);
-        }
-      }
+      uint32_t uu[ZM > 0 ? ZM : 1];
+      ell_load<ZM>(M, col[e], uu);
+      add_ell(uu, val[e], r);
     } else {
-      for (int64_t f = M.rp[j]; f < M.rp[j + 1]; ++f) {
-        const int64_t p = (int64_t)M.ci[f] - p0;
-        if (p >= 0 && p < pn) {
-          long long h, l;
-          fx_split(t * M.v[f], fs, h, l);
-          fx_add_lds(fxacc + 2 * (p * kRightRows + r), h, l);
-        }
-      }
+      add_csr(col[e], val[e], r);
     }
   }
   __syncthreads();
@@ -432,18 +629,22 @@ SpMapCols map_cols(const Map& M) {
 // Accumulator pairs per wave of k_left_fx (4 waves: 64 KB of LDS).
 constexpr int64_t kLeftSlots = 512;
 
-void launch_left_fx(hipStream_t st, bool ell, const CscWindow& H, const double* cw, int64_t window,
+// ELL entries per map column the kernels are compiled for (0: CSR maps).
+int zm_class(int z) { return z <= 0 ? 0 : (z <= 8 ? 8 : 16); }
+
+void launch_left_fx(hipStream_t st, int zm, const CscWindow& H, const double* cw, int64_t window,
                     const SpMapCols& M1, int64_t p1, int64_t n1, double* o1, int64_t ld1, const SpMapCols& M2,
                     int64_t p2, int64_t n2, double* o2, int64_t ld2) {
   const int64_t groups = cdiv(H.cols, 4);
   const unsigned grid = (unsigned)std::min<int64_t>(groups, 8192);
   const size_t lds = (size_t)4 * 2 * (n1 + n2) * sizeof(unsigned long long);
-  if (ell)
-    hipLaunchKernelGGL(k_left_fx<true>, dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, M1,
-                       p1, n1, o1, ld1, M2, p2, n2, o2, ld2);
-  else
-    hipLaunchKernelGGL(k_left_fx<false>, dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, M1,
-                       p1, n1, o1, ld1, M2, p2, n2, o2, ld2);
+#define SKEMA_LEFT_FX(ZM)                                                                                         \
+  hipLaunchKernelGGL(k_left_fx<ZM>, dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, M1, \
+                     p1, n1, o1, ld1, M2, p2, n2, o2, ld2)
+  if (zm == 8) SKEMA_LEFT_FX(8);
+  else if (zm == 16) SKEMA_LEFT_FX(16);
+  else SKEMA_LEFT_FX(0);
+#undef SKEMA_LEFT_FX
   check_launch("k_left_fx");
 }
 
@@ -454,15 +655,15 @@ void left_sparse_fx(hipStream_t st, const CscWindow& H, const Map& Ma, int64_t k
                     const Map& Mb, int64_t kb, double* ob, int64_t ldb, const double* cw, int64_t window) {
   if (H.cols == 0) return;
   const SpMapCols ca = map_cols(Ma), cb = map_cols(Mb);
-  const bool ea = Ma.ell_zeta > 0, eb = Mb.ell_zeta > 0;
-  if (ea == eb && ka + kb <= kLeftSlots) {
-    launch_left_fx(st, ea, H, cw, window, ca, 0, ka, oa, lda, cb, 0, kb, ob, ldb);
+  const int za = zm_class(Ma.ell_zeta), zb = zm_class(Mb.ell_zeta);
+  if ((za > 0) == (zb > 0) && ka + kb <= kLeftSlots) {
+    launch_left_fx(st, std::max(za, zb), H, cw, window, ca, 0, ka, oa, lda, cb, 0, kb, ob, ldb);
     return;
   }
   for (int64_t p0 = 0; p0 < ka; p0 += kLeftSlots)
-    launch_left_fx(st, ea, H, cw, window, ca, p0, std::min(kLeftSlots, ka - p0), oa, lda, ca, 0, 0, nullptr, 1);
+    launch_left_fx(st, za, H, cw, window, ca, p0, std::min(kLeftSlots, ka - p0), oa, lda, ca, 0, 0, nullptr, 1);
   for (int64_t p0 = 0; p0 < kb; p0 += kLeftSlots)
-    launch_left_fx(st, eb, H, cw, window, cb, p0, std::min(kLeftSlots, kb - p0), ob, ldb, cb, 0, 0, nullptr, 1);
+    launch_left_fx(st, zb, H, cw, window, cb, p0, std::min(kLeftSlots, kb - p0), ob, ldb, cb, 0, 0, nullptr, 1);
 }
 
 // Y = eta A M^T by row blocks, in chunks of kRightSlots map rows.
@@ -473,10 +674,13 @@ void right_sparse_fx(hipStream_t st, const int64_t* rowptr, const int32_t* col, 
   for (int64_t p0 = 0; p0 < k; p0 += kRightSlots) {
     const int64_t pn = std::min<int64_t>(kRightSlots, k - p0);
     const size_t lds = (size_t)2 * pn * kRightRows * sizeof(unsigned long long);
-    if (M.ell_zeta > 0)
-      hipLaunchKernelGGL(k_right_fx<true>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
+    const int zm = zm_class(M.ell_zeta);
+    if (zm == 8)
+      hipLaunchKernelGGL(k_right_fx<8>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
+    else if (zm == 16)
+      hipLaunchKernelGGL(k_right_fx<16>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
     else
-      hipLaunchKernelGGL(k_right_fx<false>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
+      hipLaunchKernelGGL(k_right_fx<0>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
     check_launch("k_right_fx");
   }
 }
@@ -488,16 +692,14 @@ bool csc_by_buckets(hipStream_t st, const int64_t* rowptr, const int32_t* col, c
   int64_t nt = std::min<int64_t>(kMaxTiles, n);
   const int64_t tc = cdiv(n, nt);
   nt = cdiv(n, tc);
-  if (tc > kMaxTileCols || nnz <= 0) return false;
+  if (tc > kMaxTileCols || nnz <= 0 || nnz >= ((int64_t)1 << 31)) return false;  // int positions
   const int64_t nb = cdiv(m, kL1Rows);
   SKEMA_REQUIRE(nt * nb < ((int64_t)1 << 31), kErrShape, "linear_update_csr: too many row blocks");
   const size_t o_off = align_up((size_t)nt * nb * sizeof(int), 256);
   const size_t o_tot = o_off + align_up((size_t)nt * nb * sizeof(int), 256);
   const size_t o_base = o_tot + align_up((size_t)nt * sizeof(int), 256);
-  const size_t o_brow = o_base + align_up((size_t)(nt + 1) * sizeof(int), 256);
-  const size_t o_bcol = o_brow + align_up((size_t)nnz * sizeof(int32_t), 256);
-  const size_t o_bval = o_bcol + align_up((size_t)nnz * sizeof(int32_t), 256);
-  const size_t o_ptr = o_bval + align_up((size_t)nnz * sizeof(double), 256);
+  const size_t o_brec = o_base + align_up((size_t)(nt + 1) * sizeof(int), 256);
+  const size_t o_ptr = o_brec + align_up((size_t)nnz * sizeof(CooRec), 256);
   const size_t o_row = o_ptr + align_up((size_t)(n + 1) * sizeof(int64_t), 256);
   const size_t o_val = o_row + align_up((size_t)nnz * sizeof(int32_t), 256);
   scratch.reserve(o_val + (size_t)nnz * sizeof(double));
@@ -506,9 +708,7 @@ bool csc_by_buckets(hipStream_t st, const int64_t* rowptr, const int32_t* col, c
   int* off = reinterpret_cast<int*>(b + o_off);
   int* tot = reinterpret_cast<int*>(b + o_tot);
   int* base = reinterpret_cast<int*>(b + o_base);
-  int32_t* brow = reinterpret_cast<int32_t*>(b + o_brow);
-  int32_t* bcol = reinterpret_cast<int32_t*>(b + o_bcol);
-  double* bval = reinterpret_cast<double*>(b + o_bval);
+  CooRec* brec = reinterpret_cast<CooRec*>(b + o_brec);
   H.rows = m;
   H.cols = n;
   H.nnz = nnz;
@@ -519,9 +719,9 @@ bool csc_by_buckets(hipStream_t st, const int64_t* rowptr, const int32_t* col, c
   hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)nt), dim3(1024), 0, st, cnt, (int)nb, off, tot);
   hipLaunchKernelGGL(k_tile_base, dim3(1), dim3(1024), 0, st, tot, (int)nt, base);
   hipLaunchKernelGGL(k_l1_scatter, dim3((unsigned)nb), dim3(256), 0, st, rowptr, m, col, val, (int)tc, (int)nt,
-                     (int)nb, off, base, brow, bcol, bval);
-  hipLaunchKernelGGL(k_l2_place, dim3((unsigned)nt), dim3(1024), 0, st, base, (int)nt, (int)tc, n, nnz, brow, bcol,
-                     bval, H.ptr, H.row, H.val);
+                     (int)nb, off, base, brec);
+  hipLaunchKernelGGL(k_l2_place, dim3((unsigned)nt), dim3(1024), 0, st, base, (int)nt, (int)tc, n, nnz, brec, H.ptr,
+                     H.row, H.val);
   check_launch("csc_by_buckets");
   return true;
 }

@@ -34,6 +34,39 @@ __device__ double block_sum(double v, double* red) {
   return s;
 }
 
+// Sum of v[k] over the 64 lanes of a wave for 32 values per lane; lane l
+// returns the total of value (l >> 1) & 31.  Reduce-scatter by halving
+// (16 + 8 + 4 + 2 + 1 exchanges), then one exchange with the neighbouring
+// lane: 32 exchanges instead of 32 x 6.
+template <int H, int O, int N>
+__device__ inline void halve(double (&v)[N], bool up) {
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const double send = up ? v[k] : v[k + H];
+    const double keep = up ? v[k + H] : v[k];
+    v[k] = keep + __shfl_xor(send, O);
+  }
+}
+__device__ inline double wave_sum32(double (&v)[32]) {
+  const int lane = threadIdx.x & 63;
+  halve<16, 32>(v, lane & 32);
+  halve<8, 16>(v, lane & 16);
+  halve<4, 8>(v, lane & 8);
+  halve<2, 4>(v, lane & 4);
+  halve<1, 2>(v, lane & 2);
+  return v[0] + __shfl_xor(v[0], 1);
+}
+// Same for 16 values per lane: lane l returns the total of value (l >> 2) & 15.
+__device__ inline double wave_sum16(double (&v)[16]) {
+  const int lane = threadIdx.x & 63;
+  halve<8, 32>(v, lane & 32);
+  halve<4, 16>(v, lane & 16);
+  halve<2, 8>(v, lane & 8);
+  halve<1, 4>(v, lane & 4);
+  const double t = v[0] + __shfl_xor(v[0], 2);
+  return t + __shfl_xor(t, 1);
+}
+
 // dlarfg on column `col` of length len (col[0] = alpha, col[1..] = x).
 // The thread->element assignment (i = tid + k*bs) is the one every caller
 // uses for its own writes, so no cross-thread global visibility is needed.
@@ -211,6 +244,143 @@ __global__ __launch_bounds__(256) void k_jacobi_round(double* A, int64_t lda, do
   if (tid == 0) atomicAdd(rotations, 1);
 }
 
+// Block-pair Jacobi round in registers.  The columns come in blocks of
+// kJacobiBlock = 8; a round's block pairs are disjoint and each takes one
+// 512-thread workgroup in which thread i holds row i of the pair's 16
+// columns of A and of V (orders up to 512), so a rotation is local to every
+// thread and only the inner products need a reduction.  A local round
+// rotates 8 disjoint column pairs at fixed register positions: one
+// block-wide reduction of their 24 inner products (fixed order), then each
+// thread applies the 8 rotations to its row.  Between local rounds the
+// columns move instead of the pair indices (so no register is indexed at run
+// time):
+//   intra (the first round of each sweep, every block once): pairs
+//     (p, 7 - p) within each block, then positions 1..7 of each block
+//     rotate by one (the circle method; 7 rounds, back in place after);
+//   cross: pairs (p, 8 + p), then the second block shifts by one position
+//     (8 rounds, back in place after).
+// Padding columns (>= n) are zero and never rotate.
+constexpr int kJacobiBlock = 8;
+
+// Rotation of columns x, y from their inner products (the k_jacobi_round
+// formula); false when they are already orthogonal to tol.
+__device__ inline bool jacobi_cs(double alpha, double beta, double gamma, double tol, double& c, double& s) {
+  if (!(fabs(gamma) > tol * sqrt(alpha * beta))) return false;
+  const double zeta = (beta - alpha) / (2.0 * gamma);
+  const double az = fabs(zeta);
+  const double t = (az > 1e150) ? 0.5 / zeta : copysign(1.0, zeta) / (az + sqrt(1.0 + zeta * zeta));
+  c = 1.0 / sqrt(1.0 + t * t);
+  s = c * t;
+  return true;
+}
+
+__global__ __launch_bounds__(512) void k_jacobi_rows(double* A, int64_t lda, double* V, int64_t ldv, int64_t n,
+                                                     const int2* bpairs, int intra, double tol, int* rotations) {
+  constexpr int NB = kJacobiBlock, NC = 2 * NB;
+  __shared__ double red[8][24];
+  __shared__ double par[24];
+  __shared__ double cs[8][2];
+  __shared__ int rotp[8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int2 bp = bpairs[blockIdx.x];
+  const int64_t i = tid;
+  double a[NC], v[NC];
+#pragma unroll
+  for (int t = 0; t < NC; ++t) {
+    const int64_t c = (int64_t)(t < NB ? bp.x : bp.y) * NB + (t % NB);
+    const bool ok = i < n && c < n;
+    a[t] = ok ? A[i + c * lda] : 0.0;
+    v[t] = ok ? V[i + c * ldv] : 0.0;
+  }
+  int count = 0;
+  // one local round on the pairs (pos_a(k), pos_b(k)), k < 8 (register
+  // positions known at compile time)
+  auto round = [&](auto pos_a, auto pos_b) {
+    double w[32];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double x = a[pos_a(k)], y = a[pos_b(k)];
+      w[3 * k] = x * x;
+      w[3 * k + 1] = y * y;
+      w[3 * k + 2] = x * y;
+    }
+#pragma unroll
+    for (int k = 24; k < 32; ++k) w[k] = 0.0;
+    const double tot = wave_sum32(w);
+    if ((lane & 1) == 0 && (lane >> 1) < 24) red[wave][lane >> 1] = tot;
+    __syncthreads();
+    if (tid < 24) {
+      double t = 0.0;
+      for (int q = 0; q < 8; ++q) t += red[q][tid];
+      par[tid] = t;
+    }
+    __syncthreads();
+    if (tid < 8) {  // the 8 rotations, once each (FP64 divisions and square roots)
+      double c = 1.0, s = 0.0;
+      const bool rot = jacobi_cs(par[3 * tid], par[3 * tid + 1], par[3 * tid + 2], tol, c, s);
+      cs[tid][0] = c;
+      cs[tid][1] = rot ? s : 0.0;
+      rotp[tid] = rot ? 1 : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (rotp[k]) {
+        const double c = cs[k][0], s = cs[k][1];
+        const int pa = pos_a(k), pb = pos_b(k);
+        const double x = a[pa], y = a[pb];
+        a[pa] = c * x - s * y;
+        a[pb] = s * x + c * y;
+        const double vx = v[pa], vy = v[pb];
+        v[pa] = c * vx - s * vy;
+        v[pb] = s * vx + c * vy;
+        ++count;
+      }
+    }
+    __syncthreads();  // par, red and cs are rewritten by the next round
+  };
+  if (intra) {
+    for (int r = 0; r < NB - 1; ++r) {
+      // pairs (p, 7 - p) in block X (k < 4) and block Y (k >= 4)
+      round([](int k) { return k < 4 ? k : NB + (k - 4); }, [](int k) { return k < 4 ? NB - 1 - k : NC - 1 - (k - 4); });
+      // circle method: positions 1 .. 7 of each block rotate by one
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int o = NB * b;
+        const double ta = a[o + NB - 1], tv = v[o + NB - 1];
+#pragma unroll
+        for (int p = NB - 1; p > 1; --p) {
+          a[o + p] = a[o + p - 1];
+          v[o + p] = v[o + p - 1];
+        }
+        a[o + 1] = ta;
+        v[o + 1] = tv;
+      }
+    }
+  }
+  for (int r = 0; r < NB; ++r) {
+    round([](int k) { return k; }, [](int k) { return NB + k; });
+    // the second block shifts by one position
+    const double ta = a[NB], tv = v[NB];
+#pragma unroll
+    for (int p = NB; p < NC - 1; ++p) {
+      a[p] = a[p + 1];
+      v[p] = v[p + 1];
+    }
+    a[NC - 1] = ta;
+    v[NC - 1] = tv;
+  }
+#pragma unroll
+  for (int t = 0; t < NC; ++t) {
+    const int64_t c = (int64_t)(t < NB ? bp.x : bp.y) * NB + (t % NB);
+    if (i < n && c < n) {
+      A[i + c * lda] = a[t];
+      V[i + c * ldv] = v[t];
+    }
+  }
+  if (tid == 0 && count) atomicAdd(rotations, count);
+}
+
 __global__ __launch_bounds__(256) void k_col_norms(const double* A, int64_t lda, int64_t n, double* S) {
   __shared__ double red[4];
   const double* a = A + (int64_t)blockIdx.x * lda;
@@ -252,22 +422,28 @@ __global__ void k_build_v(const double* A, int64_t lda, int64_t rows, int64_t jb
 }
 
 // dlarft (forward, columnwise) from G = V^T V: T upper triangular (jb x jb,
-// jb <= 64) with H(0) H(1) ... H(jb-1) = I - V T V^T, i.e. T(i, i) = tau_i and
-// T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).  One 64-thread block.
+// jb <= 32) with H(0) H(1) ... H(jb-1) = I - V T V^T, i.e. T(i, i) = tau_i and
+// T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).  One 64-thread block, G and T in
+// LDS.
 __global__ __launch_bounds__(64) void k_build_t(const double* G, int64_t jb, const double* tau, double* T) {
+  __shared__ double Gs[32][33], Ts[32][33];
   const int t = threadIdx.x;
+  for (int e = t; e < jb * jb; e += 64) {
+    Gs[e % jb][e / jb] = G[e];
+    Ts[e % jb][e / jb] = 0.0;
+  }
+  __syncthreads();
   for (int64_t i = 0; i < jb; ++i) {
     if (t < i) {
       double w = 0.0;
-      for (int64_t l = t; l < i; ++l) w += T[t + l * jb] * G[l + i * jb];
-      T[t + i * jb] = -tau[i] * w;
+      for (int64_t l = t; l < i; ++l) w += Ts[t][l] * Gs[l][i];
+      Ts[t][i] = -tau[i] * w;
     } else if (t == i) {
-      T[i + i * jb] = tau[i];
-    } else if (t < jb) {
-      T[t + i * jb] = 0.0;
+      Ts[i][i] = tau[i];
     }
     __syncthreads();
   }
+  for (int e = t; e < jb * jb; e += 64) T[e] = Ts[e % jb][e / jb];
 }
 
 // Q (rows x cols) = the first cols columns of the identity.
@@ -281,34 +457,306 @@ __global__ void k_eye_rect(int64_t rows, int64_t cols, double* Q, int64_t ldq) {
 
 constexpr int64_t kQrPanel = 32;
 
-unsigned grid_for_elems(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, 256), 65536)); }
+// ---- panel factorization (dgeqr2 on rows j0..m-1 of a <= 32-column panel) ----
+// Step j of Householder QR needs, over the rows i > j of the panel, the
+// squares of column j and its products with the later columns: with
+// s_c = sum_{i>j} x_i a_ic (x = column j, s_j its squared norm), alpha =
+// a_jj, beta = -sign(alpha) sqrt(alpha^2 + s_j), tau = (beta - alpha)/beta,
+// v = [1; x / (alpha - beta)], the update of column c is
+// a_c -= v tau (a_jc + s_c / (alpha - beta)).  So one reduction per column
+// (the row of products) drives the whole step.
 
-// Workspace of the blocked QR: V (m x nb), G and T (nb x nb), two nb x n.
-struct QrWork {
-  double *V, *G, *T, *W1, *W2;
-};
-QrWork qr_work(DevBuf* work, int64_t m, int64_t n) {
-  const int64_t nb = kQrPanel;
-  const size_t oV = 0, oG = align_up((size_t)m * nb * sizeof(double), 256);
-  const size_t oT = oG + align_up((size_t)nb * nb * sizeof(double), 256);
-  const size_t o1 = oT + align_up((size_t)nb * nb * sizeof(double), 256);
-  const size_t o2 = o1 + align_up((size_t)nb * n * sizeof(double), 256);
-  work->reserve(o2 + (size_t)nb * n * sizeof(double));
-  char* b = work->as<char>();
-  return QrWork{reinterpret_cast<double*>(b + oV), reinterpret_cast<double*>(b + oG),
-                reinterpret_cast<double*>(b + oT), reinterpret_cast<double*>(b + o1),
-                reinterpret_cast<double*>(b + o2)};
+// Reflector of step j from the reduced row of products ss (ss[j] = s_j) and
+// the row j of the panel: returns false when x = 0 (tau = 0, no update).
+__device__ inline bool reflector(double alpha, double xnorm2, double& beta, double& scal, double& tau) {
+  if (xnorm2 == 0.0) {
+    tau = 0.0;
+    beta = alpha;
+    scal = 0.0;
+    return false;
+  }
+  beta = -copysign(hypot(alpha, sqrt(xnorm2)), alpha);
+  scal = 1.0 / (alpha - beta);
+  tau = (beta - alpha) / beta;
+  return true;
 }
 
-// V and T of the panel at (j0, j0) with jb columns (rows j0 .. m-1).
-void panel_vt(hipStream_t st, const double* A, int64_t lda, int64_t m, int64_t j0, int64_t jb, const double* tau,
-              const QrWork& w, GemmWorkspace* gws) {
+// Short panels (mp <= 1024 rows): one block of 512 threads, two rows per
+// thread held in registers, the jb steps in one launch.
+__global__ __launch_bounds__(512) void k_panel_small(double* P, int64_t lda, int64_t mp, int jb, double* tau) {
+  constexpr int RPT = 2;
+  __shared__ double red[8][32];
+  __shared__ double ss[32], row[32], wv[32];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double a[RPT][32];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = tid + 512 * q;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) a[q][c] = (i < mp && c < jb) ? P[i + c * lda] : 0.0;
+  }
+  for (int j = 0; j < jb; ++j) {
+    double x[RPT];  // a[q][j] without a dynamic register index
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      x[q] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 32; ++c)
+        if (c == j) x[q] = a[q][c];
+    }
+    // products with the columns c >= j, 16 columns at a time (register budget)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (16 * h + 15 < j) continue;  // wave-uniform: the whole chunk is before j
+      double v[16];
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc) {
+        const int c = 16 * h + cc;
+        v[cc] = 0.0;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+          if (tid + 512 * q > j && c >= j) v[cc] += x[q] * a[q][c];
+      }
+      const double tot = wave_sum16(v);
+      if ((lane & 3) == 0) red[wave][16 * h + (lane >> 2)] = tot;
+    }
+    if (tid == j) {  // row j < 32: thread j, q = 0
+#pragma unroll
+      for (int c = 0; c < 32; ++c) row[c] = a[0][c];
+    }
+    __syncthreads();
+    if (tid >= j && tid < 32) {  // red[.][c] for c < j is stale: never read
+      double t = 0.0;
+      for (int w = 0; w < 8; ++w) t += red[w][tid];
+      ss[tid] = t;
+    }
+    __syncthreads();
+    double beta, scal, tj;
+    const bool act = reflector(row[j], ss[j], beta, scal, tj);
+    if (tid == 0) tau[j] = tj;
+    if (tid > j && tid < 32) wv[tid] = tj * (row[tid] + ss[tid] * scal);  // update coefficient of column tid
+    __syncthreads();
+    if (act) {
+      // 16 columns at a time: the memory clobber keeps the coefficient loads
+      // of the second half from being hoisted next to the first (registers)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        asm volatile("" ::: "memory");
+        if (16 * h + 15 < j) continue;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const int64_t i = tid + 512 * q;
+          if (i > j && i < mp) {
+            const double vi = x[q] * scal;
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) {
+              const int c = 16 * h + cc;
+              if (c == j) a[q][c] = vi;
+              else if (c > j) a[q][c] -= vi * wv[c];
+            }
+          } else if (i == j) {
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) {
+              const int c = 16 * h + cc;
+              if (c == j) a[q][c] = beta;
+              else if (c > j) a[q][c] -= wv[c];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();  // row, ss and wv are rewritten by the next step
+  }
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = tid + 512 * q;
+    if (i < mp) {
+#pragma unroll
+      for (int c = 0; c < 32; ++c)
+        if (c < jb) P[i + c * lda] = a[q][c];
+    }
+  }
+}
+
+// Tall panels: one launch per step j (j = -1 .. jb-1), the rows spread over
+// nblk blocks of 256 threads (RPT rows per thread, in registers).  Launch j
+// reduces the partial rows of products that launch j-1 left in part[j & 1]
+// (every block sums all blocks' partials in the same fixed order), applies
+// step j to its rows, and leaves the partials of step j+1 in
+// part[(j+1) & 1] and row j+1 of the panel in rowbuf[(j+1) & 1] (row j+1 <
+// 32 always lies in block 0; the copy keeps other blocks from reading it
+// while block 0 rewrites it).
+constexpr int kPanelThreads = 256;
+
+template <int RPT>
+__global__ __launch_bounds__(kPanelThreads) void k_panel_step(double* P, int64_t lda, int64_t mp, int jb, int j,
+                                                              double* tau, double* part, double* rowbuf, int nblk) {
+  __shared__ double red[8][32];
+  __shared__ double ss[32], row[32];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t i0 = (int64_t)blockIdx.x * (kPanelThreads * RPT);
+  const int c0 = j > 0 ? j : 0;  // columns before j are final
+  double a[RPT][32];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = i0 + tid + kPanelThreads * q;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) a[q][c] = (i < mp && c >= c0 && c < jb) ? P[i + c * lda] : 0.0;
+  }
+  if (j >= 0) {
+    {  // reduce part[j & 1]: 8 groups of 32 threads over the blocks, then the groups in order
+      const double* pj = part + (int64_t)(j & 1) * nblk * 32;
+      const int g = tid >> 5, c = tid & 31;
+      double t = 0.0;
+      for (int b = g; b < nblk; b += 8) t += pj[(int64_t)b * 32 + c];
+      red[g][c] = t;
+      if (tid < 32) row[tid] = rowbuf[(j & 1) * 32 + tid];
+    }
+    __syncthreads();
+    if (tid < 32) {
+      double t = 0.0;
+      for (int g = 0; g < 8; ++g) t += red[g][tid];
+      ss[tid] = t;
+    }
+    __syncthreads();
+    double beta, scal, tj;
+    const bool act = reflector(row[j], ss[j], beta, scal, tj);
+    if (blockIdx.x == 0 && tid == 0) tau[j] = tj;
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int64_t i = i0 + tid + kPanelThreads * q;
+        if (i > j && i < mp) {
+          double vi = 0.0;
+#pragma unroll
+          for (int c = 0; c < 32; ++c)
+            if (c == j) vi = a[q][c] * scal;
+#pragma unroll
+          for (int c = 0; c < 32; ++c) {
+            if (c == j) a[q][c] = vi;
+            else if (c > j) a[q][c] -= vi * (tj * (row[c] + ss[c] * scal));
+          }
+        } else if (i == j) {
+#pragma unroll
+          for (int c = 0; c < 32; ++c) {
+            if (c == j) a[q][c] = beta;
+            else if (c > j) a[q][c] -= tj * (row[c] + ss[c] * scal);
+          }
+        }
+        if (i < mp) {
+#pragma unroll
+          for (int c = 0; c < 32; ++c)
+            if (c >= j && c < jb) P[i + c * lda] = a[q][c];
+        }
+      }
+    }
+  }
+  const int jn = j + 1;
+  if (jn < jb) {
+    double v[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) v[c] = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int64_t i = i0 + tid + kPanelThreads * q;
+      double x = 0.0;
+#pragma unroll
+      for (int c = 0; c < 32; ++c)
+        if (c == jn) x = a[q][c];
+      if (i > jn && i < mp) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+          if (c >= jn) v[c] += x * a[q][c];
+      }
+    }
+    const double tot = wave_sum32(v);
+    if ((lane & 1) == 0) red[wave][lane >> 1] = tot;
+    if (blockIdx.x == 0 && tid == jn) {  // row jn: block 0, thread jn, q = 0
+#pragma unroll
+      for (int c = 0; c < 32; ++c) rowbuf[(jn & 1) * 32 + c] = a[0][c];
+    }
+    __syncthreads();
+    if (tid < 32) {
+      double t = 0.0;
+      for (int w = 0; w < kPanelThreads / 64; ++w) t += red[w][tid];
+      part[(int64_t)(jn & 1) * nblk * 32 + (int64_t)blockIdx.x * 32 + tid] = t;
+    }
+  }
+}
+
+unsigned grid_for_elems(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, 256), 65536)); }
+
+// Panel of jb <= 32 columns at P (rows 0..mp-1): reflectors below the
+// diagonal, R on and above it, tau[0..jb).
+constexpr int64_t kPanelSmallRows = 1024;
+int64_t panel_rpt(int64_t mp) { return mp > 32768 ? 2 : 1; }
+int64_t panel_blocks(int64_t mp) { return cdiv(mp, kPanelThreads * panel_rpt(mp)); }
+// Most blocks any panel of a matrix with m rows takes (panels have m - j0
+// rows; below 32769 rows a panel takes one row per thread).
+int64_t panel_blocks_max(int64_t m) { return std::max(panel_blocks(m), std::min<int64_t>(cdiv(m, kPanelThreads), 128)); }
+
+void factor_panel(hipStream_t st, double* P, int64_t lda, int64_t mp, int64_t jb, double* tau, double* part,
+                  double* rowbuf) {
+  if (mp <= kPanelSmallRows) {
+    hipLaunchKernelGGL(k_panel_small, dim3(1), dim3(512), 0, st, P, lda, mp, (int)jb, tau);
+    check_launch("k_panel_small");
+    return;
+  }
+  const int nblk = (int)panel_blocks(mp);
+  for (int j = -1; j < (int)jb; ++j) {
+    if (panel_rpt(mp) == 2)
+      hipLaunchKernelGGL(k_panel_step<2>, dim3(nblk), dim3(kPanelThreads), 0, st, P, lda, mp, (int)jb, j, tau, part,
+                         rowbuf, nblk);
+    else
+      hipLaunchKernelGGL(k_panel_step<1>, dim3(nblk), dim3(kPanelThreads), 0, st, P, lda, mp, (int)jb, j, tau, part,
+                         rowbuf, nblk);
+  }
+  check_launch("k_panel_step");
+}
+
+// Workspace of the blocked QR: V (m x nb), G (nb x nb), the T factor of
+// every panel (Tall, ceil(n / nb) x nb x nb, so the Q formation can reuse
+// them), two nb x n products, and the panel step's partial rows and row copy.
+struct QrWork {
+  double *V, *G, *Tall, *W1, *W2, *part, *rowbuf;
+};
+QrWork qr_work(DevBuf* work, int64_t m, int64_t n) {
+  const int64_t nb = kQrPanel, np = cdiv(n, nb);
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o += align_up(bytes, 256);
+    return at;
+  };
+  const size_t oV = take((size_t)m * nb * sizeof(double));
+  const size_t oG = take((size_t)nb * nb * sizeof(double));
+  const size_t oT = take((size_t)np * nb * nb * sizeof(double));
+  const size_t o1 = take((size_t)nb * n * sizeof(double));
+  const size_t o2 = take((size_t)nb * n * sizeof(double));
+  const size_t oP = take((size_t)2 * panel_blocks_max(m) * 32 * sizeof(double));
+  const size_t oR = take((size_t)2 * 32 * sizeof(double));
+  work->reserve(o);
+  char* b = work->as<char>();
+  auto at = [&](size_t off) { return reinterpret_cast<double*>(b + off); };
+  return QrWork{at(oV), at(oG), at(oT), at(o1), at(o2), at(oP), at(oR)};
+}
+
+// V of the panel at (j0, j0) with jb columns (rows j0 .. m-1).
+void panel_v(hipStream_t st, const double* A, int64_t lda, int64_t m, int64_t j0, int64_t jb, const QrWork& w) {
   const int64_t mp = m - j0;
   hipLaunchKernelGGL(k_build_v, dim3(grid_for_elems(mp * jb)), dim3(256), 0, st, A + j0 + j0 * lda, lda, mp, jb, w.V);
   check_launch("k_build_v");
+}
+
+// T of the panel (from G = V^T V) into its slot of Tall.
+double* panel_t(hipStream_t st, int64_t m, int64_t j0, int64_t jb, const double* tau, const QrWork& w,
+                GemmWorkspace* gws) {
+  const int64_t mp = m - j0;
+  double* T = w.Tall + (j0 / kQrPanel) * kQrPanel * kQrPanel;
   gemm_f64(st, 'T', 'N', jb, jb, mp, 1.0, w.V, mp, w.V, mp, 0.0, w.G, jb, gws);
-  hipLaunchKernelGGL(k_build_t, dim3(1), dim3(64), 0, st, w.G, jb, tau + j0, w.T);
+  hipLaunchKernelGGL(k_build_t, dim3(1), dim3(64), 0, st, w.G, jb, tau + j0, T);
   check_launch("k_build_t");
+  return T;
 }
 
 }  // namespace
@@ -318,7 +766,7 @@ void householder_qr(hipStream_t st, int64_t m, int64_t n, double* A, int64_t lda
   SKEMA_REQUIRE(m >= n && n >= 0, kErrShape, "householder_qr: needs rows >= cols");
   SKEMA_REQUIRE(lda >= m, kErrShape, "householder_qr: lda < rows");
   if (n == 0) return;
-  if (!work || n <= kQrPanel) {
+  if (!work) {
     hipLaunchKernelGGL(k_qr_first, dim3(1), dim3(kQrThreads), 0, st, A, m, tau);
     check_launch("k_qr_first");
     for (int64_t j = 0; j + 1 < n; ++j) {
@@ -330,27 +778,25 @@ void householder_qr(hipStream_t st, int64_t m, int64_t n, double* A, int64_t lda
   const QrWork w = qr_work(work, m, n);
   for (int64_t j0 = 0; j0 < n; j0 += kQrPanel) {
     const int64_t jb = std::min(kQrPanel, n - j0), mp = m - j0;
-    // the panel, column by column (dgeqr2 on rows j0..m-1 of columns j0..j0+jb-1)
-    hipLaunchKernelGGL(k_qr_first, dim3(1), dim3(kQrThreads), 0, st, A + j0 + j0 * lda, mp, tau + j0);
-    for (int64_t j = j0; j + 1 < j0 + jb; ++j)
-      hipLaunchKernelGGL(k_qr_step, dim3((unsigned)(j0 + jb - j - 1)), dim3(kQrThreads), 0, st, A, lda, m, n, j, tau);
-    check_launch("k_qr_step");
+    factor_panel(st, A + j0 + j0 * lda, lda, mp, jb, tau + j0, w.part, w.rowbuf);
+    // T of every panel (the Q formation reuses them), then the trailing
+    // columns: A2 <- (I - V T V^T)^T A2 = A2 - V (T^T (V^T A2))  (dlarfb 'L','T','F','C')
+    panel_v(st, A, lda, m, j0, jb, w);
+    const double* T = panel_t(st, m, j0, jb, tau, w, gws);
     if (j0 + jb >= n) break;
-    // trailing columns: A2 <- (I - V T V^T)^T A2 = A2 - V (T^T (V^T A2))  (dlarfb 'L','T','F','C')
     const int64_t n2 = n - j0 - jb;
     double* A2 = A + j0 + (j0 + jb) * lda;
-    panel_vt(st, A, lda, m, j0, jb, tau, w, gws);
     gemm_f64(st, 'T', 'N', jb, n2, mp, 1.0, w.V, mp, A2, lda, 0.0, w.W1, jb, gws);
-    gemm_f64(st, 'T', 'N', jb, n2, jb, 1.0, w.T, jb, w.W1, jb, 0.0, w.W2, jb, gws);
+    gemm_f64(st, 'T', 'N', jb, n2, jb, 1.0, T, jb, w.W1, jb, 0.0, w.W2, jb, gws);
     gemm_f64(st, 'N', 'N', mp, n2, jb, -1.0, w.V, mp, w.W2, jb, 1.0, A2, lda, gws);
   }
 }
 
 void householder_q(hipStream_t st, int64_t m, int64_t n, const double* A, int64_t lda, const double* tau,
-                   double* Q, int64_t ldq, DevBuf* work, GemmWorkspace* gws) {
+                   double* Q, int64_t ldq, DevBuf* work, GemmWorkspace* gws, bool reuse_t) {
   SKEMA_REQUIRE(m >= n && n >= 0 && lda >= m && ldq >= m, kErrShape, "householder_q: bad shape");
   if (n == 0) return;
-  if (!work || n <= kQrPanel) {
+  if (!work) {
     for (int64_t j = n - 1; j >= 0; --j) {
       hipLaunchKernelGGL(k_org_step, dim3((unsigned)(n - j)), dim3(kQrThreads), 0, st, A, lda, tau, Q, ldq, m, j);
     }
@@ -365,9 +811,10 @@ void householder_q(hipStream_t st, int64_t m, int64_t n, const double* A, int64_
   for (int64_t j0 = ((n - 1) / kQrPanel) * kQrPanel; j0 >= 0; j0 -= kQrPanel) {
     const int64_t jb = std::min(kQrPanel, n - j0), mp = m - j0, nq = n - j0;
     double* Q2 = Q + j0 + j0 * ldq;
-    panel_vt(st, A, lda, m, j0, jb, tau, w, gws);
+    panel_v(st, A, lda, m, j0, jb, w);
+    const double* T = reuse_t ? w.Tall + (j0 / kQrPanel) * kQrPanel * kQrPanel : panel_t(st, m, j0, jb, tau, w, gws);
     gemm_f64(st, 'T', 'N', jb, nq, mp, 1.0, w.V, mp, Q2, ldq, 0.0, w.W1, jb, gws);
-    gemm_f64(st, 'N', 'N', jb, nq, jb, 1.0, w.T, jb, w.W1, jb, 0.0, w.W2, jb, gws);
+    gemm_f64(st, 'N', 'N', jb, nq, jb, 1.0, T, jb, w.W1, jb, 0.0, w.W2, jb, gws);
     gemm_f64(st, 'N', 'N', mp, nq, jb, -1.0, w.V, mp, w.W2, jb, 1.0, Q2, ldq, gws);
   }
 }
@@ -396,8 +843,13 @@ int jacobi_svd(hipStream_t st, int64_t n, double* A, int64_t lda, double* U, int
                int64_t ldv, DevBuf* scratch) {
   if (n == 0) return 0;
   SKEMA_REQUIRE(n < (1 << 20), kErrUnsupported, "jacobi_svd: order too large");
-  // Round-robin (circle method) schedule: n' - 1 rounds of disjoint pairs.
-  const int np = (int)(n + (n & 1));
+  // Block-pair rounds in registers for orders up to 512 (one launch per
+  // round of block pairs, ~n/16 per sweep), else scalar rounds (~n).
+  const bool blocked = n > kJacobiBlock && n <= 512;
+  const int64_t units = blocked ? cdiv(n, kJacobiBlock) : n;
+  // Round-robin (circle method) schedule over the units: units' - 1 rounds
+  // of disjoint pairs.
+  const int np = (int)(units + (units & 1));
   std::vector<int> arr(np);
   std::iota(arr.begin(), arr.end(), 0);
   std::vector<int2> pairs;
@@ -407,7 +859,8 @@ int jacobi_svd(hipStream_t st, int64_t n, double* A, int64_t lda, double* U, int
     for (int i = 0; i < np / 2; ++i) {
       int p = arr[i], q = arr[np - 1 - i];
       if (p > q) std::swap(p, q);
-      if (q < n) pairs.push_back(make_int2(p, q));
+      if (blocked) pairs.push_back(make_int2(p, q));  // a block beyond the last holds only padding
+      else if (q < n) pairs.push_back(make_int2(p, q));
     }
     std::rotate(arr.begin() + 1, arr.end() - 1, arr.end());
   }
@@ -440,10 +893,14 @@ int jacobi_svd(hipStream_t st, int64_t n, double* A, int64_t lda, double* U, int
     for (int r = 0; r + 1 < (int)round_start.size(); ++r) {
       const int cnt = round_start[r + 1] - round_start[r];
       if (cnt == 0) continue;
-      hipLaunchKernelGGL(k_jacobi_round, dim3(cnt), dim3(256), 0, st, A, lda, Vw, n, n, d_pairs + round_start[r],
-                         tol, d_cnt);
+      if (blocked)
+        hipLaunchKernelGGL(k_jacobi_rows, dim3(cnt), dim3(512), 0, st, A, lda, Vw, n, n, d_pairs + round_start[r],
+                           r == 0 ? 1 : 0, tol, d_cnt);
+      else
+        hipLaunchKernelGGL(k_jacobi_round, dim3(cnt), dim3(256), 0, st, A, lda, Vw, n, n, d_pairs + round_start[r],
+                           tol, d_cnt);
     }
-    check_launch("k_jacobi_round");
+    check_launch(blocked ? "k_jacobi_rows" : "k_jacobi_round");
     int h_cnt = 0;
     SKEMA_HIP(hipMemcpyAsync(&h_cnt, d_cnt, sizeof(int), hipMemcpyDeviceToHost, st));
     SKEMA_HIP(hipStreamSynchronize(st));

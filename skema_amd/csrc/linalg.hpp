@@ -12,19 +12,22 @@ namespace skema {
 // -sign(alpha)*||x||, tau = (beta-alpha)/beta, v(0) = 1).  On return A (m x n,
 // m >= n, leading dimension lda) holds R on and above the diagonal and the
 // reflectors below it; tau has n entries.  With a workspace, blocked like
-// dgeqrf: panels of 32 columns are factored column by column and their
-// compact WY form I - V T V^T is applied to the trailing columns with FP64
-// MFMA GEMMs (same reflectors; results equal to rounding).
+// dgeqrf: panels of 32 columns are factored column by column (one launch
+// for a panel of at most 1024 rows, one launch per column with the rows
+// spread over the CUs otherwise) and their compact WY form I - V T V^T is
+// applied to the trailing columns with FP64 MFMA GEMMs (same reflectors;
+// results equal to rounding).  The T factors stay in the workspace.
 void householder_qr(hipStream_t st, int64_t m, int64_t n, double* A, int64_t lda, double* tau,
                     DevBuf* work = nullptr, GemmWorkspace* gws = nullptr);
 
 // Explicit thin Q (m x n) from the reflectors left in A by householder_qr
 // (dorg2r semantics; with a workspace blocked like dorgqr, each panel's
 // I - V T V^T applied to [I; 0] by GEMMs, last panel first).  Q must not
-// alias A.
+// alias A.  reuse_t: take the T factors householder_qr left in `work` (A was
+// the last matrix factored with this workspace) instead of forming them.
 void householder_q(hipStream_t st, int64_t m, int64_t n, const double* A, int64_t lda,
                    const double* tau, double* Q, int64_t ldq, DevBuf* work = nullptr,
-                   GemmWorkspace* gws = nullptr);
+                   GemmWorkspace* gws = nullptr, bool reuse_t = false);
 
 // R (n x n, upper, zero below) from the factored A.
 void extract_r(hipStream_t st, int64_t n, const double* A, int64_t lda, double* R, int64_t ldr);

@@ -36,6 +36,7 @@
 #include <cmath>
 #include <type_traits>
 
+#include "dma.hpp"
 #include "lowp.hpp"
 #include "util.hpp"
 
@@ -305,6 +306,14 @@ __global__ void k_convert_map(const double* src, int64_t ld, int64_t rows, int64
     hi[r * ldo + c] = h;
     if (lo) lo[r * ldo + c] = (T)(x - (float)h);
   }
+  // the padding columns [cols, ldo) are zero: kernels that copy whole
+  // 16-byte chunks past the end of a row multiply them by zero
+  const int64_t pad = ldo - cols, ptotal = rows * pad;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ptotal; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / pad, c = cols + e % pad;
+    hi[r * ldo + c] = (T)0.f;
+    if (lo) lo[r * ldo + c] = (T)0.f;
+  }
 }
 // Map row r -> e4m3 row (row-major, ld ldo) scaled by 2^-e(max_r), with
 // max_r written to amax[r].  One block per row.
@@ -332,6 +341,7 @@ __global__ __launch_bounds__(kThreads) void k_convert_map_fp8(const double* src,
     if (lim == 4) *reinterpret_cast<uint32_t*>(q + r * ldo + c0) = w;
     else for (int t = 0; t < lim; ++t) q[r * ldo + c0 + t] = (uint8_t)(w >> (8 * t));
   }
+  for (int64_t c = cols + tid; c < ldo; c += kThreads) q[r * ldo + c] = 0;  // zero padding
   if (tid == 0) amax[r] = mx;
 }
 
@@ -1003,23 +1013,6 @@ __global__ __launch_bounds__(kThreads) void k_ntx(NtArgs g) {
 // would drain every load in flight.
 // Fragment K assignment and block scales as in k_ntx.  Requires 16-byte
 // aligned rows and every split's K range a multiple of 128 bytes.
-#define SKEMA_LDS __attribute__((address_space(3)))
-
-// s_waitcnt vmcnt(N) with the other counters left alone (gfx9 encoding:
-// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
-template <int N>
-__device__ __forceinline__ void wait_vm() {
-  static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
-  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-// Barrier that orders this wave's LDS accesses around it in the compiler's
-// view (memory clobber) without the fence of __syncthreads.
-__device__ __forceinline__ void lds_barrier() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
 template <int FORM, int WM, int WN, int TM, int TN, int NS, bool ASC, bool BSC>
 __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   constexpr int NW = WM * WN;  // waves: WM along M by WN along N

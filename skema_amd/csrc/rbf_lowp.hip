@@ -13,6 +13,7 @@
 // registers/LDS.
 #include <cmath>
 
+#include "dma.hpp"
 #include "lowp.hpp"
 #include "rbf_fused.hpp"
 #include "rbf_lowp.hpp"
@@ -250,31 +251,58 @@ __global__ __launch_bounds__(kThreadsR) void k_rbf_right_bf16(RbfLpArgs g) {
 // ---- right, tall tile ------------------------------------------------------------
 // 512 threads own 256 window rows x 208 columns (13 column tiles) of
 // [y | G]; k + s columns take ceil((k+s)/208) column blocks, each forming its
-// own Gram tiles.  Per 32-point step the L2 traffic of a block is the
-// [Omega; Psi] slice (208 x 32) and the points (32 x DP) for 2 x 256 x 32 x
-// (208 + DP) MFMA flops: a quarter of a 64-row tile's bytes per flop, at
-// the price of the Gram formed once per column block.  Wave w owns rows
-// 32w .. 32w+31: it forms their Gram tiles with the 32 points of the step
-// (window-row features in registers, the points' from LDS), exponentiates
-// and writes K (bf16) to LDS, and runs the sketch MFMAs of its 2 x 13 tiles
-// with the [Omega; Psi] slice staged in LDS and shared by all waves.
-// Pipelined one step apart (sketch of t, Gram + exp of t+1, [Omega; Psi] of
-// t+1 and points of t+2 staged, [Omega; Psi] of t+2 and points of t+3
-// loaded), double-buffered LDS, one barrier per step.
-constexpr int kTallRows = 256, kTallCols = 208;
+// own Gram tiles.  Wave w owns rows 32w .. 32w+31 and all 208 columns.
+//
+// K never leaves registers: per 32-point step the wave forms the transposed
+// Gram tiles (points x its window rows) with the points as the A operand,
+// read from LDS in the order pi(4a + t) = 8a + 4h + t for sub-tile h, so
+// the f32 result (lane l: window row l%16, points pi(4 (l/16) + v)) holds
+// exactly points 8 (l/16) .. +7 of row l%16 after the two sub-tiles: the
+// A-operand layout of the sketch MFMA, in natural point order.  The
+// [Omega; Psi] slice (208 x 32 points, 64-byte rows), the step's points
+// (32 x DP) and their norms stream global -> LDS through global_load_lds
+// into a kRingSlots-slot ring, so four steps of loads are in flight while
+// one is consumed; one barrier per step.  16-byte chunks are XOR-swizzled
+// within rows so the 16 rows a lane group reads at one chunk hit distinct
+// banks.
+// kTallXT 16-row tiles per wave: 4 (one wave per SIMD, accumulators in
+// AGPRs) halves the LDS reads of the [Omega; Psi] operands per MFMA against 2.
+constexpr int kTallRows = 256, kTallCols = 208, kRingSlots = 6, kTallXT = 4;
+
+// chunk-position swizzle of a points row with CPR 16-byte chunks, for the
+// rows 8a + 4h + t (a, t < 4) of either sub-tile
+template <int CPR>
+__device__ __forceinline__ int xs_swz(int r) {
+  if constexpr (CPR == 16) return ((r >> 1) & 12) | (r & 3);
+  else if constexpr (CPR == 8) return ((r >> 2) & 6) | ((r >> 1) & 1);
+  else return (r >> 3) & 3;
+}
+// 64-byte rows of the [Omega; Psi] slice, read 16 consecutive rows at a time
+__device__ __forceinline__ int os_swz(int r) { return (r >> 2) & 3; }
+
+// exp(-gamma max(D, 0)) from u = -gamma log2(e) D = kappa g + (alpha_i + beta_j)
+// with alpha = -gamma log2(e) |x_i|^2 (likewise beta) and kappa = 2 gamma
+// log2(e): u <= 0 exactly when D >= 0.
+__device__ __forceinline__ float rbf_exp2(float g, float ab, float kappa) {
+  const float u = fmaf(g, kappa, ab);
+  return __builtin_amdgcn_exp2f(u < 0.f ? u : 0.f);
+}
 
 template <int DP>
-__global__ __launch_bounds__(512, 1) void k_rbf_right_bf16_tall(RbfLpArgs g) {
-  constexpr int BM = kTallRows, BN = kTallCols, BK = 32, TNC = BN / 16, PS = DP + 8, KS = BK + 8;
-  constexpr int PCH = BK * DP / 8;  // 16-byte chunks of a step's points
-  constexpr int OCH = BN * BK / 8;  // 16-byte chunks of a step's [Omega; Psi] slice
-  constexpr int OR = (OCH + 511) / 512;
-  static_assert(PCH <= 512, "one points chunk per thread");
-  __shared__ __attribute__((aligned(16))) __bf16 Xs[2][BK][PS];
-  __shared__ __attribute__((aligned(16))) __bf16 Ks[2][BM][KS];
-  __shared__ __attribute__((aligned(16))) __bf16 Os[2][BN][KS];
-  __shared__ float NJ[4][BK];
-  __shared__ float NI[BM];
+__global__ __launch_bounds__(64 * (kTallRows / (16 * kTallXT)), 1) void k_rbf_right_bf16_tall(RbfLpArgs g) {
+  constexpr int XT = kTallXT;                        // 16-row tiles per wave
+  constexpr int NW = kTallRows / (16 * XT);          // waves
+  constexpr int BM = kTallRows, BN = kTallCols, BK = 32, TNC = BN / 16, NS = kRingSlots;
+  constexpr int CPR = DP / 8;                       // 16-byte chunks per points row
+  constexpr int RPI = 1024 / (2 * DP);              // points rows per 1 KB copy
+  constexpr int OS_BYTES = BN * BK * 2;             // [Omega; Psi] slice
+  constexpr int XS_BYTES = BK * DP * 2;             // points
+  constexpr int NJ_OFF = OS_BYTES + XS_BYTES;       // norms: one 256-byte dword copy
+  constexpr int SLOT = (NJ_OFF + 256 + 1023) / 1024 * 1024;
+  constexpr int I_OS = OS_BYTES / 1024, I_XS = XS_BYTES / 1024, ITEMS = I_OS + I_XS + 1;
+  constexpr int IPW = (ITEMS + NW - 1) / NW;        // copy items per wave, at most
+  static_assert(OS_BYTES % 1024 == 0 && XS_BYTES % 1024 == 0 && (NS - 2) * IPW <= 62, "copy items");
+  __shared__ __attribute__((aligned(1024))) uint8_t ring[NS * SLOT];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, lq = lane >> 4;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
@@ -283,115 +311,145 @@ __global__ __launch_bounds__(512, 1) void k_rbf_right_bf16_tall(RbfLpArgs g) {
   const int64_t c0 = (int64_t)blockIdx.y * BN;
   const int64_t kbeg = (int64_t)split * g.k_per_split;
   const int64_t kend = (g.K < kbeg + g.k_per_split) ? g.K : kbeg + g.k_per_split;
-  const int64_t steps = (kend - kbeg + BK - 1) / BK;
+  const int steps = (int)((kend - kbeg + BK - 1) / BK);
+  const int dw = (ITEMS - wave + NW - 1) / NW;  // copies this wave issues per step
+  const float lg = 1.4426950408889634f * g.gamma;  // gamma log2(e)
+  const float kappa = 2.f * lg;
+  const int64_t wr0 = m0 + 16 * XT * wave;        // the wave's first row
+  const bool rows_full = wr0 + 16 * XT <= g.M;
 
-  // Gram A operands: rows m0 + 32 wave + 16 x + l16, features 32 c + 8 lq ..
-  bf16x8 xa[2][DP / 32];
+  // window rows of the wave: features (Gram B operand) and scaled norms
+  bf16x8 xa[XT][DP / 32];
+  float al[XT];
 #pragma unroll
-  for (int x = 0; x < 2; ++x) {
-    const int64_t i = m0 + 32 * wave + 16 * x + l16;
+  for (int x = 0; x < XT; ++x) {
+    const int64_t i = wr0 + 16 * x + l16;
 #pragma unroll
     for (int c = 0; c < DP / 32; ++c)
       xa[x][c] = (i < g.M) ? load8(g.P + (g.row0 + i) * g.ldpb, 32 * c + 8 * lq, g.ldpb) : bf16x8{};
+    al[x] = (i < g.M) ? -lg * g.nrm[g.row0 + i] : 0.f;
   }
-  if (tid < BM) NI[tid] = (m0 + tid < g.M) ? g.nrm[g.row0 + m0 + tid] : 0.f;
 
-  bf16x8 xr = {};
-  float njr = 0.f;
-  auto load_points = [&](int64_t u) {
-    const int64_t j0 = kbeg + u * BK;
-    if (tid < PCH) {
-      const int jj = tid / (DP / 8), fc = tid % (DP / 8);
-      const int64_t j = j0 + jj;
-      xr = (j < kend) ? load8(g.P + j * g.ldpb, 8 * fc, g.ldpb) : bf16x8{};
-    }
-    njr = (tid < BK && j0 + tid < kend) ? g.nrm[j0 + tid] : 0.f;
-  };
-  auto store_points = [&](int b, int64_t u) {
-    if (tid < PCH) *reinterpret_cast<bf16x8*>(&Xs[b][tid / (DP / 8)][8 * (tid % (DP / 8))]) = xr;
-    if (tid < BK) NJ[u & 3][tid] = njr;
-  };
-  bf16x8 orr[OR];
-  auto load_op = [&](int64_t u) {
-    const int64_t j0 = kbeg + u * BK;
+  auto issue = [&](int u) {
+    uint8_t* slot = ring + (u % NS) * SLOT;
+    const int64_t j0 = kbeg + (int64_t)u * BK;
 #pragma unroll
-    for (int q = 0; q < OR; ++q) {
-      const int ch = tid + 512 * q;
-      const int cc = ch / (BK / 8), pc = ch % (BK / 8);
-      const int64_t col = c0 + cc, j = j0 + 8 * pc;
-      orr[q] = (ch < OCH && col < g.N && j < kend) ? load8(g.OP + col * g.ldop, j, kend) : bf16x8{};
-    }
-  };
-  auto store_op = [&](int b) {
-#pragma unroll
-    for (int q = 0; q < OR; ++q) {
-      const int ch = tid + 512 * q;
-      if (ch < OCH) *reinterpret_cast<bf16x8*>(&Os[b][ch / (BK / 8)][8 * (ch % (BK / 8))]) = orr[q];
+    for (int s = 0; s < IPW; ++s) {
+      const int it = wave + NW * s;
+      if (it < I_OS) {  // 16 rows of the [Omega; Psi] slice
+        const int r = 16 * it + (lane >> 2), p = lane & 3;
+        int64_t col = c0 + r;
+        col = col < g.N ? col : g.N - 1;
+        const __bf16* src = g.OP + col * g.ldop + j0 + 8 * (p ^ os_swz(r));
+        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + 1024 * it), 16, 0, 0);
+      } else if (it < I_OS + I_XS) {  // RPI points rows
+        const int x = it - I_OS;
+        const int r = RPI * x + lane / CPR, p = lane % CPR;
+        int64_t j = j0 + r;
+        j = j < kend ? j : kend - 1;
+        const __bf16* src = g.P + j * g.ldpb + 8 * (p ^ xs_swz<CPR>(r));
+        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + OS_BYTES + 1024 * x), 16, 0,
+                                         0);
+      } else if (it == I_OS + I_XS) {  // norms of the step's points (lanes >= 32 duplicate, unused)
+        int64_t j = j0 + (lane & 31);
+        j = j < kend ? j : kend - 1;
+        __builtin_amdgcn_global_load_lds((const void*)(g.nrm + j), (SKEMA_LDS void*)(slot + NJ_OFF), 4, 0, 0);
+      }
     }
   };
-  auto gram_exp = [&](int b, int64_t u) {  // Xs[b] (step u) -> Ks[b]
+
+  // K of step u for the wave's row tiles, in registers (A operands)
+  auto gram_exp = [&](int u, bf16x8 (&kv)[XT]) {
+    const uint8_t* slot = ring + (u % NS) * SLOT;
+    const uint8_t* xs = slot + OS_BYTES;
+    const float* nj = reinterpret_cast<const float*>(slot + NJ_OFF);
+    const int64_t j0 = kbeg + (int64_t)u * BK;
+    f4v gr[2][XT];
 #pragma unroll
-    for (int pt = 0; pt < 2; ++pt) {
-      f4v gr[2] = {f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f}};
+    for (int h = 0; h < 2; ++h) {
+      const int r = 8 * (l16 >> 2) + 4 * h + (l16 & 3);  // points row read by this lane
+#pragma unroll
+      for (int x = 0; x < XT; ++x) gr[h][x] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int c = 0; c < DP / 32; ++c) {
-        const bf16x8 bx = *reinterpret_cast<const bf16x8*>(&Xs[b][16 * pt + l16][32 * c + 8 * lq]);
+        const int pos = (4 * c + lq) ^ xs_swz<CPR>(r);
+        const bf16x8 bx = *reinterpret_cast<const bf16x8*>(xs + r * (2 * DP) + 16 * pos);
 #pragma unroll
-        for (int x = 0; x < 2; ++x) gr[x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa[x][c], bx, gr[x], 0, 0, 0);
+        for (int x = 0; x < XT; ++x) gr[h][x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx, xa[x][c], gr[h][x], 0, 0, 0);
       }
-      const int jl = 16 * pt + l16;
-      const int64_t j = kbeg + u * BK + jl;
-      const float nj = NJ[u & 3][jl];
+    }
+    f4v be[2];
+    be[0] = *reinterpret_cast<const f4v*>(nj + 8 * lq);
+    be[1] = *reinterpret_cast<const f4v*>(nj + 8 * lq + 4);
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int il = 32 * wave + 16 * x + 4 * lq + v;
-          const int64_t i = m0 + il;
-          float kv = 0.f;
-          if (i < g.M && j < kend) kv = rbf_value_f(NI[il], nj, gr[x][v], g.gamma, g.row0 + i == j);
-          Ks[b][il][jl] = (__bf16)kv;
-        }
+      for (int v = 0; v < 4; ++v) be[h][v] *= -lg;
+    // interior steps (every row and point in range, the window's diagonal
+    // not among them) skip the per-element checks
+    const int64_t gi0 = g.row0 + wr0;
+    const bool interior = rows_full && j0 + BK <= kend && (j0 + BK <= gi0 || j0 >= gi0 + 16 * XT);
+    if (interior) {
+#pragma unroll
+      for (int x = 0; x < XT; ++x)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) kv[x][4 * h + v] = (__bf16)rbf_exp2(gr[h][x][v], al[x] + be[h][v], kappa);
+    } else {
+#pragma unroll
+      for (int x = 0; x < XT; ++x) {
+        const int64_t i = wr0 + 16 * x + l16;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int64_t j = j0 + 8 * lq + 4 * h + v;
+            float e = 0.f;
+            if (i < g.M && j < kend) e = (g.row0 + i == j) ? 1.f : rbf_exp2(gr[h][x][v], al[x] + be[h][v], kappa);
+            kv[x][4 * h + v] = (__bf16)e;
+          }
+      }
     }
   };
 
-  f4v acc[2][TNC];
+  f4v acc[XT][TNC];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < XT; ++x)
 #pragma unroll
     for (int y = 0; y < TNC; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
 
-  load_points(0); store_points(0, 0);
-  load_points(1); store_points(1, 1);
-  load_op(0); store_op(0);
-  __syncthreads();
-  gram_exp(0, 0);
-  load_points(2);
-  load_op(1);
-  __syncthreads();
-  for (int64_t u = 0; u < steps; ++u) {
-    const int b = (int)(u & 1), nb = b ^ 1;
+  bf16x8 kv[XT];
 #pragma unroll
-    for (int x = 0; x < 2; ++x) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&Ks[b][32 * wave + 16 * x + l16][8 * lq]);
-#pragma unroll
-      for (int y = 0; y < TNC; ++y) {
-        const bf16x8 o = *reinterpret_cast<const bf16x8*>(&Os[b][16 * y + l16][8 * lq]);
-        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, o, acc[x][y], 0, 0, 0);
-      }
-    }
-    if (u + 1 < steps) gram_exp(nb, u + 1);  // points of step u+1 sit in Xs[nb]
-    store_op(nb);                            // [Omega; Psi] of step u+1
-    store_points(b, u + 2);                  // points of step u+2 -> Xs[b]
-    load_op(u + 2);
-    load_points(u + 3);
-    __syncthreads();
+  for (int p = 0; p < NS - 1; ++p)
+    if (p < steps) issue(p);
+  if (steps > 0) {
+    wait_vm_upto(((NS - 2) < (steps - 1) ? (NS - 2) : (steps - 1)) * dw);  // step 0 complete
+    lds_barrier();
+    gram_exp(0, kv);
   }
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int u = 0; u < steps; ++u) {
+    if (u + 1 < steps) {  // step u + 1 complete: at most steps u+2 .. min(u+NS-2, steps-1) in flight
+      const int last = (u + NS - 2 < steps - 1) ? u + NS - 2 : steps - 1;
+      wait_vm_upto((last - (u + 1)) * dw);
+    }
+    lds_barrier();  // every wave's copies of step u+1 landed; slot (u-1) % NS is free
+    if (u + NS - 1 < steps) issue(u + NS - 1);
+    const uint8_t* os = ring + (u % NS) * SLOT;
 #pragma unroll
     for (int y = 0; y < TNC; ++y) {
-      const int64_t i0 = m0 + 32 * wave + 16 * x + 4 * lq;
+      const int r = 16 * y + l16;
+      const bf16x8 o = *reinterpret_cast<const bf16x8*>(os + 64 * r + 16 * (lq ^ os_swz(r)));
+#pragma unroll
+      for (int x = 0; x < XT; ++x) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(kv[x], o, acc[x][y], 0, 0, 0);
+    }
+    if (u + 1 < steps) gram_exp(u + 1, kv);
+  }
+#pragma unroll
+  for (int x = 0; x < XT; ++x)
+#pragma unroll
+    for (int y = 0; y < TNC; ++y) {
+      const int64_t i0 = wr0 + 16 * x + 4 * lq;
       const int64_t j = c0 + 16 * y + l16;
       if (i0 >= g.M || j >= g.N) continue;
       if (g.part) {
@@ -401,16 +459,191 @@ __global__ __launch_bounds__(512, 1) void k_rbf_right_bf16_tall(RbfLpArgs g) {
           if (i0 + v < g.M) p[i0 + v] = acc[x][y][v];
       } else if (i0 + 4 <= g.M && j >= g.out.nsplit) {
         __bf16* pg = g.G + i0 + (j - g.out.nsplit) * g.ldg;
-        const bf16x4 h = bf16x4{(__bf16)acc[x][y][0], (__bf16)acc[x][y][1], (__bf16)acc[x][y][2],
-                                (__bf16)acc[x][y][3]};
-        if ((((uintptr_t)pg) & 7) == 0) *reinterpret_cast<bf16x4*>(pg) = h;
-        else for (int v = 0; v < 4; ++v) pg[v] = h[v];
+        const bf16x4 hh = bf16x4{(__bf16)acc[x][y][0], (__bf16)acc[x][y][1], (__bf16)acc[x][y][2],
+                                 (__bf16)acc[x][y][3]};
+        if ((((uintptr_t)pg) & 7) == 0) *reinterpret_cast<bf16x4*>(pg) = hh;
+        else for (int v = 0; v < 4; ++v) pg[v] = hh[v];
       } else {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
           if (i0 + v < g.M) store_right(g, i0 + v, j, (double)acc[x][y][v]);
       }
     }
+}
+
+// ---- left, ring ------------------------------------------------------------------
+// X <- nu X + eta Upsilon[:, idx] K_w with kLeftRingWaves waves per block:
+// wave w owns kLeftJT 16-point tiles of the block's points (features in
+// registers) and all 16 TP map rows, so each Upsilon fragment read from LDS
+// feeds kLeftJT MFMAs.  Per step of 32 window rows each wave forms its Gram
+// tiles (window rows x its points) with the window rows as the A operand
+// read in the order pi(4a + t) = 8a + 4x + t for sub-tile x, so the f32
+// results hold window rows 8 (l/16) .. +7 of point l%16: the B operand of
+// the sketch MFMA in natural row order, matched by one 16-byte read of the
+// Upsilon slice.  The step's window rows (32 x DP), their norms and the
+// Upsilon slice (16 TP rows x 32 window rows, 64-byte rows) stream global ->
+// LDS through global_load_lds into a kRingSlots-slot ring shared by all
+// waves, one barrier per step.  Needs row0 a multiple of 8 (16-byte aligned
+// slices).
+constexpr int kLeftRingWaves = 4, kLeftJT = 4;
+
+template <int DP, int TP>
+__global__ __launch_bounds__(64 * kLeftRingWaves, 1) void k_rbf_left_bf16_ring(RbfLpArgs g) {
+  constexpr int NW = kLeftRingWaves, JT = kLeftJT, BK = 32, NS = kRingSlots;
+  constexpr int PR = 16 * TP;                       // map rows of the block
+  constexpr int CPR = DP / 8;
+  constexpr int RPI = 1024 / (2 * DP);
+  constexpr int US_BYTES = PR * BK * 2;             // Upsilon slice
+  constexpr int XS_BYTES = BK * DP * 2;             // window rows
+  constexpr int NI_OFF = US_BYTES + XS_BYTES;
+  constexpr int SLOT = (NI_OFF + 256 + 1023) / 1024 * 1024;
+  constexpr int I_US = US_BYTES / 1024, I_XS = XS_BYTES / 1024, ITEMS = I_US + I_XS + 1;
+  constexpr int IPW = (ITEMS + NW - 1) / NW;
+  static_assert(US_BYTES % 1024 == 0 && XS_BYTES % 1024 == 0 && (NS - 2) * IPW <= 62, "copy items");
+  __shared__ __attribute__((aligned(1024))) uint8_t ring[NS * SLOT];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t wj0 = (int64_t)t * 16 * JT * NW + 16 * JT * wave;  // the wave's first point
+  const int64_t p0 = (int64_t)blockIdx.y * PR;
+  const int steps = (int)((g.K + BK - 1) / BK);
+  const int dw = (ITEMS - wave + NW - 1) / NW;  // copies this wave issues per step
+  const float lg = 1.4426950408889634f * g.gamma;
+  const float kappa = 2.f * lg;
+  const bool pts_full = wj0 + 16 * JT <= g.N;
+
+  bf16x8 xb[JT][DP / 32];
+  float be[JT];
+#pragma unroll
+  for (int y = 0; y < JT; ++y) {
+    const int64_t jp = wj0 + 16 * y + l16;
+#pragma unroll
+    for (int c = 0; c < DP / 32; ++c)
+      xb[y][c] = (jp < g.N) ? load8(g.P + jp * g.ldpb, 32 * c + 8 * lq, g.ldpb) : bf16x8{};
+    be[y] = (jp < g.N) ? -lg * g.nrm[jp] : 0.f;
+  }
+
+  auto issue = [&](int u) {
+    uint8_t* slot = ring + (u % NS) * SLOT;
+    const int64_t i0 = (int64_t)u * BK;
+#pragma unroll
+    for (int s = 0; s < IPW; ++s) {
+      const int it = wave + NW * s;
+      if (it < I_US) {  // 16 map rows of the Upsilon slice
+        const int r = 16 * it + (lane >> 2), p = lane & 3;
+        int64_t row = p0 + r;
+        row = row < g.M ? row : g.M - 1;
+        const int64_t e = i0 + 8 * (p ^ os_swz(r));
+        const __bf16* src = g.U + row * g.ldu + g.row0 + (e < g.K ? e : 0);  // past the window: any finite data
+        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + 1024 * it), 16, 0, 0);
+      } else if (it < I_US + I_XS) {  // RPI window rows
+        const int x = it - I_US;
+        const int r = RPI * x + lane / CPR, p = lane % CPR;
+        int64_t i = i0 + r;
+        i = i < g.K ? i : g.K - 1;
+        const __bf16* src = g.P + (g.row0 + i) * g.ldpb + 8 * (p ^ xs_swz<CPR>(r));
+        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + US_BYTES + 1024 * x), 16, 0,
+                                         0);
+      } else if (it == I_US + I_XS) {  // norms of the step's window rows
+        int64_t i = i0 + (lane & 31);
+        i = i < g.K ? i : g.K - 1;
+        __builtin_amdgcn_global_load_lds((const void*)(g.nrm + g.row0 + i), (SKEMA_LDS void*)(slot + NI_OFF), 4, 0,
+                                         0);
+      }
+    }
+  };
+
+  // K of step u for the wave's point tiles: B operands of the sketch MFMA
+  // (lane l: point l%16 of tile y, window rows 8 (l/16) .. +7 of the step)
+  auto gram_exp = [&](int u, bf16x8 (&kb)[JT]) {
+    const uint8_t* slot = ring + (u % NS) * SLOT;
+    const uint8_t* xs = slot + US_BYTES;
+    const float* ni = reinterpret_cast<const float*>(slot + NI_OFF);
+    const int64_t i0 = (int64_t)u * BK;
+    f4v gr[2][JT];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const int r = 8 * (l16 >> 2) + 4 * x + (l16 & 3);
+#pragma unroll
+      for (int y = 0; y < JT; ++y) gr[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < DP / 32; ++c) {
+        const int pos = (4 * c + lq) ^ xs_swz<CPR>(r);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(xs + r * (2 * DP) + 16 * pos);
+#pragma unroll
+        for (int y = 0; y < JT; ++y) gr[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[y][c], gr[x][y], 0, 0, 0);
+      }
+    }
+    f4v al[2];
+    al[0] = *reinterpret_cast<const f4v*>(ni + 8 * lq);
+    al[1] = *reinterpret_cast<const f4v*>(ni + 8 * lq + 4);
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) al[x][v] *= -lg;
+    const int64_t gi0 = g.row0 + i0;  // the step's window rows, as point indices
+    const bool interior = pts_full && i0 + BK <= g.K && (gi0 + BK <= wj0 || gi0 >= wj0 + 16 * JT);
+    if (interior) {
+#pragma unroll
+      for (int y = 0; y < JT; ++y)
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) kb[y][4 * x + v] = (__bf16)rbf_exp2(gr[x][y][v], al[x][v] + be[y], kappa);
+    } else {
+#pragma unroll
+      for (int y = 0; y < JT; ++y) {
+        const int64_t jp = wj0 + 16 * y + l16;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int64_t i = i0 + 8 * lq + 4 * x + v;
+            float e = 0.f;
+            if (i < g.K && jp < g.N) e = (g.row0 + i == jp) ? 1.f : rbf_exp2(gr[x][y][v], al[x][v] + be[y], kappa);
+            kb[y][4 * x + v] = (__bf16)e;
+          }
+      }
+    }
+  };
+
+  f4v acc[JT][TP];
+#pragma unroll
+  for (int y = 0; y < JT; ++y)
+#pragma unroll
+    for (int z = 0; z < TP; ++z) acc[y][z] = f4v{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int p = 0; p < NS - 1; ++p)
+    if (p < steps) issue(p);
+  for (int u = 0; u < steps; ++u) {
+    // step u complete: at most steps u+1 .. min(u+NS-2, steps-1) in flight
+    const int last = (u + NS - 2 < steps - 1) ? u + NS - 2 : steps - 1;
+    wait_vm_upto((last - u) * dw);
+    lds_barrier();  // every wave's copies of step u landed; slot (u-1) % NS is free
+    if (u + NS - 1 < steps) issue(u + NS - 1);
+    bf16x8 kb[JT];
+    gram_exp(u, kb);
+    const uint8_t* us = ring + (u % NS) * SLOT;
+#pragma unroll
+    for (int z = 0; z < TP; ++z) {
+      const int r = 16 * z + l16;
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(us + 64 * r + 16 * (lq ^ os_swz(r)));
+#pragma unroll
+      for (int y = 0; y < JT; ++y) acc[y][z] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, kb[y], acc[y][z], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int y = 0; y < JT; ++y) {
+    const int64_t jp = wj0 + 16 * y + l16;
+#pragma unroll
+    for (int z = 0; z < TP; ++z)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t p = p0 + 16 * z + 4 * lq + v;
+        if (p < g.M && jp < g.N) store_out(g.out, p, jp, (double)acc[y][z][v]);
+      }
+  }
 }
 
 // Sum of the FP32 partials of the point splits (fixed order), then the
@@ -425,25 +658,28 @@ __global__ void k_rbf_right_bf16_reduce(RbfLpArgs g, int splits) {
 }
 
 // ---- left ------------------------------------------------------------------------
-// 256 threads own 64 points (wave w: points 16w..16w+15, features in
-// registers) and 16*TP map rows.  Per step of 32 window rows each wave forms
+// NW waves own 16 NW points (wave w: points 16w..16w+15, features in
+// registers) and 16*TP map rows; the window rows and the Upsilon slice of a
+// step are staged once per block, so more waves per block mean fewer L2
+// bytes per flop.  Per step of 32 window rows each wave forms
 // its two 16x16 Gram tiles (rows 0-15 and 16-31 of the step), exponentiates
 // them in registers and packs the 8 values of each lane into the B operand of
 // one sketch MFMA: K-slot q of lane l stands for window row
 // 16*(q/4) + 4*(l/16) + q%4, which is exactly where the f32 result layout put
 // them, and the Upsilon A operand is read from LDS in the same row order.
-template <int DP, int TP>
-__global__ __launch_bounds__(256) void k_rbf_left_bf16(RbfLpArgs g) {
+template <int DP, int TP, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void k_rbf_left_bf16(RbfLpArgs g) {
+  constexpr int NT = 64 * NW;
   constexpr int BK = 32, PS = DP + 8, PR = 16 * TP, US = BK + 8;
   constexpr int XCH = BK * DP / 8, UCH = PR * BK / 8;  // 16-byte chunks per step
-  constexpr int XR = (XCH + 255) / 256, UR = (UCH + 255) / 256;
+  constexpr int XR = (XCH + NT - 1) / NT, UR = (UCH + NT - 1) / NT;
   __shared__ __attribute__((aligned(16))) __bf16 Xw[2][BK][PS];
   __shared__ __attribute__((aligned(16))) __bf16 Us[2][PR][US];
   __shared__ float NI[2][BK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, lq = lane >> 4;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t n0 = (int64_t)t * 64;
+  const int64_t n0 = (int64_t)t * 16 * NW;
   const int64_t p0 = (int64_t)blockIdx.y * PR;
   const int64_t steps = (g.K + BK - 1) / BK;
   const int64_t jp = n0 + 16 * wave + l16;
@@ -459,14 +695,14 @@ __global__ __launch_bounds__(256) void k_rbf_left_bf16(RbfLpArgs g) {
     const int64_t i0 = u * BK;
 #pragma unroll
     for (int q = 0; q < XR; ++q) {
-      const int c = tid + 256 * q;
+      const int c = tid + NT * q;
       const int ii = c / (DP / 8), fc = c % (DP / 8);
       const int64_t i = i0 + ii;
       xr[q] = (c < XCH && i < g.K) ? load8(g.P + (g.row0 + i) * g.ldpb, 8 * fc, g.ldpb) : bf16x8{};
     }
 #pragma unroll
     for (int q = 0; q < UR; ++q) {
-      const int c = tid + 256 * q;
+      const int c = tid + NT * q;
       const int pp = c / (BK / 8), ic = c % (BK / 8);
       const int64_t p = p0 + pp;
       ur[q] = (c < UCH && p < g.M) ? load8(g.U + p * g.ldu + g.row0, i0 + 8 * ic, g.K) : bf16x8{};
@@ -476,12 +712,12 @@ __global__ __launch_bounds__(256) void k_rbf_left_bf16(RbfLpArgs g) {
   auto store_step = [&](int b) {
 #pragma unroll
     for (int q = 0; q < XR; ++q) {
-      const int c = tid + 256 * q;
+      const int c = tid + NT * q;
       if (c < XCH) *reinterpret_cast<bf16x8*>(&Xw[b][c / (DP / 8)][8 * (c % (DP / 8))]) = xr[q];
     }
 #pragma unroll
     for (int q = 0; q < UR; ++q) {
-      const int c = tid + 256 * q;
+      const int c = tid + NT * q;
       if (c < UCH) *reinterpret_cast<bf16x8*>(&Us[b][c / (BK / 8)][8 * (c % (BK / 8))]) = ur[q];
     }
     if (tid < BK) NI[b][tid] = nir;
@@ -558,14 +794,17 @@ void launch_right(hipStream_t st, const RbfLpArgs& a, dim3 grid, int tn) {
     default: hipLaunchKernelGGL((k_rbf_right_bf16<DP, 5>), grid, dim3(kThreadsR), 0, st, a); break;
   }
 }
+// Blocks of 8 waves (128 points) for the long map-row tiles, where two
+// waves per SIMD is the register-bound occupancy anyway; 4 waves otherwise.
+constexpr int kLeftWavesWide = 8;
 template <int DP>
 void launch_left(hipStream_t st, const RbfLpArgs& a, dim3 grid, int tp) {
   switch (tp) {
     case 1: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 1>), grid, dim3(256), 0, st, a); break;
     case 2: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 2>), grid, dim3(256), 0, st, a); break;
     case 4: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 4>), grid, dim3(256), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 8>), grid, dim3(256), 0, st, a); break;
-    default: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 13>), grid, dim3(256), 0, st, a); break;
+    case 8: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 8, kLeftWavesWide>), grid, dim3(64 * kLeftWavesWide), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_rbf_left_bf16<DP, 13, kLeftWavesWide>), grid, dim3(64 * kLeftWavesWide), 0, st, a); break;
   }
 }
 
@@ -616,9 +855,10 @@ void rbf_right_bf16(Ctx* c, const __bf16* P, int dp, const float* nrm, float gam
   }
   const dim3 grid((unsigned)(a.tiles_m * splits), (unsigned)col_blocks);
   if (tall) {
-    if (dp == 32) hipLaunchKernelGGL(k_rbf_right_bf16_tall<32>, grid, dim3(512), 0, st, a);
-    else if (dp == 64) hipLaunchKernelGGL(k_rbf_right_bf16_tall<64>, grid, dim3(512), 0, st, a);
-    else hipLaunchKernelGGL(k_rbf_right_bf16_tall<128>, grid, dim3(512), 0, st, a);
+    const dim3 tb(64 * (kTallRows / (16 * kTallXT)));
+    if (dp == 32) hipLaunchKernelGGL(k_rbf_right_bf16_tall<32>, grid, tb, 0, st, a);
+    else if (dp == 64) hipLaunchKernelGGL(k_rbf_right_bf16_tall<64>, grid, tb, 0, st, a);
+    else hipLaunchKernelGGL(k_rbf_right_bf16_tall<128>, grid, tb, 0, st, a);
   } else if (dp == 32) launch_right<32>(st, a, grid, tn);
   else if (dp == 64) launch_right<64>(st, a, grid, tn);
   else launch_right<128>(st, a, grid, tn);
@@ -657,11 +897,29 @@ void update_rbf_window_bf16(Sketch& sk, const double* points, int64_t ldp, int64
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
     const int tp = pick(kTpChoices, cdiv(k, 16));
-    const dim3 grid((unsigned)cdiv(n, 64), (unsigned)cdiv(k, 16 * tp));
-    if (dp == 32) launch_left<32>(st, a, grid, tp);
-    else if (dp == 64) launch_left<64>(st, a, grid, tp);
-    else launch_left<128>(st, a, grid, tp);
-    check_launch("k_rbf_left_bf16");
+    if (tp >= 8 && row0 % 8 == 0) {  // long map-row tiles on aligned windows: the ring kernel
+      const dim3 grid((unsigned)cdiv(n, 16 * kLeftJT * kLeftRingWaves), (unsigned)cdiv(k, 16 * tp));
+      const dim3 block(64 * kLeftRingWaves);
+#define SKEMA_LEFT_RING(DP, TP) hipLaunchKernelGGL((k_rbf_left_bf16_ring<DP, TP>), grid, block, 0, st, a)
+      if (tp == 8) {
+        if (dp == 32) SKEMA_LEFT_RING(32, 8);
+        else if (dp == 64) SKEMA_LEFT_RING(64, 8);
+        else SKEMA_LEFT_RING(128, 8);
+      } else {
+        if (dp == 32) SKEMA_LEFT_RING(32, 13);
+        else if (dp == 64) SKEMA_LEFT_RING(64, 13);
+        else SKEMA_LEFT_RING(128, 13);
+      }
+#undef SKEMA_LEFT_RING
+      check_launch("k_rbf_left_bf16_ring");
+    } else {
+      const int64_t pts = tp >= 8 ? 16 * kLeftWavesWide : 64;  // points per block
+      const dim3 grid((unsigned)cdiv(n, pts), (unsigned)cdiv(k, 16 * tp));
+      if (dp == 32) launch_left<32>(st, a, grid, tp);
+      else if (dp == 64) launch_left<64>(st, a, grid, tp);
+      else launch_left<128>(st, a, grid, tp);
+      check_launch("k_rbf_left_bf16");
+    }
   }
   {
     Timed t(c, "rbf_right");

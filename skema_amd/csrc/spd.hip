@@ -240,7 +240,7 @@ void spd_low_rank_approx(Spd& sp, int shift_removal, double* U, int64_t ldu, dou
     // thin SVD of E: Householder QR, Jacobi SVD of R, U = Q U_R
     householder_qr(st, n, k, E.as<double>(), n, tau.as<double>(), &c->qr, &c->gws);
     extract_r(st, k, E.as<double>(), n, Re.as<double>(), k);
-    householder_q(st, n, k, E.as<double>(), n, tau.as<double>(), Q.as<double>(), n, &c->qr, &c->gws);
+    householder_q(st, n, k, E.as<double>(), n, tau.as<double>(), Q.as<double>(), n, &c->qr, &c->gws, true);
     jacobi_svd(st, k, Re.as<double>(), k, Ur.as<double>(), k, Se.as<double>(), Vr.as<double>(), k, &scratch);
     gemm_f64(st, 'N', 'N', n, r, k, 1.0, Q.as<double>(), n, Ur.as<double>(), k, 0.0, U, ldu, &c->gws);
     hipLaunchKernelGGL(k_remove_shift, dim3(g256(r)), dim3(256), 0, st, r, Se.as<double>(),

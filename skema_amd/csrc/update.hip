@@ -304,24 +304,24 @@ void finalize_sketch(Sketch& sk) {
   // [P,~] = qr(X^T,0)  (:355-357)
   transpose(st, k, n, sk.X.as<double>(), k, W, n);
   householder_qr(st, n, k, W, n, tau, &c->qr, &c->gws);
-  householder_q(st, n, k, W, n, tau, P, n, &c->qr, &c->gws);
+  householder_q(st, n, k, W, n, tau, P, n, &c->qr, &c->gws, true);
   // [Q,~] = qr(Y,0)  (:381-383), on a copy: Y stays intact.
   copy_matrix(st, m, k, sk.Y.as<double>(), m, W, m);
   householder_qr(st, m, k, W, m, tau, &c->qr, &c->gws);
-  householder_q(st, m, k, W, m, tau, Q, m, &c->qr, &c->gws);
+  householder_q(st, m, k, W, m, tau, Q, m, &c->qr, &c->gws, true);
   tm["dgeqrf"] += sw.lap();
 
   // U1 = Phi*Q, U2 = Psi*P  (:409, :430)
   if (sk.sparse_maps) {
     spmm_csr_rows(st, sk.maps[2]->logical_csr(), 0, s, 0, m, 1.0, Q, m, k, 0.0, PhiQ, s);
-    tm["phi"] = sw.lap();
+    tm["phi"] += sw.lap();
     spmm_csr_rows(st, sk.maps[3]->logical_csr(), 0, s, 0, n, 1.0, P, n, k, 0.0, PsiP, s);
-    tm["psi"] = sw.lap();
+    tm["psi"] += sw.lap();
   } else {
     gemm_f64(st, 'N', 'N', s, k, m, 1.0, sk.phiL, sk.ldphi, Q, m, 0.0, PhiQ, s, &c->gws);
-    tm["phi"] = sw.lap();
+    tm["phi"] += sw.lap();
     gemm_f64(st, 'N', 'N', s, k, n, 1.0, sk.omps.as<double>() + k, k + s, P, n, 0.0, PsiP, s, &c->gws);
-    tm["psi"] = sw.lap();
+    tm["psi"] += sw.lap();
   }
 
   // [U1,T1] = qr(Phi*Q,0); [U2,T2] = qr(Psi*P,0)  (:453, :476)
@@ -335,11 +335,11 @@ void finalize_sketch(Sketch& sk) {
   copy_matrix(st, s, k, PhiQ, s, F, s);
   householder_qr(st, s, k, F, s, tau, &c->qr, &c->gws);
   extract_r(st, k, F, s, T1, k);
-  householder_q(st, s, k, F, s, tau, U1, s, &c->qr, &c->gws);
+  householder_q(st, s, k, F, s, tau, U1, s, &c->qr, &c->gws, true);
   copy_matrix(st, s, k, PsiP, s, F, s);
   householder_qr(st, s, k, F, s, tau, &c->qr, &c->gws);
   extract_r(st, k, F, s, T2, k);
-  householder_q(st, s, k, F, s, tau, U2, s, &c->qr, &c->gws);
+  householder_q(st, s, k, F, s, tau, U2, s, &c->qr, &c->gws, true);
   tm["dgeqrf"] += sw.lap();
 
   // Z2 = (U1^T * Z) * U2  (:500, :515)
@@ -368,7 +368,7 @@ void finalize_sketch(Sketch& sk) {
     householder_qr(st, k, k, Cw, k, tau, &c->qr, &c->gws);
     extract_r(st, k, Cw, k, Z2, k);
     transpose(st, k, k, Z2, k, Rt, k);
-    householder_q(st, k, k, Cw, k, tau, Qc, k, &c->qr, &c->gws);
+    householder_q(st, k, k, Cw, k, tau, Qc, k, &c->qr, &c->gws, true);
     jacobi_svd(st, k, Rt, k, Ur, k, Sv, Vr, k, &c->scratch);
     gemm_f64(st, 'N', 'N', k, k, k, 1.0, Qc, k, Vr, k, 0.0, Uc, k, &c->gws);
     copy_matrix(st, k, k, Ur, k, Vc, k);

@@ -295,7 +295,7 @@ BF16_TOL = 2e-2
 
 
 @pytest.mark.parametrize("n,d,w,rank", [(333, 37, 50, 4), (257, 128, 64, 6), (130, 3, 17, 3), (2600, 64, 300, 10),
-                                        (1200, 20, 256, 60)])
+                                        (1200, 20, 256, 60), (3000, 50, 512, 25), (1000, 100, 300, 40)])
 def test_rbf_bf16_form_against_oracle(S, n, d, w, rank):
     print(f"case rbf bf16 n={n} d={d} w={w} r={rank}", flush=True)
     rng = np.random.default_rng(n + d + 1)
