@@ -16,7 +16,7 @@
 //         lanes over its nonzeros, each nonzero's term c_i a added into the
 //         rows p of the maps' column i;
 //   right (Y): one block per 32 rows of A, threads over their nonzeros.
-//   Both accumulate in LDS in exact two-word fixed point (fixedpoint.hpp),
+//   Both accumulate in LDS in int64 fixed point (fixedpoint.hpp),
 //   so every sum is the same whatever order the atomics land in: X, Y, Z
 //   are bitwise reproducible.
 // Dense maps: one wave per column (left) or row (right), lanes own map rows,
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(1024) void k_l2_place(const int* base, int nt, int 
   }
 }
 
-// ---- sparse-map products in exact fixed point --------------------------------
+// ---- sparse-map products in fixed point --------------------------------------
 // A sparse logical map seen by its columns: the ELL form of its transpose
 // (z entries of one magnitude per column: ell[c z + f] = row | sign << 31),
 // or the CSR of its transpose (rp, ci, v).  Either way at most zmax entries
@@ -268,17 +268,16 @@ __device__ inline void ell_load(const SpMapCols& M, int64_t c, uint32_t (&u)[ZM]
   }
 }
 
-// Adds the fixed-point term (H, L), signed by each entry, into the pairs
-// acc[2 (p - p0)] of the entries' rows p in [p0, p0 + pn).
+// Adds the fixed-point term V, signed by each entry, into the accumulators
+// acc[p - p0] of the entries' rows p in [p0, p0 + pn).
 template <int ZM>
-__device__ inline void ell_add(const uint32_t (&u)[ZM], int z, long long H, long long L, int64_t p0, int64_t pn,
+__device__ inline void ell_add(const uint32_t (&u)[ZM], int z, long long V, int64_t p0, int64_t pn,
                                unsigned long long* acc) {
 #pragma unroll
   for (int f = 0; f < ZM; ++f) {
     const int64_t p = (int64_t)(u[f] & 0x7fffffffu) - p0;
     if (f < z && p >= 0 && p < pn) {
-      if (u[f] >> 31) fx_add_lds(acc + 2 * p, -H, -L);
-      else fx_add_lds(acc + 2 * p, H, L);
+      fx_add_lds(acc + p, (u[f] >> 31) ? -V : V);
     }
   }
 }
@@ -289,9 +288,7 @@ __device__ inline void csr_add(const SpMapCols& M, int64_t c, double t, const Fx
   for (int64_t f = M.rp[c]; f < M.rp[c + 1]; ++f) {
     const int64_t p = (int64_t)M.ci[f] - p0;
     if (p >= 0 && p < pn) {
-      long long h, l;
-      fx_split(t * M.v[f], fs, h, l);
-      fx_add_lds(acc + 2 * p, h, l);
+      fx_add_lds(acc + p, fx_term(t * M.v[f], fs));
     }
   }
 }
@@ -301,12 +298,14 @@ __device__ inline void csr_add(const SpMapCols& M, int64_t c, double t, const Fx
 // cw[i / window].  One wave per column j: a first pass takes the column's
 // terms c_i a (up to kLeftCache per lane kept in registers) and their
 // largest magnitude (the fixed-point scale of the column's accumulators); a
-// second adds every term into the n1 + n2 accumulator pairs of the wave in
+// second adds every term into the n1 + n2 accumulators of the wave in
 // LDS, the map columns of all cached terms loaded before any is added.  ELL
 // terms carry the map magnitude only at the end.  ZM: ELL entries per map
 // column at most (both maps), 0 for maps given as CSR.  Four waves per
-// block, columns in a block-strided loop with a uniform trip count
-// (barriers).
+// block, each on its own columns with its own accumulators: a wave's LDS
+// operations complete in issue order, so it needs no barrier between
+// clearing, adding and reading back (only a compiler fence), and the waves
+// of a block never wait for each other.
 constexpr int kLeftCache = 4;
 
 template <int ZM>
@@ -314,18 +313,17 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
                                                 int64_t ncols, const double* cw, int64_t window, SpMapCols M1,
                                                 int64_t p1, int64_t n1, double* out1, int64_t ld1, SpMapCols M2,
                                                 int64_t p2, int64_t n2, double* out2, int64_t ld2) {
-  extern __shared__ unsigned long long fxacc[];  // 4 waves x 2 (n1 + n2)
+  extern __shared__ unsigned long long fxacc[];  // 4 waves x (n1 + n2)
   constexpr bool ELL = ZM > 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t na = n1 + n2;
-  unsigned long long* acc = fxacc + (int64_t)wave * 2 * na;
-  const int64_t groups = (ncols + 3) / 4;
+  unsigned long long* acc = fxacc + (int64_t)wave * na;
   const uint32_t wdw = (uint32_t)window;
-  for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
-    const int64_t j = 4 * g + wave;
-    for (int64_t q = lane; q < 2 * na; q += 64) acc[q] = 0ull;
+  for (int64_t j = 4 * (int64_t)blockIdx.x + wave; j < ncols; j += 4 * (int64_t)gridDim.x) {
+    for (int64_t q = lane; q < na; q += 64) acc[q] = 0ull;
+    asm volatile("" ::: "memory");
     FxScale fs;
-    if (j < ncols) {
+    {
       const int64_t e0 = cptr[j], e1 = cptr[j + 1];
       double tq[kLeftCache];
       int32_t iq[kLeftCache];
@@ -374,10 +372,9 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
 #pragma unroll
           for (int q = 0; q < kLeftCache; ++q) {
             if (iq[q] >= 0) {
-              long long H, L;
-              fx_split(tq[q], fs, H, L);
-              ell_add<ZM>(u1[q], M1.z, H, L, p1, n1, acc);
-              if (n2) ell_add<ZM>(u2[q], M2.z, H, L, p2, n2, acc + 2 * n1);
+              const long long V = fx_term(tq[q], fs);
+              ell_add<ZM>(u1[q], M1.z, V, p1, n1, acc);
+              if (n2) ell_add<ZM>(u2[q], M2.z, V, p2, n2, acc + n1);
             }
           }
           for (int64_t e = e0 + 64 * kLeftCache + lane; e < e1; e += 64) {
@@ -386,37 +383,33 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
             uint32_t v1[ZM], v2[ZM];
             ell_load<ZM>(M1, i, v1);
             if (n2) ell_load<ZM>(M2, i, v2);
-            long long H, L;
-            fx_split(t, fs, H, L);
-            ell_add<ZM>(v1, M1.z, H, L, p1, n1, acc);
-            if (n2) ell_add<ZM>(v2, M2.z, H, L, p2, n2, acc + 2 * n1);
+            const long long V = fx_term(t, fs);
+            ell_add<ZM>(v1, M1.z, V, p1, n1, acc);
+            if (n2) ell_add<ZM>(v2, M2.z, V, p2, n2, acc + n1);
           }
         } else {
 #pragma unroll
           for (int q = 0; q < kLeftCache; ++q) {
             if (iq[q] >= 0) {
               csr_add(M1, iq[q], tq[q], fs, p1, n1, acc);
-              if (n2) csr_add(M2, iq[q], tq[q], fs, p2, n2, acc + 2 * n1);
+              if (n2) csr_add(M2, iq[q], tq[q], fs, p2, n2, acc + n1);
             }
           }
           for (int64_t e = e0 + 64 * kLeftCache + lane; e < e1; e += 64) {
             const int32_t i = crow[e];
             const double t = cval[e] * cw[(uint32_t)i / wdw];
             csr_add(M1, i, t, fs, p1, n1, acc);
-            if (n2) csr_add(M2, i, t, fs, p2, n2, acc + 2 * n1);
+            if (n2) csr_add(M2, i, t, fs, p2, n2, acc + n1);
           }
         }
       }
     }
-    __syncthreads();  // the wave's LDS atomics are complete before it reads back
-    if (j < ncols) {
-      for (int64_t q = lane; q < n1; q += 64)
-        out1[(p1 + q) + j * ld1] = (ELL ? M1.mag : 1.0) * fx_value((long long)acc[2 * q], (long long)acc[2 * q + 1], fs);
-      for (int64_t q = lane; q < n2; q += 64)
-        out2[(p2 + q) + j * ld2] =
-            (ELL ? M2.mag : 1.0) * fx_value((long long)acc[2 * (n1 + q)], (long long)acc[2 * (n1 + q) + 1], fs);
-    }
-    __syncthreads();  // read back before the next column clears the accumulators
+    asm volatile("" ::: "memory");  // the wave's atomics precede its reads (LDS order within a wave)
+    for (int64_t q = lane; q < n1; q += 64)
+      out1[(p1 + q) + j * ld1] = (ELL ? M1.mag : 1.0) * fx_value((long long)acc[q], fs);
+    for (int64_t q = lane; q < n2; q += 64)
+      out2[(p2 + q) + j * ld2] = (ELL ? M2.mag : 1.0) * fx_value((long long)acc[n1 + q], fs);
+    asm volatile("" ::: "memory");  // read back before the next column clears the accumulators
   }
 }
 
@@ -425,7 +418,7 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
 // thread takes the block's nonzeros tid, tid + 256, ... (kRightCache of them
 // held in registers, with their map columns loaded at once), the rows'
 // largest |a| (their fixed-point scales) by integer atomicMax on the IEEE
-// bits, then adds every term into acc[slot][row] pairs; the rows are written
+// bits, then adds every term into acc[slot][row]; the rows are written
 // out coalesced (consecutive threads, consecutive rows).  ZM as for
 // k_left_fx.
 constexpr int kRightRows = 32;
@@ -436,7 +429,7 @@ template <int ZM>
 __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const int32_t* col, const double* val,
                                                  int64_t rows, SpMapCols M, int64_t p0, int64_t pn, double eta,
                                                  double* Y, int64_t ldy) {
-  extern __shared__ unsigned long long fxacc[];  // 2 x pn x kRightRows
+  extern __shared__ unsigned long long fxacc[];  // pn x kRightRows
   __shared__ int64_t rp[kRightRows + 1];
   __shared__ unsigned long long rmax[kRightRows];  // bits of the row's largest |a| (non-negative doubles order as integers)
   __shared__ FxScale rs[kRightRows];
@@ -444,7 +437,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
   const int tid = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * kRightRows;
   const int nr = (int)((r0 + kRightRows < rows) ? kRightRows : rows - r0);
-  for (int q = tid; q < 2 * pn * kRightRows; q += 256) fxacc[q] = 0ull;
+  for (int q = tid; q < pn * kRightRows; q += 256) fxacc[q] = 0ull;
   for (int i = tid; i <= nr; i += 256) rp[i] = rowptr[r0 + i];
   if (tid < kRightRows) rmax[tid] = 0ull;
   __syncthreads();
@@ -494,15 +487,12 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
   auto add_ell = [&](const uint32_t (&uu)[ZM > 0 ? ZM : 1], double t, int rr) {
     const FxScale fs = rs[rr];
     if (!fs.finite) return;
-    long long H, L;
-    fx_split(t, fs, H, L);
+    const long long V = fx_term(t, fs);
 #pragma unroll
     for (int f = 0; f < (ZM > 0 ? ZM : 1); ++f) {
       const int64_t p = (int64_t)(uu[f] & 0x7fffffffu) - p0;
       if (f < M.z && p >= 0 && p < pn) {
-        unsigned long long* a = fxacc + 2 * (p * kRightRows + rr);
-        if (uu[f] >> 31) fx_add_lds(a, -H, -L);
-        else fx_add_lds(a, H, L);
+        fx_add_lds(fxacc + p * kRightRows + rr, (uu[f] >> 31) ? -V : V);
       }
     }
   };
@@ -512,9 +502,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
     for (int64_t f = M.rp[j]; f < M.rp[j + 1]; ++f) {
       const int64_t p = (int64_t)M.ci[f] - p0;
       if (p >= 0 && p < pn) {
-        long long h, l;
-        fx_split(t * M.v[f], fs, h, l);
-        fx_add_lds(fxacc + 2 * (p * kRightRows + rr), h, l);
+        fx_add_lds(fxacc + p * kRightRows + rr, fx_term(t * M.v[f], fs));
       }
     }
   };
@@ -540,7 +528,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
     const int rr = q % kRightRows;
     const int64_t p = q / kRightRows;
     if (rr < nr)
-      Y[(r0 + rr) + (p0 + p) * ldy] = scale * fx_value((long long)fxacc[2 * q], (long long)fxacc[2 * q + 1], rs[rr]);
+      Y[(r0 + rr) + (p0 + p) * ldy] = scale * fx_value((long long)fxacc[q], rs[rr]);
   }
 }
 
@@ -626,7 +614,7 @@ SpMapCols map_cols(const Map& M) {
   return c;
 }
 
-// Accumulator pairs per wave of k_left_fx (4 waves: 64 KB of LDS).
+// Accumulators per wave of k_left_fx (4 waves: 16 KB of LDS).
 constexpr int64_t kLeftSlots = 512;
 
 // ELL entries per map column the kernels are compiled for (0: CSR maps).
@@ -637,7 +625,7 @@ void launch_left_fx(hipStream_t st, int zm, const CscWindow& H, const double* cw
                     int64_t p2, int64_t n2, double* o2, int64_t ld2) {
   const int64_t groups = cdiv(H.cols, 4);
   const unsigned grid = (unsigned)std::min<int64_t>(groups, 8192);
-  const size_t lds = (size_t)4 * 2 * (n1 + n2) * sizeof(unsigned long long);
+  const size_t lds = (size_t)4 * (n1 + n2) * sizeof(unsigned long long);
 #define SKEMA_LEFT_FX(ZM)                                                                                         \
   hipLaunchKernelGGL(k_left_fx<ZM>, dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, M1, \
                      p1, n1, o1, ld1, M2, p2, n2, o2, ld2)
@@ -673,7 +661,7 @@ void right_sparse_fx(hipStream_t st, const int64_t* rowptr, const int32_t* col, 
   const dim3 grid((unsigned)cdiv(m, kRightRows));
   for (int64_t p0 = 0; p0 < k; p0 += kRightSlots) {
     const int64_t pn = std::min<int64_t>(kRightSlots, k - p0);
-    const size_t lds = (size_t)2 * pn * kRightRows * sizeof(unsigned long long);
+    const size_t lds = (size_t)pn * kRightRows * sizeof(unsigned long long);
     const int zm = zm_class(M.ell_zeta);
     if (zm == 8)
       hipLaunchKernelGGL(k_right_fx<8>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);

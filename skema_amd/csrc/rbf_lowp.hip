@@ -330,41 +330,61 @@ __global__ __launch_bounds__(64 * (kTallRows / (16 * kTallXT)), 1) void k_rbf_ri
     al[x] = (i < g.M) ? -lg * g.nrm[g.row0 + i] : 0.f;
   }
 
+  // Per-lane sources of the wave's copies at the first step; a step moves
+  // each by BK points (2 BK bytes along an [Omega; Psi] row, BK points rows,
+  // 4 BK bytes of norms).  Near the end of the split the points rows and
+  // norms are clamped to the last point instead.
+  const char* dsrc[IPW];
+#pragma unroll
+  for (int s = 0; s < IPW; ++s) {
+    const int it = wave + NW * s;
+    if (it < I_OS) {  // 16 rows of the [Omega; Psi] slice
+      const int r = 16 * it + (lane >> 2), p = lane & 3;
+      int64_t col = c0 + r;
+      col = col < g.N ? col : g.N - 1;
+      dsrc[s] = reinterpret_cast<const char*>(g.OP + col * g.ldop + kbeg + 8 * (p ^ os_swz(r)));
+    } else if (it < I_OS + I_XS) {  // RPI points rows
+      const int x = it - I_OS;
+      const int r = RPI * x + lane / CPR, p = lane % CPR;
+      dsrc[s] = reinterpret_cast<const char*>(g.P + (kbeg + r) * g.ldpb + 8 * (p ^ xs_swz<CPR>(r)));
+    } else {  // norms of the step's points (lanes >= 32 duplicate, unused)
+      dsrc[s] = reinterpret_cast<const char*>(g.nrm + kbeg + (lane & 31));
+    }
+  }
   auto issue = [&](int u) {
     uint8_t* slot = ring + (u % NS) * SLOT;
-    const int64_t j0 = kbeg + (int64_t)u * BK;
+    const bool tail = kbeg + (int64_t)(u + 1) * BK > kend;
 #pragma unroll
     for (int s = 0; s < IPW; ++s) {
       const int it = wave + NW * s;
-      if (it < I_OS) {  // 16 rows of the [Omega; Psi] slice
-        const int r = 16 * it + (lane >> 2), p = lane & 3;
-        int64_t col = c0 + r;
-        col = col < g.N ? col : g.N - 1;
-        const __bf16* src = g.OP + col * g.ldop + j0 + 8 * (p ^ os_swz(r));
-        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + 1024 * it), 16, 0, 0);
-      } else if (it < I_OS + I_XS) {  // RPI points rows
+      if (it < I_OS) {
+        __builtin_amdgcn_global_load_lds((const void*)(dsrc[s] + (int64_t)u * (2 * BK)),
+                                         (SKEMA_LDS void*)(slot + 1024 * it), 16, 0, 0);
+      } else if (it < I_OS + I_XS) {
         const int x = it - I_OS;
-        const int r = RPI * x + lane / CPR, p = lane % CPR;
-        int64_t j = j0 + r;
-        j = j < kend ? j : kend - 1;
-        const __bf16* src = g.P + j * g.ldpb + 8 * (p ^ xs_swz<CPR>(r));
-        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + OS_BYTES + 1024 * x), 16, 0,
-                                         0);
-      } else if (it == I_OS + I_XS) {  // norms of the step's points (lanes >= 32 duplicate, unused)
-        int64_t j = j0 + (lane & 31);
-        j = j < kend ? j : kend - 1;
-        __builtin_amdgcn_global_load_lds((const void*)(g.nrm + j), (SKEMA_LDS void*)(slot + NJ_OFF), 4, 0, 0);
+        const char* src = dsrc[s] + (int64_t)u * BK * (2 * DP);
+        if (tail) {
+          const int r = RPI * x + lane / CPR, p = lane % CPR;
+          int64_t j = kbeg + (int64_t)u * BK + r;
+          j = j < kend ? j : kend - 1;
+          src = reinterpret_cast<const char*>(g.P + j * g.ldpb + 8 * (p ^ xs_swz<CPR>(r)));
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + OS_BYTES + 1024 * x), 16, 0, 0);
+      } else if (it == I_OS + I_XS) {
+        const char* src = dsrc[s] + (int64_t)u * (4 * BK);
+        if (tail) {
+          int64_t j = kbeg + (int64_t)u * BK + (lane & 31);
+          j = j < kend ? j : kend - 1;
+          src = reinterpret_cast<const char*>(g.nrm + j);
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + NJ_OFF), 4, 0, 0);
       }
     }
   };
 
-  // K of step u for the wave's row tiles, in registers (A operands)
-  auto gram_exp = [&](int u, bf16x8 (&kv)[XT]) {
-    const uint8_t* slot = ring + (u % NS) * SLOT;
-    const uint8_t* xs = slot + OS_BYTES;
-    const float* nj = reinterpret_cast<const float*>(slot + NJ_OFF);
-    const int64_t j0 = kbeg + (int64_t)u * BK;
-    f4v gr[2][XT];
+  // Gram tiles of step u (transposed: points x the wave's window rows)
+  auto gram = [&](int u, f4v (&gr)[2][XT]) {
+    const uint8_t* xs = ring + (u % NS) * SLOT + OS_BYTES;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int r = 8 * (l16 >> 2) + 4 * h + (l16 & 3);  // points row read by this lane
@@ -378,38 +398,43 @@ __global__ __launch_bounds__(64 * (kTallRows / (16 * kTallXT)), 1) void k_rbf_ri
         for (int x = 0; x < XT; ++x) gr[h][x] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx, xa[x][c], gr[h][x], 0, 0, 0);
       }
     }
+  };
+  // K of step u from its Gram tiles, in registers (A operands of the
+  // sketch), without checks: interior steps (every row and point in range,
+  // the window's diagonal not among them)
+  auto exp_fast = [&](int u, const f4v (&gr)[2][XT], bf16x8 (&kv)[XT]) {
+    const float* nj = reinterpret_cast<const float*>(ring + (u % NS) * SLOT + NJ_OFF);
     f4v be[2];
     be[0] = *reinterpret_cast<const f4v*>(nj + 8 * lq);
     be[1] = *reinterpret_cast<const f4v*>(nj + 8 * lq + 4);
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int x = 0; x < XT; ++x)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) be[h][v] *= -lg;
-    // interior steps (every row and point in range, the window's diagonal
-    // not among them) skip the per-element checks
-    const int64_t gi0 = g.row0 + wr0;
-    const bool interior = rows_full && j0 + BK <= kend && (j0 + BK <= gi0 || j0 >= gi0 + 16 * XT);
-    if (interior) {
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int x = 0; x < XT; ++x)
+        for (int v = 0; v < 4; ++v) kv[x][4 * h + v] = (__bf16)rbf_exp2(gr[h][x][v], al[x] - lg * be[h][v], kappa);
+  };
+  auto interior = [&](int u) {
+    const int64_t j0 = kbeg + (int64_t)u * BK, gi0 = g.row0 + wr0;
+    return rows_full && j0 + BK <= kend && (j0 + BK <= gi0 || j0 >= gi0 + 16 * XT);
+  };
+  // the same with the range and diagonal checks
+  auto exp_edge = [&](int u, const f4v (&gr)[2][XT], bf16x8 (&kv)[XT]) {
+    const float* nj = reinterpret_cast<const float*>(ring + (u % NS) * SLOT + NJ_OFF);
+    const int64_t j0 = kbeg + (int64_t)u * BK;
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+    for (int x = 0; x < XT; ++x) {
+      const int64_t i = wr0 + 16 * x + l16;
 #pragma unroll
-          for (int v = 0; v < 4; ++v) kv[x][4 * h + v] = (__bf16)rbf_exp2(gr[h][x][v], al[x] + be[h][v], kappa);
-    } else {
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int x = 0; x < XT; ++x) {
-        const int64_t i = wr0 + 16 * x + l16;
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int64_t j = j0 + 8 * lq + 4 * h + v;
-            float e = 0.f;
-            if (i < g.M && j < kend) e = (g.row0 + i == j) ? 1.f : rbf_exp2(gr[h][x][v], al[x] + be[h][v], kappa);
-            kv[x][4 * h + v] = (__bf16)e;
-          }
-      }
+        for (int v = 0; v < 4; ++v) {
+          const int64_t j = j0 + 8 * lq + 4 * h + v;
+          float e = 0.f;
+          if (i < g.M && j < kend)
+            e = (g.row0 + i == j) ? 1.f : rbf_exp2(gr[h][x][v], al[x] - lg * nj[8 * lq + 4 * h + v], kappa);
+          kv[x][4 * h + v] = (__bf16)e;
+        }
     }
   };
 
@@ -426,7 +451,10 @@ __global__ __launch_bounds__(64 * (kTallRows / (16 * kTallXT)), 1) void k_rbf_ri
   if (steps > 0) {
     wait_vm_upto(((NS - 2) < (steps - 1) ? (NS - 2) : (steps - 1)) * dw);  // step 0 complete
     lds_barrier();
-    gram_exp(0, kv);
+    f4v gr[2][XT];
+    gram(0, gr);
+    if (interior(0)) exp_fast(0, gr, kv);
+    else exp_edge(0, gr, kv);
   }
   for (int u = 0; u < steps; ++u) {
     if (u + 1 < steps) {  // step u + 1 complete: at most steps u+2 .. min(u+NS-2, steps-1) in flight
@@ -435,15 +463,43 @@ __global__ __launch_bounds__(64 * (kTallRows / (16 * kTallXT)), 1) void k_rbf_ri
     }
     lds_barrier();  // every wave's copies of step u+1 landed; slot (u-1) % NS is free
     if (u + NS - 1 < steps) issue(u + NS - 1);
+    const int un = (u + 1 < steps) ? u + 1 : u;  // the last step forms a Gram it does not use
+    f4v gr[2][XT];
+    gram(un, gr);
+    // the sketch MFMAs of step u, column tile by column tile, each followed
+    // by a share of the exponentials of step u+1 (which need only the Gram
+    // above): the matrix cores run one tile while the VALU forms the next
+    // K values (sched_barrier keeps the compiler from regrouping them)
     const uint8_t* os = ring + (u % NS) * SLOT;
+    const float* nj = reinterpret_cast<const float*>(ring + (un % NS) * SLOT + NJ_OFF);
+    f4v be[2];
+    be[0] = *reinterpret_cast<const f4v*>(nj + 8 * lq);
+    be[1] = *reinterpret_cast<const f4v*>(nj + 8 * lq + 4);
+    bf16x8 kn[XT];
+    bf16x8 o = *reinterpret_cast<const bf16x8*>(os + 64 * l16 + 16 * (lq ^ os_swz(l16)));
 #pragma unroll
     for (int y = 0; y < TNC; ++y) {
-      const int r = 16 * y + l16;
-      const bf16x8 o = *reinterpret_cast<const bf16x8*>(os + 64 * r + 16 * (lq ^ os_swz(r)));
+      bf16x8 on = o;
+      if (y + 1 < TNC) {
+        const int r = 16 * (y + 1) + l16;
+        on = *reinterpret_cast<const bf16x8*>(os + 64 * r + 16 * (lq ^ os_swz(r)));
+      }
 #pragma unroll
       for (int x = 0; x < XT; ++x) acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(kv[x], o, acc[x][y], 0, 0, 0);
+      // K values e in [32 y / TNC, 32 (y + 1) / TNC) of this lane's 32 (x, h, v)
+#pragma unroll
+      for (int e = (8 * XT * y) / TNC; e < (8 * XT * (y + 1)) / TNC; ++e) {
+        const int x = e / 8, h = (e / 4) % 2, v = e % 4;
+        float k = rbf_exp2(gr[h][x][v], al[x] - lg * be[h][v], kappa);
+        asm volatile("" : "+v"(k));  // formed here, not sunk to its use
+        kn[x][4 * h + v] = (__bf16)k;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      o = on;
     }
-    if (u + 1 < steps) gram_exp(u + 1, kv);
+    if (!interior(un)) exp_edge(un, gr, kn);
+#pragma unroll
+    for (int x = 0; x < XT; ++x) kv[x] = kn[x];
   }
 #pragma unroll
   for (int x = 0; x < XT; ++x)
@@ -523,44 +579,65 @@ __global__ __launch_bounds__(64 * kLeftRingWaves, 1) void k_rbf_left_bf16_ring(R
     be[y] = (jp < g.N) ? -lg * g.nrm[jp] : 0.f;
   }
 
+  // Per-lane sources of the wave's copies at the first step; a step moves
+  // each by BK window rows (2 BK bytes along an Upsilon row, BK points rows,
+  // 4 BK bytes of norms).  Past the window's end the sources are clamped.
+  const char* dsrc[IPW];
+#pragma unroll
+  for (int s = 0; s < IPW; ++s) {
+    const int it = wave + NW * s;
+    if (it < I_US) {  // 16 map rows of the Upsilon slice
+      const int r = 16 * it + (lane >> 2), p = lane & 3;
+      int64_t row = p0 + r;
+      row = row < g.M ? row : g.M - 1;
+      dsrc[s] = reinterpret_cast<const char*>(g.U + row * g.ldu + g.row0 + 8 * (p ^ os_swz(r)));
+    } else if (it < I_US + I_XS) {  // RPI window rows
+      const int x = it - I_US;
+      const int r = RPI * x + lane / CPR, p = lane % CPR;
+      dsrc[s] = reinterpret_cast<const char*>(g.P + (g.row0 + r) * g.ldpb + 8 * (p ^ xs_swz<CPR>(r)));
+    } else {  // norms of the step's window rows (lanes >= 32 duplicate, unused)
+      dsrc[s] = reinterpret_cast<const char*>(g.nrm + g.row0 + (lane & 31));
+    }
+  }
   auto issue = [&](int u) {
     uint8_t* slot = ring + (u % NS) * SLOT;
     const int64_t i0 = (int64_t)u * BK;
+    const bool tail = i0 + BK > g.K;
 #pragma unroll
     for (int s = 0; s < IPW; ++s) {
       const int it = wave + NW * s;
-      if (it < I_US) {  // 16 map rows of the Upsilon slice
-        const int r = 16 * it + (lane >> 2), p = lane & 3;
-        int64_t row = p0 + r;
-        row = row < g.M ? row : g.M - 1;
-        const int64_t e = i0 + 8 * (p ^ os_swz(r));
-        const __bf16* src = g.U + row * g.ldu + g.row0 + (e < g.K ? e : 0);  // past the window: any finite data
+      if (it < I_US) {
+        const char* src = dsrc[s] + i0 * 2;
+        if (tail) {  // a chunk starting past the window reads the row's first chunk (finite, multiplied by zero)
+          const int r = 16 * it + (lane >> 2), p = lane & 3;
+          if (i0 + 8 * (p ^ os_swz(r)) >= g.K) src = dsrc[s] - 16 * (p ^ os_swz(r));
+        }
         __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + 1024 * it), 16, 0, 0);
-      } else if (it < I_US + I_XS) {  // RPI window rows
+      } else if (it < I_US + I_XS) {
         const int x = it - I_US;
-        const int r = RPI * x + lane / CPR, p = lane % CPR;
-        int64_t i = i0 + r;
-        i = i < g.K ? i : g.K - 1;
-        const __bf16* src = g.P + (g.row0 + i) * g.ldpb + 8 * (p ^ xs_swz<CPR>(r));
-        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + US_BYTES + 1024 * x), 16, 0,
-                                         0);
-      } else if (it == I_US + I_XS) {  // norms of the step's window rows
-        int64_t i = i0 + (lane & 31);
-        i = i < g.K ? i : g.K - 1;
-        __builtin_amdgcn_global_load_lds((const void*)(g.nrm + g.row0 + i), (SKEMA_LDS void*)(slot + NI_OFF), 4, 0,
-                                         0);
+        const char* src = dsrc[s] + i0 * (2 * DP);
+        if (tail) {
+          const int r = RPI * x + lane / CPR, p = lane % CPR;
+          int64_t i = i0 + r;
+          i = i < g.K ? i : g.K - 1;
+          src = reinterpret_cast<const char*>(g.P + (g.row0 + i) * g.ldpb + 8 * (p ^ xs_swz<CPR>(r)));
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + US_BYTES + 1024 * x), 16, 0, 0);
+      } else if (it == I_US + I_XS) {
+        const char* src = dsrc[s] + i0 * 4;
+        if (tail) {
+          int64_t i = i0 + (lane & 31);
+          i = i < g.K ? i : g.K - 1;
+          src = reinterpret_cast<const char*>(g.nrm + g.row0 + i);
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (SKEMA_LDS void*)(slot + NI_OFF), 4, 0, 0);
       }
     }
   };
 
-  // K of step u for the wave's point tiles: B operands of the sketch MFMA
-  // (lane l: point l%16 of tile y, window rows 8 (l/16) .. +7 of the step)
-  auto gram_exp = [&](int u, bf16x8 (&kb)[JT]) {
-    const uint8_t* slot = ring + (u % NS) * SLOT;
-    const uint8_t* xs = slot + US_BYTES;
-    const float* ni = reinterpret_cast<const float*>(slot + NI_OFF);
-    const int64_t i0 = (int64_t)u * BK;
-    f4v gr[2][JT];
+  // Gram tiles of step u (window rows x the wave's points)
+  auto gram = [&](int u, f4v (&gr)[2][JT]) {
+    const uint8_t* xs = ring + (u % NS) * SLOT + US_BYTES;
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
       const int r = 8 * (l16 >> 2) + 4 * x + (l16 & 3);
@@ -574,37 +651,42 @@ __global__ __launch_bounds__(64 * kLeftRingWaves, 1) void k_rbf_left_bf16_ring(R
         for (int y = 0; y < JT; ++y) gr[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, xb[y][c], gr[x][y], 0, 0, 0);
       }
     }
-    f4v al[2];
-    al[0] = *reinterpret_cast<const f4v*>(ni + 8 * lq);
-    al[1] = *reinterpret_cast<const f4v*>(ni + 8 * lq + 4);
+  };
+  auto interior = [&](int u) {
+    const int64_t i0 = (int64_t)u * BK, gi0 = g.row0 + i0;  // the step's window rows, as point indices
+    return pts_full && i0 + BK <= g.K && (gi0 + BK <= wj0 || gi0 >= wj0 + 16 * JT);
+  };
+  // K of step u (B operands: lane l holds point l%16 of tile y, window rows
+  // 8 (l/16) .. +7 of the step) with the range and diagonal checks
+  auto exp_edge = [&](int u, const f4v (&gr)[2][JT], bf16x8 (&kb)[JT]) {
+    const float* ni = reinterpret_cast<const float*>(ring + (u % NS) * SLOT + NI_OFF);
+    const int64_t i0 = (int64_t)u * BK;
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < JT; ++y) {
+      const int64_t jp = wj0 + 16 * y + l16;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) al[x][v] *= -lg;
-    const int64_t gi0 = g.row0 + i0;  // the step's window rows, as point indices
-    const bool interior = pts_full && i0 + BK <= g.K && (gi0 + BK <= wj0 || gi0 >= wj0 + 16 * JT);
-    if (interior) {
+      for (int x = 0; x < 2; ++x)
 #pragma unroll
-      for (int y = 0; y < JT; ++y)
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) kb[y][4 * x + v] = (__bf16)rbf_exp2(gr[x][y][v], al[x][v] + be[y], kappa);
-    } else {
-#pragma unroll
-      for (int y = 0; y < JT; ++y) {
-        const int64_t jp = wj0 + 16 * y + l16;
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int64_t i = i0 + 8 * lq + 4 * x + v;
-            float e = 0.f;
-            if (i < g.K && jp < g.N) e = (g.row0 + i == jp) ? 1.f : rbf_exp2(gr[x][y][v], al[x][v] + be[y], kappa);
-            kb[y][4 * x + v] = (__bf16)e;
-          }
-      }
+        for (int v = 0; v < 4; ++v) {
+          const int64_t i = i0 + 8 * lq + 4 * x + v;
+          float e = 0.f;
+          if (i < g.K && jp < g.N)
+            e = (g.row0 + i == jp) ? 1.f : rbf_exp2(gr[x][y][v], be[y] - lg * ni[8 * lq + 4 * x + v], kappa);
+          kb[y][4 * x + v] = (__bf16)e;
+        }
     }
+  };
+  auto exp_fast = [&](int u, const f4v (&gr)[2][JT], bf16x8 (&kb)[JT]) {
+    const float* ni = reinterpret_cast<const float*>(ring + (u % NS) * SLOT + NI_OFF);
+    const f4v m0 = *reinterpret_cast<const f4v*>(ni + 8 * lq);
+    const f4v m1 = *reinterpret_cast<const f4v*>(ni + 8 * lq + 4);
+#pragma unroll
+    for (int y = 0; y < JT; ++y)
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          kb[y][4 * x + v] = (__bf16)rbf_exp2(gr[x][y][v], be[y] - lg * (x ? m1[v] : m0[v]), kappa);
   };
 
   f4v acc[JT][TP];
@@ -613,25 +695,59 @@ __global__ __launch_bounds__(64 * kLeftRingWaves, 1) void k_rbf_left_bf16_ring(R
 #pragma unroll
     for (int z = 0; z < TP; ++z) acc[y][z] = f4v{0.f, 0.f, 0.f, 0.f};
 
+  bf16x8 kb[JT];
 #pragma unroll
   for (int p = 0; p < NS - 1; ++p)
     if (p < steps) issue(p);
+  if (steps > 0) {
+    wait_vm_upto(((NS - 2) < (steps - 1) ? (NS - 2) : (steps - 1)) * dw);  // step 0 complete
+    lds_barrier();
+    f4v gr[2][JT];
+    gram(0, gr);
+    if (interior(0)) exp_fast(0, gr, kb);
+    else exp_edge(0, gr, kb);
+  }
   for (int u = 0; u < steps; ++u) {
-    // step u complete: at most steps u+1 .. min(u+NS-2, steps-1) in flight
-    const int last = (u + NS - 2 < steps - 1) ? u + NS - 2 : steps - 1;
-    wait_vm_upto((last - u) * dw);
-    lds_barrier();  // every wave's copies of step u landed; slot (u-1) % NS is free
+    if (u + 1 < steps) {  // step u + 1 complete: at most steps u+2 .. min(u+NS-2, steps-1) in flight
+      const int last = (u + NS - 2 < steps - 1) ? u + NS - 2 : steps - 1;
+      wait_vm_upto((last - (u + 1)) * dw);
+    }
+    lds_barrier();  // every wave's copies of step u+1 landed; slot (u-1) % NS is free
     if (u + NS - 1 < steps) issue(u + NS - 1);
-    bf16x8 kb[JT];
-    gram_exp(u, kb);
+    const int un = (u + 1 < steps) ? u + 1 : u;  // the last step forms a Gram it does not use
+    f4v gr[2][JT];
+    gram(un, gr);
+    // the sketch MFMAs of step u, map-row tile by map-row tile, each
+    // followed by a share of the exponentials of step u+1
     const uint8_t* us = ring + (u % NS) * SLOT;
+    const float* ni = reinterpret_cast<const float*>(ring + (un % NS) * SLOT + NI_OFF);
+    f4v mi[2];
+    mi[0] = *reinterpret_cast<const f4v*>(ni + 8 * lq);
+    mi[1] = *reinterpret_cast<const f4v*>(ni + 8 * lq + 4);
+    bf16x8 kn[JT];
+    bf16x8 a = *reinterpret_cast<const bf16x8*>(us + 64 * l16 + 16 * (lq ^ os_swz(l16)));
 #pragma unroll
     for (int z = 0; z < TP; ++z) {
-      const int r = 16 * z + l16;
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(us + 64 * r + 16 * (lq ^ os_swz(r)));
+      bf16x8 an = a;
+      if (z + 1 < TP) {
+        const int r = 16 * (z + 1) + l16;
+        an = *reinterpret_cast<const bf16x8*>(us + 64 * r + 16 * (lq ^ os_swz(r)));
+      }
 #pragma unroll
       for (int y = 0; y < JT; ++y) acc[y][z] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, kb[y], acc[y][z], 0, 0, 0);
+#pragma unroll
+      for (int e = (8 * JT * z) / TP; e < (8 * JT * (z + 1)) / TP; ++e) {
+        const int y = e / 8, x = (e / 4) % 2, v = e % 4;
+        float k = rbf_exp2(gr[x][y][v], be[y] - lg * mi[x][v], kappa);
+        asm volatile("" : "+v"(k));  // formed here, not sunk to its use
+        kn[y][4 * x + v] = (__bf16)k;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      a = an;
     }
+    if (!interior(un)) exp_edge(un, gr, kn);
+#pragma unroll
+    for (int y = 0; y < JT; ++y) kb[y] = kn[y];
   }
 #pragma unroll
   for (int y = 0; y < JT; ++y) {
