@@ -155,7 +155,10 @@ void check_window(const Sketch& sk, int64_t row0, int64_t rows) {
 
 // Reads rowptr[0] and rowptr[rows] of a device CSR window and checks that
 // col/val cover them.
-void check_csr_window(const int64_t* rowptr, const int32_t* col, const double* val, int64_t rows, hipStream_t st) {
+// ends_out (optional): rowptr[0] and rowptr[rows], read here anyway, so the
+// caller's own validation needs no second round trip.
+void check_csr_window(const int64_t* rowptr, const int32_t* col, const double* val, int64_t rows, hipStream_t st,
+                      int64_t* ends_out = nullptr) {
   check_dev(rowptr, (size_t)(rows + 1) * sizeof(int64_t), "rowptr");
   int64_t ends[2];
   SKEMA_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -165,6 +168,10 @@ void check_csr_window(const int64_t* rowptr, const int32_t* col, const double* v
   if (ends[1] > 0) {
     check_dev(col, (size_t)ends[1] * sizeof(int32_t), "col");
     check_dev(val, (size_t)ends[1] * sizeof(double), "val");
+  }
+  if (ends_out) {
+    ends_out[0] = ends[0];
+    ends_out[1] = ends[1];
   }
 }
 
@@ -607,11 +614,12 @@ int skema_linear_update_csr(skema_sketch* h, const int64_t* rowptr, const int32_
     Sketch& sk = h->s;
     set_device(sk.ctx);
     SKEMA_REQUIRE(sk.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "linear_update_csr on an RBF sketch");
-    check_csr_window(rowptr, col, val, sk.p.m, sk.ctx->stream);
+    int64_t ends[2];
+    check_csr_window(rowptr, col, val, sk.p.m, sk.ctx->stream, ends);
     // the window loop from a zero state in one pass over the whole CSR; the
     // pass writes every entry of X, Y and Z, so the state is not cleared first
     sk.finalized = false;
-    update_csr_all(sk, rowptr, col, val);
+    update_csr_all(sk, rowptr, col, val, ends);
     sk.ctx->sync_if_needed();
   });
 }

@@ -737,7 +737,7 @@ void left_dense(hipStream_t st, const CscWindow& H, const double* M, int64_t ldm
 
 }  // namespace
 
-void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const double* val) {
+void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const double* val, const int64_t* ends) {
   Ctx* c = sk.ctx;
   hipStream_t st = c->stream;
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n;
@@ -770,7 +770,7 @@ void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const
     bool done = false;
     if (sk.sparse_maps) {
       // the fixed-point products take the column entries in any order
-      const int64_t nnz = validate_csr_window(st, rowptr, col, m, n, flag);
+      const int64_t nnz = validate_csr_window(st, rowptr, col, m, n, flag, ends);
       done = csc_by_buckets(st, rowptr, col, val, m, n, nnz, sk.win, H);
     }
     if (!done) H = csr_window_to_csc(st, rowptr, col, val, m, n, sk.win, /*sorted=*/true);

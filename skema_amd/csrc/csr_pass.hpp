@@ -5,6 +5,7 @@
 
 namespace skema {
 
-void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const double* val);
+// ends: rowptr[0] and rowptr[m] as already read by the caller.
+void update_csr_all(Sketch& sk, const int64_t* rowptr, const int32_t* col, const double* val, const int64_t* ends);
 
 }  // namespace skema

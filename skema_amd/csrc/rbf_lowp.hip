@@ -303,7 +303,10 @@ __global__ __launch_bounds__(64 * (kTallRows / (16 * kTallXT)), 1) void k_rbf_ri
   constexpr int IPW = (ITEMS + NW - 1) / NW;        // copy items per wave, at most
   static_assert(OS_BYTES % 1024 == 0 && XS_BYTES % 1024 == 0 && (NS - 2) * IPW <= 62, "copy items");
   __shared__ __attribute__((aligned(1024))) uint8_t ring[NS * SLOT];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // wave index as a scalar: the copy items, their counts and the vmcnt waits
+  // below branch on it, and a per-lane value would turn every wave-uniform
+  // branch into exec-masked code that runs each wait case in turn
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int l16 = lane & 15, lq = lane >> 4;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t m0 = (int64_t)(t % g.tiles_m) * BM;
@@ -557,7 +560,10 @@ __global__ __launch_bounds__(64 * kLeftRingWaves, 1) void k_rbf_left_bf16_ring(R
   constexpr int IPW = (ITEMS + NW - 1) / NW;
   static_assert(US_BYTES % 1024 == 0 && XS_BYTES % 1024 == 0 && (NS - 2) * IPW <= 62, "copy items");
   __shared__ __attribute__((aligned(1024))) uint8_t ring[NS * SLOT];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // wave index as a scalar: the copy items, their counts and the vmcnt waits
+  // below branch on it, and a per-lane value would turn every wave-uniform
+  // branch into exec-masked code that runs each wait case in turn
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int l16 = lane & 15, lq = lane >> 4;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t wj0 = (int64_t)t * 16 * JT * NW + 16 * JT * wave;  // the wave's first point

@@ -414,11 +414,16 @@ void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, co
 }
 
 int64_t validate_csr_window(hipStream_t st, const int64_t* rowptr, const int32_t* col, int64_t rows, int64_t cols,
-                            int* flag) {
+                            int* flag, const int64_t* known) {
   int64_t ends[2] = {0, 0};
-  SKEMA_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  SKEMA_HIP(hipMemcpyAsync(&ends[1], rowptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  SKEMA_HIP(hipStreamSynchronize(st));
+  if (known) {
+    ends[0] = known[0];
+    ends[1] = known[1];
+  } else {
+    SKEMA_HIP(hipMemcpyAsync(&ends[0], rowptr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SKEMA_HIP(hipMemcpyAsync(&ends[1], rowptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SKEMA_HIP(hipStreamSynchronize(st));
+  }
   const int64_t nnz = ends[1] - ends[0];
   SKEMA_REQUIRE(nnz >= 0 && nnz < ((int64_t)1 << 31), kErrArg, "CSR window: bad rowptr range");
   // Reject malformed windows before any kernel indexes with them.

@@ -21,9 +21,10 @@ void spmm_dense_csrT(hipStream_t st, int64_t M, const double* B, int64_t ldb, co
 
 // Checks a device CSR window (rowptr non-decreasing, column indices in
 // [0, cols)) and returns its nonzero count; flag is one device int of
-// scratch.  Throws on a malformed window before any kernel indexes with it.
+// scratch; ends (optional): rowptr[0] and rowptr[rows] already read on the
+// host.  Throws on a malformed window before any kernel indexes with it.
 int64_t validate_csr_window(hipStream_t st, const int64_t* rowptr, const int32_t* col, int64_t rows, int64_t cols,
-                            int* flag);
+                            int* flag, const int64_t* ends = nullptr);
 
 // Column-compressed copy of a CSR window (rowptr may start at an offset).
 struct CscWindow {
