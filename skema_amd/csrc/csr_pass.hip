@@ -234,6 +234,7 @@ __global__ __launch_bounds__(1024) void k_l2_place(const int* base, int nt, int 
 // per column (rows may repeat) and |v| <= vmax.
 struct SpMapCols {
   const uint32_t* ell = nullptr;
+  const uint16_t* ell16 = nullptr;  // the same entries in 16 bits, when the rows fit
   int z = 0;
   double mag = 1.0;
   const int64_t* rp = nullptr;
@@ -250,9 +251,22 @@ __device__ inline double wave_max(double x) {
 
 // ELL column c of M (z <= ZM entries): the ZM loads are issued together so
 // one memory latency covers the whole column.
-template <int ZM>
+// E16: the map has the 16-bit copy (chosen at launch, so each kernel
+// instance carries one load path); its entries stay packed two to a word in
+// u[0 .. ZM/2) and ell_entry unpacks them at use.
+template <int ZM, bool E16 = false>
 __device__ inline void ell_load(const SpMapCols& M, int64_t c, uint32_t (&u)[ZM]) {
-  if (M.z == ZM) {  // the column is ZM / 4 aligned 16-byte words: one request each
+  if (E16 && M.z == ZM) {  // ZM / 8 aligned 16-byte words of 16-bit entries
+    const uint4* src = reinterpret_cast<const uint4*>(M.ell16 + c * ZM);
+#pragma unroll
+    for (int w = 0; w < ZM / 8; ++w) {
+      const uint4 x = src[w];
+      u[4 * w] = x.x;
+      u[4 * w + 1] = x.y;
+      u[4 * w + 2] = x.z;
+      u[4 * w + 3] = x.w;
+    }
+  } else if (!E16 && M.z == ZM) {  // the column is ZM / 4 aligned 16-byte words: one request each
     const uint4* src = reinterpret_cast<const uint4*>(M.ell + c * ZM);
 #pragma unroll
     for (int w = 0; w < ZM / 4; ++w) {
@@ -268,17 +282,27 @@ __device__ inline void ell_load(const SpMapCols& M, int64_t c, uint32_t (&u)[ZM]
   }
 }
 
+// Entry f of a loaded column as row | sign << 31 (E16 only when the column
+// was loaded packed, i.e. z == ZM).
+template <int ZM, bool E16>
+__device__ inline uint32_t ell_entry(const uint32_t (&u)[ZM], int z, int f) {
+  if (E16 && z == ZM) {
+    const uint32_t h = (u[f / 2] >> (16 * (f % 2))) & 0xffffu;
+    return (h & 0x7fffu) | ((h & 0x8000u) << 16);
+  }
+  return u[f];
+}
+
 // Adds the fixed-point term V, signed by each entry, into the accumulators
 // acc[p - p0] of the entries' rows p in [p0, p0 + pn).
-template <int ZM>
+template <int ZM, bool E16>
 __device__ inline void ell_add(const uint32_t (&u)[ZM], int z, long long V, int64_t p0, int64_t pn,
                                unsigned long long* acc) {
 #pragma unroll
   for (int f = 0; f < ZM; ++f) {
-    const int64_t p = (int64_t)(u[f] & 0x7fffffffu) - p0;
-    if (f < z && p >= 0 && p < pn) {
-      fx_add_lds(acc + p, (u[f] >> 31) ? -V : V);
-    }
+    const uint32_t e = ell_entry<ZM, E16>(u, z, f);
+    const int64_t p = (int64_t)(e & 0x7fffffffu) - p0;
+    if (f < z && p >= 0 && p < pn) fx_add_lds(acc + p, (e >> 31) ? -V : V);
   }
 }
 
@@ -308,7 +332,7 @@ __device__ inline void csr_add(const SpMapCols& M, int64_t c, double t, const Fx
 // of a block never wait for each other.
 constexpr int kLeftCache = 4;
 
-template <int ZM>
+template <int ZM, bool E16>
 __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int32_t* crow, const double* cval,
                                                 int64_t ncols, const double* cw, int64_t window, SpMapCols M1,
                                                 int64_t p1, int64_t n1, double* out1, int64_t ld1, SpMapCols M2,
@@ -345,8 +369,8 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
 #pragma unroll
         for (int q = 0; q < kLeftCache; ++q) {
           if (iq[q] >= 0) {
-            ell_load<ZM>(M1, iq[q], u1[q]);
-            if (n2) ell_load<ZM>(M2, iq[q], u2[q]);
+            ell_load<ZM, E16>(M1, iq[q], u1[q]);
+            if (n2) ell_load<ZM, E16>(M2, iq[q], u2[q]);
           }
         }
       }
@@ -373,19 +397,19 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
           for (int q = 0; q < kLeftCache; ++q) {
             if (iq[q] >= 0) {
               const long long V = fx_term(tq[q], fs);
-              ell_add<ZM>(u1[q], M1.z, V, p1, n1, acc);
-              if (n2) ell_add<ZM>(u2[q], M2.z, V, p2, n2, acc + n1);
+              ell_add<ZM, E16>(u1[q], M1.z, V, p1, n1, acc);
+              if (n2) ell_add<ZM, E16>(u2[q], M2.z, V, p2, n2, acc + n1);
             }
           }
           for (int64_t e = e0 + 64 * kLeftCache + lane; e < e1; e += 64) {
             const int32_t i = crow[e];
             const double t = cval[e] * cw[(uint32_t)i / wdw];
             uint32_t v1[ZM], v2[ZM];
-            ell_load<ZM>(M1, i, v1);
-            if (n2) ell_load<ZM>(M2, i, v2);
+            ell_load<ZM, E16>(M1, i, v1);
+            if (n2) ell_load<ZM, E16>(M2, i, v2);
             const long long V = fx_term(t, fs);
-            ell_add<ZM>(v1, M1.z, V, p1, n1, acc);
-            if (n2) ell_add<ZM>(v2, M2.z, V, p2, n2, acc + n1);
+            ell_add<ZM, E16>(v1, M1.z, V, p1, n1, acc);
+            if (n2) ell_add<ZM, E16>(v2, M2.z, V, p2, n2, acc + n1);
           }
         } else {
 #pragma unroll
@@ -425,11 +449,15 @@ constexpr int kRightRows = 32;
 constexpr int kRightSlots = 128;
 constexpr int kRightCache = 3;
 
-template <int ZM>
+template <int ZM, bool E16>
 __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const int32_t* col, const double* val,
                                                  int64_t rows, SpMapCols M, int64_t p0, int64_t pn, double eta,
                                                  double* Y, int64_t ldy) {
-  extern __shared__ unsigned long long fxacc[];  // pn x kRightRows
+  // kRightRows x ps accumulators, row-major with an odd stride: the lanes of
+  // a wave mostly work on one row, so their slots p fall in distinct banks
+  // (a slot-major layout put every lane of a row in the same bank)
+  extern __shared__ unsigned long long fxacc[];
+  const int64_t ps = pn | 1;
   __shared__ int64_t rp[kRightRows + 1];
   __shared__ unsigned long long rmax[kRightRows];  // bits of the row's largest |a| (non-negative doubles order as integers)
   __shared__ FxScale rs[kRightRows];
@@ -437,7 +465,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
   const int tid = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * kRightRows;
   const int nr = (int)((r0 + kRightRows < rows) ? kRightRows : rows - r0);
-  for (int q = tid; q < pn * kRightRows; q += 256) fxacc[q] = 0ull;
+  for (int q = tid; q < ps * kRightRows; q += 256) fxacc[q] = 0ull;
   for (int i = tid; i <= nr; i += 256) rp[i] = rowptr[r0 + i];
   if (tid < kRightRows) rmax[tid] = 0ull;
   __syncthreads();
@@ -464,7 +492,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
   if constexpr (ELL) {
 #pragma unroll
     for (int q = 0; q < kRightCache; ++q)
-      if (jq[q] >= 0) ell_load<ZM>(M, jq[q], u[q]);
+      if (jq[q] >= 0) ell_load<ZM, E16>(M, jq[q], u[q]);
   }
 #pragma unroll
   for (int q = 0; q < kRightCache; ++q)
@@ -490,10 +518,9 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
     const long long V = fx_term(t, fs);
 #pragma unroll
     for (int f = 0; f < (ZM > 0 ? ZM : 1); ++f) {
-      const int64_t p = (int64_t)(uu[f] & 0x7fffffffu) - p0;
-      if (f < M.z && p >= 0 && p < pn) {
-        fx_add_lds(fxacc + p * kRightRows + rr, (uu[f] >> 31) ? -V : V);
-      }
+      const uint32_t e = ell_entry<(ZM > 0 ? ZM : 1), E16>(uu, M.z, f);
+      const int64_t p = (int64_t)(e & 0x7fffffffu) - p0;
+      if (f < M.z && p >= 0 && p < pn) fx_add_lds(fxacc + rr * ps + p, (e >> 31) ? -V : V);
     }
   };
   auto add_csr = [&](int64_t j, double t, int rr) {
@@ -502,7 +529,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
     for (int64_t f = M.rp[j]; f < M.rp[j + 1]; ++f) {
       const int64_t p = (int64_t)M.ci[f] - p0;
       if (p >= 0 && p < pn) {
-        fx_add_lds(fxacc + p * kRightRows + rr, fx_term(t * M.v[f], fs));
+        fx_add_lds(fxacc + rr * ps + p, fx_term(t * M.v[f], fs));
       }
     }
   };
@@ -516,7 +543,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
     while (rp[r + 1] <= e) ++r;
     if constexpr (ELL) {
       uint32_t uu[ZM > 0 ? ZM : 1];
-      ell_load<ZM>(M, col[e], uu);
+      ell_load<ZM, E16>(M, col[e], uu);
       add_ell(uu, val[e], r);
     } else {
       add_csr(col[e], val[e], r);
@@ -528,7 +555,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
     const int rr = q % kRightRows;
     const int64_t p = q / kRightRows;
     if (rr < nr)
-      Y[(r0 + rr) + (p0 + p) * ldy] = scale * fx_value((long long)fxacc[q], rs[rr]);
+      Y[(r0 + rr) + (p0 + p) * ldy] = scale * fx_value((long long)fxacc[rr * ps + p], rs[rr]);
   }
 }
 
@@ -599,6 +626,7 @@ SpMapCols map_cols(const Map& M) {
   SpMapCols c;
   if (M.ell_zeta > 0) {
     c.ell = M.ell.as<uint32_t>();
+    c.ell16 = M.ell16.p ? M.ell16.as<uint16_t>() : nullptr;
     c.z = M.ell_zeta;
     c.mag = M.ell_mag;
     c.vmax = M.ell_mag;
@@ -626,12 +654,14 @@ void launch_left_fx(hipStream_t st, int zm, const CscWindow& H, const double* cw
   const int64_t groups = cdiv(H.cols, 4);
   const unsigned grid = (unsigned)std::min<int64_t>(groups, 8192);
   const size_t lds = (size_t)4 * (n1 + n2) * sizeof(unsigned long long);
-#define SKEMA_LEFT_FX(ZM)                                                                                         \
-  hipLaunchKernelGGL(k_left_fx<ZM>, dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, M1, \
-                     p1, n1, o1, ld1, M2, p2, n2, o2, ld2)
-  if (zm == 8) SKEMA_LEFT_FX(8);
-  else if (zm == 16) SKEMA_LEFT_FX(16);
-  else SKEMA_LEFT_FX(0);
+#define SKEMA_LEFT_FX(ZM, E16)                                                                                    \
+  hipLaunchKernelGGL((k_left_fx<ZM, E16>), dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, \
+                     M1, p1, n1, o1, ld1, M2, p2, n2, o2, ld2)
+  // the 32-bit entries here: the packed ones cost this kernel registers
+  // (its cached map columns) below the four waves per SIMD it runs at
+  if (zm == 8) SKEMA_LEFT_FX(8, false);
+  else if (zm == 16) SKEMA_LEFT_FX(16, false);
+  else SKEMA_LEFT_FX(0, false);
 #undef SKEMA_LEFT_FX
   check_launch("k_left_fx");
 }
@@ -661,14 +691,16 @@ void right_sparse_fx(hipStream_t st, const int64_t* rowptr, const int32_t* col, 
   const dim3 grid((unsigned)cdiv(m, kRightRows));
   for (int64_t p0 = 0; p0 < k; p0 += kRightSlots) {
     const int64_t pn = std::min<int64_t>(kRightSlots, k - p0);
-    const size_t lds = (size_t)pn * kRightRows * sizeof(unsigned long long);
+    const size_t lds = (size_t)(pn | 1) * kRightRows * sizeof(unsigned long long);
     const int zm = zm_class(M.ell_zeta);
-    if (zm == 8)
-      hipLaunchKernelGGL(k_right_fx<8>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
-    else if (zm == 16)
-      hipLaunchKernelGGL(k_right_fx<16>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
-    else
-      hipLaunchKernelGGL(k_right_fx<0>, grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy);
+#define SKEMA_RIGHT_FX(ZM, E16) \
+  hipLaunchKernelGGL((k_right_fx<ZM, E16>), grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy)
+    if (zm == 8 && c.ell16) SKEMA_RIGHT_FX(8, true);
+    else if (zm == 8) SKEMA_RIGHT_FX(8, false);
+    else if (zm == 16 && c.ell16) SKEMA_RIGHT_FX(16, true);
+    else if (zm == 16) SKEMA_RIGHT_FX(16, false);
+    else SKEMA_RIGHT_FX(0, false);
+#undef SKEMA_RIGHT_FX
     check_launch("k_right_fx");
   }
 }

@@ -151,6 +151,14 @@ Map* make_sparse_map(Ctx* ctx, int64_t nrow, int64_t ncol, int64_t nnz, const in
       SKEMA_HIP(hipStreamSynchronize(ctx->stream));
       m->ell_zeta = (int)z;
       m->ell_mag = mag;
+      if ((st ? ncol : nrow) <= 0x7fff) {  // every row index fits 15 bits
+        std::vector<uint16_t> h16((size_t)nnz);
+        for (int64_t e = 0; e < nnz; ++e) h16[(size_t)e] = (uint16_t)(c[e] | (val[e] < 0.0 ? 0x8000 : 0));
+        m->ell16.alloc((size_t)nnz * sizeof(uint16_t));
+        SKEMA_HIP(hipMemcpyAsync(m->ell16.p, h16.data(), (size_t)nnz * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                 ctx->stream));
+        SKEMA_HIP(hipStreamSynchronize(ctx->stream));
+      }
     }
   }
   return m.release();

@@ -98,6 +98,9 @@ struct Map {
   DevBuf ell;
   int ell_zeta = 0;
   double ell_mag = 0.0;
+  // The same entries in 16 bits (row | sign << 15) when every row is below
+  // 2^15: half the bytes, one 16-byte load per column of 8.  Empty otherwise.
+  DevBuf ell16;
   // Sparse maps: the largest |entry| and the most entries of one logical
   // column (bounds for exact fixed-point accumulation).
   double sp_vmax = 0.0;
