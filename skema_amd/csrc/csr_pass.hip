@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void k_l1_scatter(const int64_t* rowptr, int64
   __shared__ int wsum[4];
   __shared__ int64_t rp[kL1Rows + 1];
   __shared__ CooRec sb[kL1Batch];
+  __shared__ uint8_t rowof[kL1Batch];  // row (within the block) of each entry of the batch
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int64_t r0 = (int64_t)blockIdx.x * kL1Rows;
   const int nr = (int)((r0 + kL1Rows < rows) ? kL1Rows : rows - r0);
@@ -133,10 +134,15 @@ __global__ __launch_bounds__(256) void k_l1_scatter(const int64_t* rowptr, int64
   for (int i = tid; i <= nr; i += 256) rp[i] = rowptr[r0 + i];
   __syncthreads();
   const int64_t e_end = rp[nr];
-  int r = 0;  // row of the thread's current nonzero (advances monotonically)
+  static_assert(kL1Rows <= 256, "rows of a block fit a byte");
   for (int64_t b0 = rp[0]; b0 < e_end; b0 += kL1Batch) {
     const int nbat = (int)((e_end - b0 < kL1Batch) ? e_end - b0 : kL1Batch);
     cnt[tid] = 0;  // nt <= 256 = blockDim
+    if (tid < nr) {  // thread t labels the entries of row t that fall in this batch
+      const int64_t a = rp[tid] > b0 ? rp[tid] : b0;
+      const int64_t b = rp[tid + 1] < b0 + nbat ? rp[tid + 1] : b0 + nbat;
+      for (int64_t e = a; e < b; ++e) rowof[e - b0] = (uint8_t)tid;
+    }
     __syncthreads();
     CooRec rec[E];
     int slot[E];
@@ -145,8 +151,7 @@ __global__ __launch_bounds__(256) void k_l1_scatter(const int64_t* rowptr, int64
       const int x = tid + 256 * q;
       if (x < nbat) {
         const int64_t e = b0 + x;
-        while (rp[r + 1] <= e) ++r;
-        rec[q].row = (int32_t)(r0 + r);
+        rec[q].row = (int32_t)(r0 + rowof[x]);
         rec[q].col = col[e];
         rec[q].val = val[e];
       }
@@ -294,15 +299,19 @@ __device__ inline uint32_t ell_entry(const uint32_t (&u)[ZM], int z, int f) {
 }
 
 // Adds the fixed-point term V, signed by each entry, into the accumulators
-// acc[p - p0] of the entries' rows p in [p0, p0 + pn).
+// acc[p - p0] of the entries' rows p in [p0, p0 + pn).  Branch-free: an
+// entry outside the range (or past z) adds into acc[dummy], a slot nobody
+// reads, so every lane issues every atomic with 32-bit index arithmetic.
 template <int ZM, bool E16>
-__device__ inline void ell_add(const uint32_t (&u)[ZM], int z, long long V, int64_t p0, int64_t pn,
+__device__ inline void ell_add(const uint32_t (&u)[ZM], int z, long long V, int p0, int pn, int dummy,
                                unsigned long long* acc) {
+  const unsigned long long vp = (unsigned long long)V, vn = (unsigned long long)(-V);
 #pragma unroll
   for (int f = 0; f < ZM; ++f) {
     const uint32_t e = ell_entry<ZM, E16>(u, z, f);
-    const int64_t p = (int64_t)(e & 0x7fffffffu) - p0;
-    if (f < z && p >= 0 && p < pn) fx_add_lds(acc + p, (e >> 31) ? -V : V);
+    const int p = (int)(e & 0x7fffffffu) - p0;
+    const bool ok = f < z && (unsigned)p < (unsigned)pn;
+    atomicAdd(acc + (ok ? p : dummy), (e >> 31) ? vn : vp);
   }
 }
 
@@ -337,11 +346,12 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
                                                 int64_t ncols, const double* cw, int64_t window, SpMapCols M1,
                                                 int64_t p1, int64_t n1, double* out1, int64_t ld1, SpMapCols M2,
                                                 int64_t p2, int64_t n2, double* out2, int64_t ld2) {
-  extern __shared__ unsigned long long fxacc[];  // 4 waves x (n1 + n2)
+  extern __shared__ unsigned long long fxacc[];  // 4 waves x (n1 + n2 + 1): the last slot of a wave is its dummy
   constexpr bool ELL = ZM > 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t na = n1 + n2;
-  unsigned long long* acc = fxacc + (int64_t)wave * na;
+  unsigned long long* acc = fxacc + (int64_t)wave * (na + 1);
+  const int dummy1 = (int)na, dummy2 = (int)n2;  // acc[na], seen from acc and from acc + n1
   const uint32_t wdw = (uint32_t)window;
   for (int64_t j = 4 * (int64_t)blockIdx.x + wave; j < ncols; j += 4 * (int64_t)gridDim.x) {
     for (int64_t q = lane; q < na; q += 64) acc[q] = 0ull;
@@ -397,8 +407,8 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
           for (int q = 0; q < kLeftCache; ++q) {
             if (iq[q] >= 0) {
               const long long V = fx_term(tq[q], fs);
-              ell_add<ZM, E16>(u1[q], M1.z, V, p1, n1, acc);
-              if (n2) ell_add<ZM, E16>(u2[q], M2.z, V, p2, n2, acc + n1);
+              ell_add<ZM, E16>(u1[q], M1.z, V, (int)p1, (int)n1, dummy1, acc);
+              if (n2) ell_add<ZM, E16>(u2[q], M2.z, V, (int)p2, (int)n2, dummy2, acc + n1);
             }
           }
           for (int64_t e = e0 + 64 * kLeftCache + lane; e < e1; e += 64) {
@@ -408,8 +418,8 @@ __global__ __launch_bounds__(256) void k_left_fx(const int64_t* cptr, const int3
             ell_load<ZM, E16>(M1, i, v1);
             if (n2) ell_load<ZM, E16>(M2, i, v2);
             const long long V = fx_term(t, fs);
-            ell_add<ZM, E16>(v1, M1.z, V, p1, n1, acc);
-            if (n2) ell_add<ZM, E16>(v2, M2.z, V, p2, n2, acc + n1);
+            ell_add<ZM, E16>(v1, M1.z, V, (int)p1, (int)n1, dummy1, acc);
+            if (n2) ell_add<ZM, E16>(v2, M2.z, V, (int)p2, (int)n2, dummy2, acc + n1);
           }
         } else {
 #pragma unroll
@@ -458,9 +468,11 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
   // (a slot-major layout put every lane of a row in the same bank)
   extern __shared__ unsigned long long fxacc[];
   const int64_t ps = pn | 1;
+  const int dummy = kRightRows * (int)ps;  // one slot past the rows, never read
   __shared__ int64_t rp[kRightRows + 1];
   __shared__ unsigned long long rmax[kRightRows];  // bits of the row's largest |a| (non-negative doubles order as integers)
   __shared__ FxScale rs[kRightRows];
+  __shared__ uint8_t rowof[256 * kRightCache];  // row of each cached entry
   constexpr bool ELL = ZM > 0;
   const int tid = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * kRightRows;
@@ -470,18 +482,22 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
   if (tid < kRightRows) rmax[tid] = 0ull;
   __syncthreads();
   const int64_t e_end = rp[nr];
+  if (tid < nr) {  // thread t labels the cached entries of row t
+    const int64_t c1 = rp[0] + 256 * kRightCache;
+    const int64_t b = rp[tid + 1] < c1 ? rp[tid + 1] : c1;
+    for (int64_t e = rp[tid]; e < b; ++e) rowof[e - rp[0]] = (uint8_t)tid;
+  }
+  __syncthreads();
   int64_t jq[kRightCache];
   double tq[kRightCache];
   int rq[kRightCache];
-  int r = 0;
 #pragma unroll
   for (int q = 0; q < kRightCache; ++q) {
     const int64_t e = rp[0] + tid + 256 * q;
     if (e < e_end) {
-      while (rp[r + 1] <= e) ++r;
       jq[q] = col[e];
       tq[q] = val[e];
-      rq[q] = r;
+      rq[q] = rowof[tid + 256 * q];
     } else {
       jq[q] = -1;
       tq[q] = 0.0;
@@ -497,6 +513,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
 #pragma unroll
   for (int q = 0; q < kRightCache; ++q)
     if (jq[q] >= 0) atomicMax(&rmax[rq[q]], (unsigned long long)__double_as_longlong(fabs(tq[q])));  // NaN orders above inf
+  int r = rq[kRightCache - 1];  // row of the thread's last cached entry (rows past it follow in order)
   {
     int rr = r;
     for (int64_t e = rp[0] + tid + 256 * kRightCache; e < e_end; e += 256) {
@@ -516,12 +533,7 @@ __global__ __launch_bounds__(256) void k_right_fx(const int64_t* rowptr, const i
     const FxScale fs = rs[rr];
     if (!fs.finite) return;
     const long long V = fx_term(t, fs);
-#pragma unroll
-    for (int f = 0; f < (ZM > 0 ? ZM : 1); ++f) {
-      const uint32_t e = ell_entry<(ZM > 0 ? ZM : 1), E16>(uu, M.z, f);
-      const int64_t p = (int64_t)(e & 0x7fffffffu) - p0;
-      if (f < M.z && p >= 0 && p < pn) fx_add_lds(fxacc + rr * ps + p, (e >> 31) ? -V : V);
-    }
+    ell_add<(ZM > 0 ? ZM : 1), E16>(uu, M.z, V, (int)p0, (int)pn, dummy - rr * (int)ps, fxacc + rr * ps);
   };
   auto add_csr = [&](int64_t j, double t, int rr) {
     const FxScale fs = rs[rr];
@@ -653,7 +665,7 @@ void launch_left_fx(hipStream_t st, int zm, const CscWindow& H, const double* cw
                     int64_t p2, int64_t n2, double* o2, int64_t ld2) {
   const int64_t groups = cdiv(H.cols, 4);
   const unsigned grid = (unsigned)std::min<int64_t>(groups, 8192);
-  const size_t lds = (size_t)4 * (n1 + n2) * sizeof(unsigned long long);
+  const size_t lds = (size_t)4 * (n1 + n2 + 1) * sizeof(unsigned long long);
 #define SKEMA_LEFT_FX(ZM, E16)                                                                                    \
   hipLaunchKernelGGL((k_left_fx<ZM, E16>), dim3(grid), dim3(256), lds, st, H.ptr, H.row, H.val, H.cols, cw, window, \
                      M1, p1, n1, o1, ld1, M2, p2, n2, o2, ld2)
@@ -691,7 +703,7 @@ void right_sparse_fx(hipStream_t st, const int64_t* rowptr, const int32_t* col, 
   const dim3 grid((unsigned)cdiv(m, kRightRows));
   for (int64_t p0 = 0; p0 < k; p0 += kRightSlots) {
     const int64_t pn = std::min<int64_t>(kRightSlots, k - p0);
-    const size_t lds = (size_t)(pn | 1) * kRightRows * sizeof(unsigned long long);
+    const size_t lds = ((size_t)(pn | 1) * kRightRows + 1) * sizeof(unsigned long long);
     const int zm = zm_class(M.ell_zeta);
 #define SKEMA_RIGHT_FX(ZM, E16) \
   hipLaunchKernelGGL((k_right_fx<ZM, E16>), grid, dim3(256), lds, st, rowptr, col, val, m, c, p0, pn, eta, Y, ldy)
