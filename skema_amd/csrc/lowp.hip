@@ -377,6 +377,10 @@ struct NtArgs {
   int64_t ldas;
   const uint8_t* bsc;
   int64_t ldbs;
+  // k_mx (fp8, A with block scales) only: A stored in 64-byte column strips
+  // when > 0 -- row r, K byte k at (k / 64) a_strip + 64 r + k % 64; a_strip
+  // = bytes per strip
+  int64_t a_strip;
 };
 
 // Rows i0 .. i0+3 of column j (the four accumulator values of one lane in
@@ -1054,12 +1058,19 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
     dal = static_cast<const uint8_t*>(g.al) - static_cast<const uint8_t*>(g.ah);
     dbl = static_cast<const uint8_t*>(g.bl) - static_cast<const uint8_t*>(g.bh);
   }
+  // bytes from one K step of A to the next: 128 in rows, two strips in
+  // 64-byte column strips
+  const int64_t astep = (FORM == kNxFp8 && ASC && g.a_strip) ? 2 * g.a_strip : SB;
 #pragma unroll
   for (int i = 0; i < IA; ++i) {
     const int r = 8 * (wave * IA + i) + lr;
     int64_t row = m0 + r;
     row = row < g.M ? row : g.M - 1;
-    ga[i] = static_cast<const uint8_t*>(g.ah) + (row * g.lda + g.koff_a) * ES + kb0 + 16 * (lp ^ ((r >> 1) & 7));
+    const int c = lp ^ ((r >> 1) & 7);  // the 16-byte chunk of the step this lane fetches
+    if (FORM == kNxFp8 && ASC && g.a_strip)
+      ga[i] = static_cast<const uint8_t*>(g.ah) + (kb0 / SB) * astep + (c >> 2) * g.a_strip + row * 64 + 16 * (c & 3);
+    else
+      ga[i] = static_cast<const uint8_t*>(g.ah) + (row * g.lda + g.koff_a) * ES + kb0 + 16 * c;
   }
 #pragma unroll
   for (int j = 0; j < IB; ++j) {
@@ -1084,10 +1095,10 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   }
   auto issue = [&](int st) {
     uint8_t* slot = smem + (st % NS) * STAGE;
-    const int64_t off = (int64_t)st * SB;
+    const int64_t off = (int64_t)st * SB, offa = (int64_t)st * astep;
 #pragma unroll
     for (int i = 0; i < IA; ++i)
-      __builtin_amdgcn_global_load_lds((const void*)(ga[i] + off),
+      __builtin_amdgcn_global_load_lds((const void*)(ga[i] + offa),
                                        (SKEMA_LDS void*)(slot + 1024 * (wave * IA + i)), 16, 0, 0);
 #pragma unroll
     for (int j = 0; j < IB; ++j)
@@ -1209,6 +1220,288 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   }
   nt_epilogue_at<TM, TN, FORM == kNxFp8 && !ASC, FORM == kNxFp8 && !BSC>(g, acc, m0 + wm * (16 * TM),
                                                                           n0 + wn * (16 * TN), lane);
+}
+
+// ---- fp8 window: quantise and left product in one pass ---------------------
+// In the block-scaled fp8 form the left product x = nu x + eta Upsilon_w
+// Q(A_w) uses L only as the B operand of its MFMAs.  This kernel quantises
+// each 128-row x 64-column tile of the FP64 window into LDS and consumes it
+// there: L never goes to HBM, and since a block owns 64 columns of x over all
+// rows of the window the product needs no K split.  It also writes R and its
+// scales for the right product, as k_quant_fp8_mx does, and its L blocks and
+// scales are k_quant_fp8_mx's: each column of a step cut into four 32-row
+// blocks, each scaled by 2^-e of its own max |x|.
+// Block: 8 waves (two per SIMD) in two groups with separate registers and
+// separate load counts.  Per 128-row step:
+//  - waves 0-3 (tile waves) hold the tile in registers: wave w loads
+//    columns w + 4 e (e < 16), two rows a lane (one contiguous 1 KB column
+//    segment per instruction), one step ahead, so the next tile is in flight
+//    for a whole step.  Every load is issued (addresses clamped into the
+//    window, out-of-window values zeroed afterwards) and every thread stores
+//    the same way (out-of-window R blocks go to a sink row), so the
+//    compiler's counts agree on every path and a wait for the step's tile
+//    never waits for the next one.  They write the tile to LDS as floats (t,
+//    rows padded to 132 words) and, after a barrier, quantise it: each thread
+//    two halves of L blocks (16 rows of a column: e4m3 bytes into the L tile,
+//    128-byte rows with 16-byte chunks XOR-swizzled as in k_mx, and the scale
+//    byte) and two halves of R blocks (16 columns of a row, straight to HBM),
+//    the block maximum shared with the lane holding the other half.
+//  - waves 4-7 (MFMA waves) hold the accumulators: wave 4 + m owns Upsilon
+//    rows [16 TM m, 16 TM (m + 1)) against the tile's four 16-column groups
+//    (tiles past the map's rows are computed on zero rows, never stored),
+//    its Upsilon fragments loaded from global memory at the start of the step
+//    (the window's slice of Upsilon, at most 448 x w bytes, stays in L2) and
+//    the L tile read after the second barrier.  The L tile is double
+//    buffered, so the tile waves quantise the next step while the MFMAs run.
+// Blocks start at different steps (block b at step b mod the step count)
+// and wrap around, so the blocks in flight spread over the window's rows.
+// max |x| over 16 values as float bits (non-negative floats order as
+// unsigned integers)
+__device__ __forceinline__ uint32_t abs_max_bits(const float (&v)[16]) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const uint32_t a = __float_as_uint(v[c]) & 0x7fffffffu;
+    m = m > a ? m : a;
+  }
+  return m;
+}
+// max with lane l ^ 1 (DPP quad_perm [1, 0, 3, 2], every lane active)
+__device__ __forceinline__ uint32_t pair_max_dpp(uint32_t m) {
+  const uint32_t o = (uint32_t)__builtin_amdgcn_mov_dpp((int)m, 0xb1, 0xf, 0xf, false);
+  return m > o ? m : o;
+}
+// max with lane l ^ 32
+__device__ __forceinline__ uint32_t half_max_swap(uint32_t m) {
+  const uint32_t o = (uint32_t)__shfl_xor((int)m, 32);
+  return m > o ? m : o;
+}
+struct QlArgs {
+  const double* A;
+  int64_t lda, rows, n;
+  uint8_t* R;
+  int64_t ldr;
+  uint8_t* Rs;
+  int64_t ldrs;
+  int64_t ldu;  // row length of the fp8 Upsilon (bytes, >= 256 past the last column)
+  int64_t rstrip;  // > 0: R in 64-column strips of rstrip bytes ((rows + 1) x 64)
+};
+constexpr int kQlCols = 64, kQlRows = 128, kQlThreads = 512, kQlPad = 132;
+// Rows of the fp8 Upsilon the fused kernel reads for range k: 4 MFMA waves
+// of 16 TM rows (TM = 1, 2, 4, 7); 0 when k is beyond it.
+__host__ __device__ constexpr int ql_tm(int64_t k) { return k <= 64 ? 1 : (k <= 128 ? 2 : (k <= 256 ? 4 : (k <= 448 ? 7 : 0))); }
+template <int TM>
+__global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlArgs q) {
+  static_assert(kQlCols == 64 && kQlRows == 128 && kQlThreads == 512, "tile / block shape");
+  constexpr int NE = kQlCols / 4;  // tile columns per tile wave
+  // LDS: t[j][i] (the tile as floats), the e4m3 L tiles Lt[buf][j] (row j =
+  // column j) and their scales sc[buf][j][b] (block b of column j).  After
+  // the last step, su holds half of the block's x tile at a time (FP32,
+  // column-major, ld kXRows) and t the Upsilon row scales.
+  constexpr int kXRows = 4 * 16 * TM;
+  __shared__ __attribute__((aligned(16))) float t[kQlCols][kQlPad];
+  __shared__ __attribute__((aligned(16))) uint8_t Lt[2][kQlCols][128];
+  __shared__ uint8_t sc[2][kQlCols][4];
+  __shared__ __attribute__((aligned(16))) float su[kQlCols / 2 * kXRows];
+  static_assert(kXRows * 8 <= (int)sizeof(t), "row scales fit in t");
+  const int tid = threadIdx.x, lane = tid & 63, l16 = lane & 15, lq = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t j0 = (int64_t)blockIdx.x * kQlCols;
+  const int64_t rows = q.rows, n = q.n;
+  // an even number of steps: a padding step (odd step counts) reads clamped
+  // rows, zeroes them, writes no R row and adds zero products
+  const int nsteps = (int)((((rows + kQlRows - 1) / kQlRows) + 1) & ~1);
+  const int s0 = (int)(blockIdx.x % (unsigned)nsteps);
+  auto step_of = [&](int i) { return (i + s0 < nsteps) ? i + s0 : i + s0 - nsteps; };
+
+  f4v acc[TM][4];
+#pragma unroll
+  for (int x = 0; x < TM; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  if (wave < 4) {
+    // ---- tile waves
+    auto load_a = [&](int st, d2v (&v)[NE]) {
+      const int64_t gi = (int64_t)st * kQlRows + 2 * lane;  // even; rows is even
+      const uint32_t boff = (uint32_t)((gi < rows ? gi : rows - 2) * 8);
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        const int64_t gj = j0 + wave + 4 * e;
+        const char* col = reinterpret_cast<const char*>(q.A + (gj < n ? gj : n - 1) * q.lda);
+        v[e] = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(col + boff));
+      }
+    };
+    auto step = [&](int i, d2v (&cur)[NE], d2v (&nxt)[NE]) {
+      const int st = step_of(i), sn = (st + 1 < nsteps) ? st + 1 : 0;
+      load_a(sn, nxt);
+      __builtin_amdgcn_sched_barrier(0);
+      const bool rval = (int64_t)st * kQlRows + 2 * lane < rows;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        const int jt = wave + 4 * e;
+        const bool ok = rval && j0 + jt < n;
+        const float2 f = ok ? float2{(float)cur[e][0], (float)cur[e][1]} : float2{0.f, 0.f};
+        *reinterpret_cast<float2*>(&t[jt][2 * lane]) = f;
+      }
+      // the tile's LDS writes land before the barrier (lds_barrier waits for
+      // nothing; __syncthreads would also wait for the next tile's loads)
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      lds_barrier();  // 1: t complete
+      const int buf = i & 1;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int idx = tid + 256 * p;
+        {
+          // L: half h of block b (rows 32 b + 16 h .. + 15) of column j;
+          // lanes 2 c and 2 c + 1 hold the two halves of one block
+          const int h = idx & 1, j = (idx >> 1) & (kQlCols - 1), bb = idx >> 7;
+          float v[16];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const f4v x = *reinterpret_cast<const f4v*>(&t[j][32 * bb + 16 * h + 4 * c]);
+            v[4 * c] = x[0]; v[4 * c + 1] = x[1]; v[4 * c + 2] = x[2]; v[4 * c + 3] = x[3];
+          }
+          const int ex = block_exponent(__uint_as_float(pair_max_dpp(abs_max_bits(v))));
+          const float inv = ldexpf(1.f, -ex);
+          i4v w;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            w[c] = (int)pack4_e4m3(v[4 * c] * inv, v[4 * c + 1] * inv, v[4 * c + 2] * inv, v[4 * c + 3] * inv);
+          *reinterpret_cast<i4v*>(&Lt[buf][j][16 * ((2 * bb + h) ^ ((j >> 1) & 7))]) = w;
+          sc[buf][j][bb] = (uint8_t)(127 + ex);  // both halves store the same byte
+        }
+        __builtin_amdgcn_sched_barrier(0);  // one half block at a time: fewer live registers
+        {
+          // R: half h of the 32-column block cb = p (columns 32 p + 16 h ..
+          // + 15 of the tile) of row ri; lanes l and l + 32 hold the halves
+          const int h = lane >> 5, ri = wave * 32 + (lane & 31);
+          float v[16];
+#pragma unroll
+          for (int c = 0; c < 16; ++c) v[c] = t[32 * p + 16 * h + c][ri];
+          const int ex = block_exponent(__uint_as_float(half_max_swap(abs_max_bits(v))));
+          const float inv = ldexpf(1.f, -ex);
+          i4v w;
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            w[c] = (int)pack4_e4m3(v[4 * c] * inv, v[4 * c + 1] * inv, v[4 * c + 2] * inv, v[4 * c + 3] * inv);
+          const int64_t gr = (int64_t)st * kQlRows + ri, gc = j0 + 32 * p, bk = gc / 32;
+          const bool ok = gr < rows && gc < n;
+          // R in rows (ld ldr) or in 64-byte column strips (a block's strip
+          // is its 64 columns: whole cache lines per step)
+          uint8_t* rp = q.rstrip ? q.R + blockIdx.x * q.rstrip + (ok ? gr : rows) * 64 + 32 * p + 16 * h
+                                 : (ok ? q.R + gr * q.ldr + gc + 16 * h : q.R + rows * q.ldr);
+          uint8_t* sp = ok ? q.Rs + gr * q.ldrs + (bk & 3) * (q.ldrs / 4) + (bk >> 2) : q.Rs + rows * q.ldrs;
+          *reinterpret_cast<i4v*>(rp) = w;
+          *sp = (uint8_t)(127 + ex);  // both halves store the same byte
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      lds_barrier();  // 2: L tile and scales complete; t free
+    };
+    d2v va[NE], vb[NE];
+    load_a(step_of(0), va);
+    for (int i = 0; i < nsteps; i += 2) {
+      step(i, va, vb);
+      step(i + 1, vb, va);
+    }
+  } else {
+    // ---- MFMA waves: the map holds 64 TM rows (zero rows past M), so no
+    // row is clamped and the tiles of a wave differ by a uniform offset
+    const int m = wave - 4;
+    const uint32_t uoff = (uint32_t)(((int64_t)m * 16 * TM + l16) * q.ldu + 16 * lq);
+    // swizzled byte offsets of this lane's two chunks (lq and 4 + lq) in a
+    // row of the L tile (rows 16 y + l16 all share the swizzle (l16 >> 1) & 7)
+    const int so0 = 16 * (lq ^ ((l16 >> 1) & 7)), so1 = so0 ^ 64;
+    for (int i = 0; i < nsteps; ++i) {
+      const int st = step_of(i), buf = i & 1;
+      i8v u[TM];
+      const uint8_t* base = static_cast<const uint8_t*>(g.ah) + g.koff_a + (int64_t)st * kQlRows;
+#pragma unroll
+      for (int x = 0; x < TM; ++x) {
+        const uint8_t* p = base + 16 * x * q.ldu + uoff;
+        const i4v lo = *reinterpret_cast<const i4v*>(p);
+        const i4v hi = *reinterpret_cast<const i4v*>(p + 64);
+        u[x] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      }
+      lds_barrier();  // 1
+      lds_barrier();  // 2: the step's L tile is complete
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int col = 16 * y + l16;
+        const i4v lo = *reinterpret_cast<const i4v*>(&Lt[buf][col][so0]);
+        const i4v hi = *reinterpret_cast<const i4v*>(&Lt[buf][col][so1]);
+        const i8v bf = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        const int sb = sc[buf][col][lq];
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(u[x], bf, acc[x][y], 0, 0, 0, 127, 0, sb);
+      }
+    }
+  }
+
+  // Epilogue: the block's columns of x are one contiguous range of HBM (x is
+  // k x n with ld k = M), so the tile goes through LDS and back out in
+  // coalesced read-modify-write passes by all waves, x = beta x + alpha
+  // 2^e(row) acc (the row's Upsilon scale), two elements (16 bytes) a thread
+  // and round; 32 columns a pass.
+  __syncthreads();
+  float* xs = su;
+  double* rsc = reinterpret_cast<double*>(&t[0][0]);
+  const int64_t M = g.M;
+  for (int i = tid; i < M; i += kQlThreads) rsc[i] = ldexp(1.0, block_exponent(g.amax_a[i]));
+  const int64_t ncols = n - j0 < kQlCols ? n - j0 : kQlCols;
+  const double alpha = g.out.alpha1, beta = g.out.beta1;
+  constexpr int64_t kStride = 2 * kQlThreads;
+  constexpr int kEB = 8;
+  const int64_t di = kStride % M, dj = kStride / M;
+#pragma unroll
+  for (int hc = 0; hc < 2; ++hc) {
+    if (hc) __syncthreads();  // the first half's reads of xs are done
+    // xs[j][i] = column 32 hc + j, ld kXRows: rows past M land in unused slots
+    if (wave >= 4) {
+#pragma unroll
+      for (int x = 0; x < TM; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          *reinterpret_cast<f4v*>(&xs[(16 * y + l16) * kXRows + (wave - 4) * 16 * TM + 16 * x + 4 * lq]) =
+              acc[x][2 * hc + y];
+    }
+    __syncthreads();
+    const int64_t len = (ncols - 32 * hc < 32 ? ncols - 32 * hc : 32) * M;  // elements (<= 0: none)
+    double* xo = g.out.C1 + (j0 + 32 * hc) * g.out.ldc1;
+    // element e = jl M + i of the range is xs[jl][i]; rounds of kEB pairs a
+    // thread, the reads of x of a round all issued before the first write
+    int64_t i = (2 * (int64_t)tid) % M, jl = (2 * (int64_t)tid) / M;
+    for (int64_t e0 = 2 * (int64_t)tid; e0 + 1 < len; e0 += kEB * kStride) {
+      d2v old[kEB];
+#pragma unroll
+      for (int b = 0; b < kEB; ++b) {
+        const int64_t e = e0 + b * kStride;
+        old[b] = (beta != 0.0 && e + 1 < len) ? *reinterpret_cast<const d2v*>(xo + e) : d2v{0.0, 0.0};
+      }
+#pragma unroll
+      for (int b = 0; b < kEB; ++b) {
+        const int64_t e = e0 + b * kStride;
+        if (e + 1 < len) {
+          // the pair's second element: next row, or row 0 of the next column
+          const bool wrap = i + 1 == M;
+          const double a0 = (double)xs[jl * kXRows + i], a1 = (double)xs[(jl + wrap) * kXRows + (wrap ? 0 : i + 1)];
+          const double v0 = alpha * a0 * rsc[i], v1 = alpha * a1 * rsc[wrap ? 0 : i + 1];
+          *reinterpret_cast<d2v*>(xo + e) = d2v{v0 + beta * old[b][0], v1 + beta * old[b][1]};
+        }
+        i += di;
+        jl += dj;
+        if (i >= M) { i -= M; ++jl; }
+      }
+    }
+    if (len > 0 && (len & 1) && tid == 0) {
+      const int64_t e = len - 1, ie = e % M;
+      const double v = alpha * (double)xs[(e / M) * kXRows + ie] * rsc[ie];
+      xo[e] = (beta != 0.0) ? v + beta * xo[e] : v;
+    }
+  }
 }
 
 // Sum of the FP32 partials of the K splits (in FP64, fixed order), then the
@@ -1430,7 +1723,17 @@ void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
 
 // t (optional): a timing scope closed right after the product kernel, so
 // it covers that kernel and not the split-K reduction that follows.
+// Whether gemm_nt runs this product on the direct-to-LDS kernel k_mx (the
+// one consumer of A in 64-byte column strips).
+bool nt_picks_mx(int prec, const NtArgs& a) {
+  NtPlan pl = plan_nt(prec, a.M, a.N, a.K);
+  if ((a.asc || a.bsc) && pl.tm == 0) return false;
+  const int kstep_mx = prec == SKEMA_PREC_FP8 ? 128 : 64;
+  return pl.ns > 0 && ntx_eligible(prec, a) && a.K % kstep_mx == 0 && launch_mx(0, prec, a, pl, dim3(), true);
+}
+
 void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr) {
+  SKEMA_REQUIRE(!a.a_strip || nt_picks_mx(prec, a), kErrUnsupported, "A in column strips needs the k_mx kernel");
   NtPlan pl = plan_nt(prec, a.M, a.N, a.K);
   SKEMA_REQUIRE(!(a.asc && a.bsc), kErrUnsupported, "block scales on both NT operands");
   if ((a.asc || a.bsc) && pl.tm == 0) pl = plan_ntx(prec, a.M, a.N, a.K);
@@ -1475,6 +1778,12 @@ unsigned grid1(int64_t e) { return (unsigned)std::max<int64_t>(1, std::min<int64
 
 }  // namespace
 
+// Rows of the fp8 Upsilon: k rounded up to the rows the fused quantise +
+// left kernel covers (4 waves of 16 TM rows), the extra rows zero; the row
+// maxima (floats) follow at this byte offset.
+static int64_t fp8_ups_rows(int64_t k) { return ql_tm(k) ? 64 * ql_tm(k) : k; }
+static size_t fp8_ups_amax_offset(const Sketch& sk) { return align_up((size_t)fp8_ups_rows(sk.k) * sk.lp_ldups, 256); }
+
 void prepare_lowp_maps(Sketch& sk) {
   const int prec = sk.p.precision;
   SKEMA_REQUIRE(prec == SKEMA_PREC_BF16 || prec == SKEMA_PREC_BF16X3 || prec == SKEMA_PREC_FP8 ||
@@ -1483,15 +1792,21 @@ void prepare_lowp_maps(Sketch& sk) {
   hipStream_t st = sk.ctx->stream;
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n;
   // Row-major reduced maps, K contiguous, leading dimension padded to 128.
-  sk.lp_ldups = cdiv(m, 128) * 128;
+  // two spare 128-element steps past the last column: the fused fp8
+  // quantise + left product reads whole steps of Upsilon (and one padding
+  // step) past the window's end
+  sk.lp_ldups = cdiv(m, 128) * 128 + 256;
   sk.lp_ldomps = cdiv(n, 128) * 128;
   const size_t ue = (size_t)k * sk.lp_ldups, oe = (size_t)(k + s) * sk.lp_ldomps;
   if (prec == SKEMA_PREC_FP8) {
-    // e4m3 rows followed by one float (row max |x|) per row
-    sk.lp_ups.alloc(align_up(ue, 256) + (size_t)k * sizeof(float));
+    // e4m3 rows followed by one float (row max |x|) per row; Upsilon's rows
+    // padded with zero rows to the fused kernel's row count
+    const size_t uo = fp8_ups_amax_offset(sk);
+    sk.lp_ups.alloc(uo + (size_t)k * sizeof(float));
     sk.lp_omps.alloc(align_up(oe, 256) + (size_t)(k + s) * sizeof(float));
+    if (uo > ue) fill_i32(st, (int*)(sk.lp_ups.as<uint8_t>() + ue), (int64_t)((uo - ue) / 4), 0);
     hipLaunchKernelGGL(k_convert_map_fp8, dim3((unsigned)k), dim3(kThreads), 0, st, sk.upsL, sk.ldups, k, m,
-                       sk.lp_ups.as<uint8_t>(), sk.lp_ldups, (float*)(sk.lp_ups.as<uint8_t>() + align_up(ue, 256)));
+                       sk.lp_ups.as<uint8_t>(), sk.lp_ldups, (float*)(sk.lp_ups.as<uint8_t>() + uo));
     hipLaunchKernelGGL(k_convert_map_fp8, dim3((unsigned)(k + s)), dim3(kThreads), 0, st, sk.omps.as<double>(),
                        k + s, k + s, n, sk.lp_omps.as<uint8_t>(), sk.lp_ldomps,
                        (float*)(sk.lp_omps.as<uint8_t>() + align_up(oe, 256)));
@@ -1570,7 +1885,7 @@ WinLayout win_layout(const Sketch& sk, int64_t row0, int64_t rows) {
   w.ldls = 4 * align_up((size_t)cdiv(rows, 128), 4);
   w.ldrs = 4 * align_up((size_t)cdiv(n, 128), 4);
   w.le = (size_t)n * w.ldl;
-  w.re = (size_t)rows * w.ldr;
+  w.re = (size_t)(rows + 1) * w.ldr;  // one spare row: the sink of k_quant_left_fp8's out-of-window stores
   w.ge = (size_t)s * w.ldl;
   w.o_l = 0;
   w.o_r = align_up(w.le * w.esz * w.planes, 256);
@@ -1584,6 +1899,20 @@ WinLayout win_layout(const Sketch& sk, int64_t row0, int64_t rows) {
   return w;
 }
 
+// Block scales of a window (the block-scaled fp8 form).  Unit scales (0x7f)
+// wherever no block writes one (K padding past the window): every window of
+// the same row count writes the same blocks, so the padding is refilled only
+// when the row count (the layout) changes.
+uint8_t* mx_scales(Sketch& sk, int64_t rows, const WinLayout& w, hipStream_t st) {
+  const size_t sbytes = w.o_rs + align_up((size_t)(rows + 1) * w.ldrs, 256);  // + the sink row
+  if (sk.lp_scales.bytes < sbytes || sk.lp_scales_rows != rows) {
+    sk.lp_scales.reserve(sbytes);
+    fill_i32(st, sk.lp_scales.as<int>(), (int64_t)(sbytes / 4), 0x7f7f7f7f);
+    sk.lp_scales_rows = rows;
+  }
+  return sk.lp_scales.as<uint8_t>();
+}
+
 // The quantise pass of one window into `base` on stream st.
 void quantize_window(Sketch& sk, const double* H, int64_t ldh, int64_t rows, const WinLayout& w, char* base,
                      hipStream_t st) {
@@ -1593,16 +1922,7 @@ void quantize_window(Sketch& sk, const double* H, int64_t ldh, int64_t rows, con
   Timed t(sk.ctx, "window", st);
   dim3 grid((unsigned)cdiv(rows, kQ), (unsigned)cdiv(n, kQ));
   if (w.mx) {
-    // Unit scales (0x7f) wherever no block writes one (K padding past the
-    // window): every window of the same row count writes the same blocks, so
-    // the padding is refilled only when the row count (the layout) changes.
-    const size_t sbytes = w.o_rs + align_up((size_t)rows * w.ldrs, 256);
-    if (sk.lp_scales.bytes < sbytes || sk.lp_scales_rows != rows) {
-      sk.lp_scales.reserve(sbytes);
-      fill_i32(st, sk.lp_scales.as<int>(), (int64_t)(sbytes / 4), 0x7f7f7f7f);
-      sk.lp_scales_rows = rows;
-    }
-    uint8_t* sc = sk.lp_scales.as<uint8_t>();
+    uint8_t* sc = mx_scales(sk, rows, w, st);
     dim3 gmx((unsigned)cdiv(rows, kMxR), (unsigned)cdiv(n, kMxC));
     hipLaunchKernelGGL(k_quant_fp8_mx, gmx, dim3(kThreads), 0, st, H, ldh, rows, n, (uint8_t*)(base + w.o_l), w.ldl,
                        sc + w.o_ls, w.ldls, (uint8_t*)(base + w.o_r), w.ldr, sc + w.o_rs, w.ldrs);
@@ -1629,23 +1949,44 @@ void quantize_window(Sketch& sk, const double* H, int64_t ldh, int64_t rows, con
 
 // The three window products from the quantised copies in `base`, on the
 // sketch's stream.
-void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w, char* base) {
+// The right product's NT arguments (A = R, B = [Omega; Psi]) without its
+// outputs.
+NtArgs right_args(const Sketch& sk, int64_t rows, const WinLayout& w, char* base) {
+  const int64_t k = sk.k, s = sk.s, n = sk.p.n;
+  const size_t oe = (size_t)(k + s) * sk.lp_ldomps;
+  NtArgs a{};
+  a.M = rows; a.N = k + s; a.K = n;
+  a.ah = base + w.o_r;
+  a.al = w.split ? (const void*)((const __bf16*)(base + w.o_r) + w.re) : nullptr;
+  a.amax_a = (w.fp8 && !w.mx) ? (const float*)(base + w.o_rmax) : nullptr;
+  if (w.mx) { a.asc = sk.lp_scales.as<uint8_t>() + w.o_rs; a.ldas = w.ldrs; }
+  a.lda = w.ldr; a.koff_a = 0;
+  a.bh = sk.lp_omps.p;
+  a.bl = w.split ? (const void*)(sk.lp_omps.as<__bf16>() + oe) : nullptr;
+  a.amax_b = w.fp8 ? (const float*)(sk.lp_omps.as<uint8_t>() + align_up(oe, 256)) : nullptr;
+  a.ldb = sk.lp_ldomps;
+  return a;
+}
+
+// left = false: the left product was done by k_quant_left_fp8; rstrip > 0:
+// R in 64-column strips of rstrip bytes.
+void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w, char* base, bool left = true,
+                     int64_t rstrip = 0) {
   Ctx* c = sk.ctx;
   hipStream_t st = c->stream;
   const int prec = sk.p.precision;
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n;
   const double eta = sk.p.eta, nu = sk.p.nu;
   const bool fp8 = w.fp8, split = w.split, mx = w.mx;
-  float* rmax = (float*)(base + w.o_rmax);
   float* cmax = (float*)(base + w.o_cmax);
-  const size_t ue = (size_t)k * sk.lp_ldups, oe = (size_t)(k + s) * sk.lp_ldomps;
-  {
+  const size_t ue = (size_t)k * sk.lp_ldups;
+  if (left) {
     Timed t(c, "sketch_left");
     NtArgs a{};
     a.M = k; a.N = n; a.K = rows;
     a.ah = sk.lp_ups.p;
     a.al = split ? (const void*)(sk.lp_ups.as<__bf16>() + ue) : nullptr;
-    a.amax_a = fp8 ? (const float*)(sk.lp_ups.as<uint8_t>() + align_up(ue, 256)) : nullptr;
+    a.amax_a = fp8 ? (const float*)(sk.lp_ups.as<uint8_t>() + fp8_ups_amax_offset(sk)) : nullptr;
     a.lda = sk.lp_ldups; a.koff_a = row0;
     a.bh = base + w.o_l;
     a.bl = split ? (const void*)((const __bf16*)(base + w.o_l) + w.le) : nullptr;
@@ -1658,17 +1999,8 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
   }
   {
     Timed t(c, "sketch_right");
-    NtArgs a{};
-    a.M = rows; a.N = k + s; a.K = n;
-    a.ah = base + w.o_r;
-    a.al = split ? (const void*)((const __bf16*)(base + w.o_r) + w.re) : nullptr;
-    a.amax_a = (fp8 && !mx) ? rmax : nullptr;
-    if (mx) { a.asc = sk.lp_scales.as<uint8_t>() + w.o_rs; a.ldas = w.ldrs; }
-    a.lda = w.ldr; a.koff_a = 0;
-    a.bh = sk.lp_omps.p;
-    a.bl = split ? (const void*)(sk.lp_omps.as<__bf16>() + oe) : nullptr;
-    a.amax_b = fp8 ? (const float*)(sk.lp_omps.as<uint8_t>() + align_up(oe, 256)) : nullptr;
-    a.ldb = sk.lp_ldomps;
+    NtArgs a = right_args(sk, rows, w, base);
+    a.a_strip = rstrip;
     a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
     a.gtype = w.gtype; a.gh = base + w.o_g; a.gl = base + w.o_g + w.ge * w.gsz; a.ldg = w.ldl;
     gemm_nt(st, prec, a, &c->gws.buf, &t);
@@ -1690,12 +2022,55 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
   }
 }
 
+// Quantise pass and left product in one kernel (k_quant_left_fp8) for
+// block-scaled fp8 windows with k <= 448, an even row count and 16-byte
+// aligned column pairs; false (nothing launched) otherwise.
+bool quant_left_fp8(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows, const WinLayout& w,
+                    char* base, int64_t* rstrip_out) {
+  const int64_t k = sk.k, n = sk.p.n;
+  if (!w.mx || !ql_tm(k) || rows < 2 || (rows & 1) || (ldh & 1) || ((uintptr_t)H & 15) || n < 1) return false;
+  // 32-bit offsets: the lane's rows of the window column, Upsilon rows
+  if (rows * 8 >= ((int64_t)1 << 31) || k * sk.lp_ldups >= ((int64_t)1 << 31)) return false;
+  Ctx* c = sk.ctx;
+  hipStream_t st = c->stream;
+  uint8_t* sc = mx_scales(sk, rows, w, st);
+  Timed t(c, "sketch_left");
+  NtArgs a{};
+  a.M = k; a.N = n; a.K = rows;
+  a.ah = sk.lp_ups.p;
+  a.amax_a = (const float*)(sk.lp_ups.as<uint8_t>() + fp8_ups_amax_offset(sk));
+  a.lda = sk.lp_ldups; a.koff_a = row0;
+  // x is k x n with ld k: the kernel's epilogue takes a block's columns as
+  // one contiguous range
+  a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = sk.p.eta; a.out.beta1 = sk.p.nu; a.out.nsplit = n;
+  a.out.C2 = a.out.C1; a.out.ldc2 = k;
+  // R in 64-column strips when the right product runs on k_mx (whole
+  // cache lines per block and step), otherwise in rows
+  const int64_t rstrip = nt_picks_mx(sk.p.precision, right_args(sk, rows, w, base)) ? (rows + 1) * 64 : 0;
+  *rstrip_out = rstrip;
+  QlArgs q{H, ldh, rows, n, (uint8_t*)(base + w.o_r), w.ldr, sc + w.o_rs, w.ldrs, sk.lp_ldups, rstrip};
+  const dim3 grid((unsigned)cdiv(n, kQlCols));
+  switch (ql_tm(k)) {
+    case 1: hipLaunchKernelGGL(k_quant_left_fp8<1>, grid, dim3(kQlThreads), 0, st, a, q); break;
+    case 2: hipLaunchKernelGGL(k_quant_left_fp8<2>, grid, dim3(kQlThreads), 0, st, a, q); break;
+    case 4: hipLaunchKernelGGL(k_quant_left_fp8<4>, grid, dim3(kQlThreads), 0, st, a, q); break;
+    default: hipLaunchKernelGGL(k_quant_left_fp8<7>, grid, dim3(kQlThreads), 0, st, a, q); break;
+  }
+  check_launch("k_quant_left_fp8");
+  return true;
+}
+
 }  // namespace
 
 void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows) {
   const WinLayout w = win_layout(sk, row0, rows);
   sk.win.reserve(w.total);
   char* base = sk.win.as<char>();
+  int64_t rstrip = 0;
+  if (quant_left_fp8(sk, H, ldh, row0, rows, w, base, &rstrip)) {
+    window_products(sk, row0, rows, w, base, false, rstrip);
+    return;
+  }
   quantize_window(sk, H, ldh, rows, w, base, sk.ctx->stream);
   window_products(sk, row0, rows, w, base);
 }
