@@ -1249,9 +1249,9 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
 //  - waves 4-7 (MFMA waves) hold the accumulators: wave 4 + m owns Upsilon
 //    rows [16 TM m, 16 TM (m + 1)) against the tile's four 16-column groups
 //    (tiles past the map's rows are computed on zero rows, never stored),
-//    its Upsilon fragments loaded from global memory at the start of the step
-//    (the window's slice of Upsilon, at most 448 x w bytes, stays in L2) and
-//    the L tile read after the second barrier.  The L tile is double
+//    its Upsilon fragments loaded from global memory one step ahead, between
+//    the step's barriers (the window's slice of Upsilon, at most 448 x w
+//    bytes, stays in L2), and the L tile read after the second barrier.  The L tile is double
 //    buffered, so the tile waves quantise the next step while the MFMAs run.
 // Blocks start at different steps (block b at step b mod the step count)
 // and wrap around, so the blocks in flight spread over the window's rows.
@@ -1402,6 +1402,21 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
     };
     d2v va[NE], vb[NE];
     load_a(step_of(0), va);
+    // The stores of a step (two 16-byte R pieces and two scale bytes a
+    // thread) sit between a tile's loads and the wait for them; the same
+    // stores here, to the sink row, give the first step the same count, so
+    // the compiler's wait (the minimum over the loop's entries) does not make
+    // every step wait for the previous step's stores.
+    {
+      uint8_t* rp = q.rstrip ? q.R + blockIdx.x * q.rstrip + rows * 64 : q.R + rows * q.ldr;
+      uint8_t* sp = q.Rs + rows * q.ldrs;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        *reinterpret_cast<i4v*>(rp + 16 * p) = i4v{0, 0, 0, 0};
+        sp[p] = (uint8_t)127;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
     for (int i = 0; i < nsteps; i += 2) {
       step(i, va, vb);
       step(i + 1, vb, va);
@@ -1414,9 +1429,10 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
     // swizzled byte offsets of this lane's two chunks (lq and 4 + lq) in a
     // row of the L tile (rows 16 y + l16 all share the swizzle (l16 >> 1) & 7)
     const int so0 = 16 * (lq ^ ((l16 >> 1) & 7)), so1 = so0 ^ 64;
-    for (int i = 0; i < nsteps; ++i) {
-      const int st = step_of(i), buf = i & 1;
-      i8v u[TM];
+    // Fragments one step ahead, loaded between the step's barriers: the loads
+    // queue behind the tile waves' loads, so issued before barrier 1 they
+    // would hold the MFMA waves (and with them the tile waves) at it.
+    auto load_u = [&](int st, i8v (&u)[TM]) {
       const uint8_t* base = static_cast<const uint8_t*>(g.ah) + g.koff_a + (int64_t)st * kQlRows;
 #pragma unroll
       for (int x = 0; x < TM; ++x) {
@@ -1425,7 +1441,11 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
         const i4v hi = *reinterpret_cast<const i4v*>(p + 64);
         u[x] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
       }
+    };
+    auto step = [&](int i, i8v (&u)[TM], i8v (&un)[TM]) {
+      const int sn = step_of(i + 1 < nsteps ? i + 1 : 0), buf = i & 1;
       lds_barrier();  // 1
+      load_u(sn, un);
       lds_barrier();  // 2: the step's L tile is complete
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
@@ -1438,6 +1458,12 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
         for (int x = 0; x < TM; ++x)
           acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(u[x], bf, acc[x][y], 0, 0, 0, 127, 0, sb);
       }
+    };
+    i8v ua[TM], ub[TM];
+    load_u(step_of(0), ua);
+    for (int i = 0; i < nsteps; i += 2) {
+      step(i, ua, ub);
+      step(i + 1, ub, ua);
     }
   }
 
