@@ -1249,9 +1249,9 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
 //  - waves 4-7 (MFMA waves) hold the accumulators: wave 4 + m owns Upsilon
 //    rows [16 TM m, 16 TM (m + 1)) against the tile's four 16-column groups
 //    (tiles past the map's rows are computed on zero rows, never stored),
-//    its Upsilon fragments loaded from global memory one step ahead, between
-//    the step's barriers (the window's slice of Upsilon, at most 448 x w
-//    bytes, stays in L2), and the L tile read after the second barrier.  The L tile is double
+//    its Upsilon fragments copied to LDS by DMA one step ahead (the window's
+//    slice of Upsilon, at most 448 x w bytes, stays in L2), and the L tile
+//    read after the second barrier.  The L tile is double
 //    buffered, so the tile waves quantise the next step while the MFMAs run.
 // Blocks start at different steps (block b at step b mod the step count)
 // and wrap around, so the blocks in flight spread over the window's rows.
@@ -1423,48 +1423,66 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
     }
   } else {
     // ---- MFMA waves: the map holds 64 TM rows (zero rows past M), so no
-    // row is clamped and the tiles of a wave differ by a uniform offset
+    // row is clamped.  Each wave copies its fragment rows of the next step
+    // into its own region of su by DMA (global_load_lds_dwordx4) as soon as
+    // it has read this step's into registers: the copy has a whole step to
+    // land (it queues behind the tile waves' loads), holds no registers, and
+    // needs no barrier (a wave reads only what it copied).  Chunk c of row r
+    // sits at position c ^ ((r >> 1) & 7), so the fragment reads of 16
+    // consecutive rows at one chunk touch 16 distinct bank groups; one DMA
+    // instruction fills 8 rows, lane l landing at row l / 8, position l % 8
+    // and fetching the chunk that belongs there.
     const int m = wave - 4;
-    const uint32_t uoff = (uint32_t)(((int64_t)m * 16 * TM + l16) * q.ldu + 16 * lq);
-    // swizzled byte offsets of this lane's two chunks (lq and 4 + lq) in a
-    // row of the L tile (rows 16 y + l16 all share the swizzle (l16 >> 1) & 7)
-    const int so0 = 16 * (lq ^ ((l16 >> 1) & 7)), so1 = so0 ^ 64;
-    // Fragments one step ahead, loaded between the step's barriers: the loads
-    // queue behind the tile waves' loads, so issued before barrier 1 they
-    // would hold the MFMA waves (and with them the tile waves) at it.
-    auto load_u = [&](int st, i8v (&u)[TM]) {
+    constexpr int kURows = 16 * TM;
+    static_assert(4 * kURows * 128 <= (int)sizeof(su), "fragment rows fit in su");
+    uint8_t* ureg = reinterpret_cast<uint8_t*>(su) + m * kURows * 128;
+    const uint32_t urow = (uint32_t)(((int64_t)m * kURows + (lane >> 3)) * q.ldu);
+    const uint32_t uev = urow + 16 * ((lane & 7) ^ ((lane >> 4) & 7));
+    const uint32_t uod = urow + 16 * ((lane & 7) ^ ((4 + (lane >> 4)) & 7));
+    auto issue_u = [&](int st) {
       const uint8_t* base = static_cast<const uint8_t*>(g.ah) + g.koff_a + (int64_t)st * kQlRows;
 #pragma unroll
+      for (int d = 0; d < 2 * TM; ++d)
+        __builtin_amdgcn_global_load_lds((const void*)(base + 8 * d * q.ldu + ((d & 1) ? uod : uev)),
+                                         (SKEMA_LDS void*)(ureg + 1024 * d), 16, 0, 0);
+    };
+    // swizzled byte offsets of this lane's two chunks (lq and 4 + lq) in a
+    // row of the L tile or of the fragment rows (rows 16 y + l16 all share
+    // the swizzle (l16 >> 1) & 7)
+    const int so0 = 16 * (lq ^ ((l16 >> 1) & 7)), so1 = so0 ^ 64;
+    issue_u(step_of(0));
+    for (int i = 0; i < nsteps; ++i) {
+      const int buf = i & 1;
+      lds_barrier();  // 1
+      lds_barrier();  // 2: the step's L tile is complete
+      wait_vm<0>();   // this wave's fragment copies (issued a step ago)
+      i8v u[TM];
+#pragma unroll
       for (int x = 0; x < TM; ++x) {
-        const uint8_t* p = base + 16 * x * q.ldu + uoff;
-        const i4v lo = *reinterpret_cast<const i4v*>(p);
-        const i4v hi = *reinterpret_cast<const i4v*>(p + 64);
+        const i4v lo = *reinterpret_cast<const i4v*>(ureg + (16 * x + l16) * 128 + so0);
+        const i4v hi = *reinterpret_cast<const i4v*>(ureg + (16 * x + l16) * 128 + so1);
         u[x] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
       }
-    };
-    auto step = [&](int i, i8v (&u)[TM], i8v (&un)[TM]) {
-      const int sn = step_of(i + 1 < nsteps ? i + 1 : 0), buf = i & 1;
-      lds_barrier();  // 1
-      load_u(sn, un);
-      lds_barrier();  // 2: the step's L tile is complete
+      i8v bf[4];
+      int sb[4];
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
         const int col = 16 * y + l16;
         const i4v lo = *reinterpret_cast<const i4v*>(&Lt[buf][col][so0]);
         const i4v hi = *reinterpret_cast<const i4v*>(&Lt[buf][col][so1]);
-        const i8v bf = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-        const int sb = sc[buf][col][lq];
+        bf[y] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        sb[y] = sc[buf][col][lq];
+      }
+      // the fragments are in registers before the next copy lands in ureg
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      issue_u(step_of(i + 1 < nsteps ? i + 1 : 0));
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
 #pragma unroll
         for (int x = 0; x < TM; ++x)
-          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(u[x], bf, acc[x][y], 0, 0, 0, 127, 0, sb);
-      }
-    };
-    i8v ua[TM], ub[TM];
-    load_u(step_of(0), ua);
-    for (int i = 0; i < nsteps; i += 2) {
-      step(i, ua, ub);
-      step(i + 1, ub, ua);
+          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(u[x], bf[y], acc[x][y], 0, 0, 0, 127, 0, sb[y]);
     }
+    wait_vm<0>();  // the last (unused) copy lands before su is reused
   }
 
   // Epilogue: the block's columns of x are one contiguous range of HBM (x is
