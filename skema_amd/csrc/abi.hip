@@ -603,7 +603,10 @@ int skema_linear_update_dense(skema_sketch* h, const double* A, int64_t lda) {
     SKEMA_REQUIRE(sk.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "linear_update_dense on an RBF sketch");
     check_mat(A, sk.p.m, sk.p.n, lda, "linear_update_dense A");
     zero_state(sk);
-    for_each_window(sk, [&](int64_t r0, int64_t w) { update_dense_window(sk, A + r0, lda, r0, w); });
+    // fp8: one quantise + left-product pass over every window, then the
+    // windows' right and core products
+    if (!update_dense_all_lowp(sk, A, lda))
+      for_each_window(sk, [&](int64_t r0, int64_t w) { update_dense_window(sk, A + r0, lda, r0, w); });
     sk.ctx->sync_if_needed();
   });
 }

@@ -1253,8 +1253,8 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
 //    slice of Upsilon, at most 448 x w bytes, stays in L2), and the L tile
 //    read after the second barrier.  The L tile is double
 //    buffered, so the tile waves quantise the next step while the MFMAs run.
-// Blocks start at different steps (block b at step b mod the step count)
-// and wrap around, so the blocks in flight spread over the window's rows.
+// Blocks start at different steps (block b at step b mod the steps of a
+// window) and wrap around, so the blocks in flight spread over the rows.
 // max |x| over 16 values as float bits (non-negative floats order as
 // unsigned integers)
 __device__ __forceinline__ uint32_t abs_max_bits(const float (&v)[16]) {
@@ -1284,7 +1284,12 @@ struct QlArgs {
   uint8_t* Rs;
   int64_t ldrs;
   int64_t ldu;  // row length of the fp8 Upsilon (bytes, >= 256 past the last column)
-  int64_t rstrip;  // > 0: R in 64-column strips of rstrip bytes ((rows + 1) x 64)
+  int64_t rstrip;  // > 0: R in 64-column strips of rstrip bytes ((wrows + 1) x 64)
+  // Windows: the pass covers rows in windows of wrows rows (a multiple of
+  // 128; the last may be shorter), window w's R and scales at R + w rwin and
+  // Rs + w swin (rows of the window, the sink row wrows); x sums the
+  // windows' products.  One window: rwin = 0 and wrows = rows.
+  int64_t wrows, rwin, swin;
 };
 constexpr int kQlCols = 64, kQlRows = 128, kQlThreads = 512, kQlPad = 132;
 // Rows of the fp8 Upsilon the fused kernel reads for range k: 4 MFMA waves
@@ -1311,7 +1316,10 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
   // an even number of steps: a padding step (odd step counts) reads clamped
   // rows, zeroes them, writes no R row and adds zero products
   const int nsteps = (int)((((rows + kQlRows - 1) / kQlRows) + 1) & ~1);
-  const int s0 = (int)(blockIdx.x % (unsigned)nsteps);
+  const int spw = q.rwin ? (int)(q.wrows / kQlRows) : nsteps;  // steps per window
+  // staggered over a window's steps at most, so the blocks in flight read
+  // one or two windows' slices of Upsilon (which stay in L2)
+  const int s0 = (int)(blockIdx.x % (unsigned)(spw < nsteps ? spw : nsteps));
   auto step_of = [&](int i) { return (i + s0 < nsteps) ? i + s0 : i + s0 - nsteps; };
 
   f4v acc[TM][4];
@@ -1387,11 +1395,16 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
             w[c] = (int)pack4_e4m3(v[4 * c] * inv, v[4 * c + 1] * inv, v[4 * c + 2] * inv, v[4 * c + 3] * inv);
           const int64_t gr = (int64_t)st * kQlRows + ri, gc = j0 + 32 * p, bk = gc / 32;
           const bool ok = gr < rows && gc < n;
+          // the row within its window (out of range: the sink row wrows of
+          // window 0; a padding step may lie past the last window)
+          const int wi = ok ? st / spw : 0;
+          const int64_t lr = ok ? gr - (int64_t)wi * q.wrows : q.wrows;
+          uint8_t* rw = q.R + wi * q.rwin;
           // R in rows (ld ldr) or in 64-byte column strips (a block's strip
           // is its 64 columns: whole cache lines per step)
-          uint8_t* rp = q.rstrip ? q.R + blockIdx.x * q.rstrip + (ok ? gr : rows) * 64 + 32 * p + 16 * h
-                                 : (ok ? q.R + gr * q.ldr + gc + 16 * h : q.R + rows * q.ldr);
-          uint8_t* sp = ok ? q.Rs + gr * q.ldrs + (bk & 3) * (q.ldrs / 4) + (bk >> 2) : q.Rs + rows * q.ldrs;
+          uint8_t* rp = q.rstrip ? rw + blockIdx.x * q.rstrip + lr * 64 + 32 * p + 16 * h
+                                 : rw + lr * q.ldr + (ok ? gc + 16 * h : 0);
+          uint8_t* sp = q.Rs + wi * q.swin + lr * q.ldrs + (ok ? (bk & 3) * (q.ldrs / 4) + (bk >> 2) : 0);
           *reinterpret_cast<i4v*>(rp) = w;
           *sp = (uint8_t)(127 + ex);  // both halves store the same byte
         }
@@ -1408,8 +1421,8 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
     // the compiler's wait (the minimum over the loop's entries) does not make
     // every step wait for the previous step's stores.
     {
-      uint8_t* rp = q.rstrip ? q.R + blockIdx.x * q.rstrip + rows * 64 : q.R + rows * q.ldr;
-      uint8_t* sp = q.Rs + rows * q.ldrs;
+      uint8_t* rp = q.rstrip ? q.R + blockIdx.x * q.rstrip + q.wrows * 64 : q.R + q.wrows * q.ldr;
+      uint8_t* sp = q.Rs + q.wrows * q.ldrs;
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
         *reinterpret_cast<i4v*>(rp + 16 * p) = i4v{0, 0, 0, 0};
@@ -1949,10 +1962,11 @@ WinLayout win_layout(const Sketch& sk, int64_t row0, int64_t rows) {
 // when the row count (the layout) changes.
 uint8_t* mx_scales(Sketch& sk, int64_t rows, const WinLayout& w, hipStream_t st) {
   const size_t sbytes = w.o_rs + align_up((size_t)(rows + 1) * w.ldrs, 256);  // + the sink row
-  if (sk.lp_scales.bytes < sbytes || sk.lp_scales_rows != rows) {
+  if (sk.lp_scales.bytes < sbytes || sk.lp_scales_rows != rows || sk.lp_scales_windows != 1) {
     sk.lp_scales.reserve(sbytes);
     fill_i32(st, sk.lp_scales.as<int>(), (int64_t)(sbytes / 4), 0x7f7f7f7f);
     sk.lp_scales_rows = rows;
+    sk.lp_scales_windows = 1;
   }
   return sk.lp_scales.as<uint8_t>();
 }
@@ -1995,7 +2009,7 @@ void quantize_window(Sketch& sk, const double* H, int64_t ldh, int64_t rows, con
 // sketch's stream.
 // The right product's NT arguments (A = R, B = [Omega; Psi]) without its
 // outputs.
-NtArgs right_args(const Sketch& sk, int64_t rows, const WinLayout& w, char* base) {
+NtArgs right_args(const Sketch& sk, int64_t rows, const WinLayout& w, char* base, size_t scale_off = 0) {
   const int64_t k = sk.k, s = sk.s, n = sk.p.n;
   const size_t oe = (size_t)(k + s) * sk.lp_ldomps;
   NtArgs a{};
@@ -2003,7 +2017,7 @@ NtArgs right_args(const Sketch& sk, int64_t rows, const WinLayout& w, char* base
   a.ah = base + w.o_r;
   a.al = w.split ? (const void*)((const __bf16*)(base + w.o_r) + w.re) : nullptr;
   a.amax_a = (w.fp8 && !w.mx) ? (const float*)(base + w.o_rmax) : nullptr;
-  if (w.mx) { a.asc = sk.lp_scales.as<uint8_t>() + w.o_rs; a.ldas = w.ldrs; }
+  if (w.mx) { a.asc = sk.lp_scales.as<uint8_t>() + scale_off + w.o_rs; a.ldas = w.ldrs; }
   a.lda = w.ldr; a.koff_a = 0;
   a.bh = sk.lp_omps.p;
   a.bl = w.split ? (const void*)(sk.lp_omps.as<__bf16>() + oe) : nullptr;
@@ -2013,9 +2027,10 @@ NtArgs right_args(const Sketch& sk, int64_t rows, const WinLayout& w, char* base
 }
 
 // left = false: the left product was done by k_quant_left_fp8; rstrip > 0:
-// R in 64-column strips of rstrip bytes.
+// R in 64-column strips of rstrip bytes; scale_off: byte offset of this
+// window's block scales in lp_scales.
 void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w, char* base, bool left = true,
-                     int64_t rstrip = 0) {
+                     int64_t rstrip = 0, size_t scale_off = 0) {
   Ctx* c = sk.ctx;
   hipStream_t st = c->stream;
   const int prec = sk.p.precision;
@@ -2043,7 +2058,7 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
   }
   {
     Timed t(c, "sketch_right");
-    NtArgs a = right_args(sk, rows, w, base);
+    NtArgs a = right_args(sk, rows, w, base, scale_off);
     a.a_strip = rstrip;
     a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
     a.gtype = w.gtype; a.gh = base + w.o_g; a.gl = base + w.o_g + w.ge * w.gsz; a.ldg = w.ldl;
@@ -2064,6 +2079,17 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.out.C2 = a.out.C1; a.out.ldc2 = s;
     gemm_nt(st, w.core_prec, a, &c->gws.buf, &t);
   }
+}
+
+void launch_quant_left(hipStream_t st, int64_t k, int64_t n, const NtArgs& a, const QlArgs& q) {
+  const dim3 grid((unsigned)cdiv(n, kQlCols));
+  switch (ql_tm(k)) {
+    case 1: hipLaunchKernelGGL(k_quant_left_fp8<1>, grid, dim3(kQlThreads), 0, st, a, q); break;
+    case 2: hipLaunchKernelGGL(k_quant_left_fp8<2>, grid, dim3(kQlThreads), 0, st, a, q); break;
+    case 4: hipLaunchKernelGGL(k_quant_left_fp8<4>, grid, dim3(kQlThreads), 0, st, a, q); break;
+    default: hipLaunchKernelGGL(k_quant_left_fp8<7>, grid, dim3(kQlThreads), 0, st, a, q); break;
+  }
+  check_launch("k_quant_left_fp8");
 }
 
 // Quantise pass and left product in one kernel (k_quant_left_fp8) for
@@ -2092,17 +2118,12 @@ bool quant_left_fp8(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int6
   // cache lines per block and step), otherwise in rows
   const int64_t rstrip = nt_picks_mx(sk.p.precision, right_args(sk, rows, w, base)) ? (rows + 1) * 64 : 0;
   *rstrip_out = rstrip;
-  QlArgs q{H, ldh, rows, n, (uint8_t*)(base + w.o_r), w.ldr, sc + w.o_rs, w.ldrs, sk.lp_ldups, rstrip};
-  const dim3 grid((unsigned)cdiv(n, kQlCols));
-  switch (ql_tm(k)) {
-    case 1: hipLaunchKernelGGL(k_quant_left_fp8<1>, grid, dim3(kQlThreads), 0, st, a, q); break;
-    case 2: hipLaunchKernelGGL(k_quant_left_fp8<2>, grid, dim3(kQlThreads), 0, st, a, q); break;
-    case 4: hipLaunchKernelGGL(k_quant_left_fp8<4>, grid, dim3(kQlThreads), 0, st, a, q); break;
-    default: hipLaunchKernelGGL(k_quant_left_fp8<7>, grid, dim3(kQlThreads), 0, st, a, q); break;
-  }
-  check_launch("k_quant_left_fp8");
+  QlArgs q{H, ldh, rows, n, (uint8_t*)(base + w.o_r), w.ldr, sc + w.o_rs, w.ldrs, sk.lp_ldups, rstrip,
+           rows, 0, 0};
+  launch_quant_left(st, k, n, a, q);
   return true;
 }
+
 
 }  // namespace
 
@@ -2117,6 +2138,57 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
   }
   quantize_window(sk, H, ldh, rows, w, base, sk.ctx->stream);
   window_products(sk, row0, rows, w, base);
+}
+
+bool update_dense_all_lowp(Sketch& sk, const double* A, int64_t lda) {
+  const int64_t m = sk.p.m, n = sk.p.n, k = sk.k, wr = sk.p.window;
+  const int prec = sk.p.precision;
+  // nu = 1: the windows' left products are one plain sum (other nu would
+  // weigh them in the FP32 accumulators, not exactly as the FP64 state does)
+  if (prec != SKEMA_PREC_FP8 || sk.sparse_maps || sk.p.nu != 1.0) return false;
+  // several windows of whole 128-row steps (one window takes the per-window path)
+  if (wr <= 0 || wr % kQlRows != 0 || wr >= m) return false;
+  if (!ql_tm(k) || (m & 1) || (lda & 1) || ((uintptr_t)A & 15) || n < 1) return false;
+  // 32-bit offsets: the lane's rows of a column, Upsilon rows
+  if (m * 8 >= ((int64_t)1 << 31) || k * sk.lp_ldups >= ((int64_t)1 << 31)) return false;
+  Ctx* c = sk.ctx;
+  hipStream_t st = c->stream;
+  const int64_t W = cdiv(m, wr), last = m - (W - 1) * wr;
+  // every window in the full window's layout, one after another
+  const WinLayout w = win_layout(sk, 0, wr);
+  const size_t stride = align_up(w.total, 256);
+  sk.win.reserve((size_t)W * stride);
+  char* base = sk.win.as<char>();
+  const size_t sb = align_up(w.o_rs + align_up((size_t)(wr + 1) * w.ldrs, 256), 256);
+  if (sk.lp_scales.bytes < (size_t)W * sb || sk.lp_scales_rows != wr || sk.lp_scales_windows != W) {
+    sk.lp_scales.reserve((size_t)W * sb);
+    fill_i32(st, sk.lp_scales.as<int>(), (int64_t)((size_t)W * sb / 4), 0x7f7f7f7f);
+    sk.lp_scales_rows = wr;
+    sk.lp_scales_windows = W;
+  }
+  // R in 64-column strips when every window's right product runs on k_mx
+  const bool strips = nt_picks_mx(prec, right_args(sk, wr, w, base)) && nt_picks_mx(prec, right_args(sk, last, w, base));
+  const int64_t rstrip = strips ? (wr + 1) * 64 : 0;
+  {
+    Timed t(c, "sketch_left");
+    NtArgs a{};
+    a.M = k; a.N = n; a.K = m;
+    a.ah = sk.lp_ups.p;
+    a.amax_a = (const float*)(sk.lp_ups.as<uint8_t>() + fp8_ups_amax_offset(sk));
+    a.lda = sk.lp_ldups; a.koff_a = 0;
+    // x (zero before the update, k x n with ld k) = eta sum_w Upsilon_w
+    // Q(A_w) (nu = 1): written once, no read
+    a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = sk.p.eta; a.out.beta1 = 0.0; a.out.nsplit = n;
+    a.out.C2 = a.out.C1; a.out.ldc2 = k;
+    QlArgs q{A, lda, m, n, (uint8_t*)(base + w.o_r), w.ldr, sk.lp_scales.as<uint8_t>() + w.o_rs, w.ldrs,
+             sk.lp_ldups, rstrip, wr, (int64_t)stride, (int64_t)sb};
+    launch_quant_left(st, k, n, a, q);
+  }
+  for (int64_t i = 0; i < W; ++i) {
+    const int64_t row0 = i * wr, rows = (i + 1 < W) ? wr : last;
+    window_products(sk, row0, rows, w, base + i * stride, false, rstrip, (size_t)i * sb);
+  }
+  return true;
 }
 
 void gemm_nt_bf16(hipStream_t st, int64_t M, int64_t N, int64_t K, const __bf16* A, int64_t lda, const __bf16* B,

@@ -7,6 +7,11 @@ namespace skema {
 
 void prepare_lowp_maps(Sketch& sk);
 void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t row0, int64_t rows);
+// Every window of a dense update (A: m x n, ld lda; the state zeroed) in
+// one quantise + left-product pass over all rows, then each window's right
+// and core products; false (nothing done) where that pass does not apply
+// (forms other than fp8, windows not a multiple of 128 rows, ...).
+bool update_dense_all_lowp(Sketch& sk, const double* A, int64_t lda);
 // Z <- nu Z + eta Phi[:, row0:row0+rows] G for a BF16 G (rows x s,
 // column-major, ld ldg) on the BF16 NT GEMM (bf16 form maps).
 void lowp_core_bf16(Sketch& sk, const __bf16* G, int64_t ldg, int64_t row0, int64_t rows);

@@ -131,6 +131,7 @@ struct Sketch {
   // when the window row count changes
   DevBuf lp_scales;
   int64_t lp_scales_rows = -1;
+  int64_t lp_scales_windows = 1;  // windows whose block scales lp_scales holds (one layout each)
   int64_t lp_ldups = 0, lp_ldomps = 0;
   // State.
   DevBuf X, Y, Z;  // k x n, m x k, s x s

@@ -61,6 +61,37 @@ def test_lowp_state_against_oracle(S, form, m, n, window):
     assert ex <= tol and ey <= tol and ez <= 2 * tol, (ex, ey, ez)
 
 
+@pytest.mark.parametrize("m,n,window", [(384, 200, 128), (1152, 700, 256), (6144, 8192, 2048)])
+def test_fp8_one_pass_over_all_windows(S, m, n, window):
+    """nu = 1 and windows of whole 128-row steps: fp8 quantises every window
+    and forms x in one pass (one launch), then runs each window's right and
+    core products.  (384, 200, 128): an odd step count, so the pass's padding
+    step lies past the last window; (1152, 700, 256): a short last window
+    and a ragged last 64-column strip; (6144, 8192, 2048): R in 64-column
+    strips (every window's right product on the direct-to-LDS kernel).
+    Every row and column of X and Y is checked."""
+    print(f"case fp8 one pass {m}x{n} w={window}", flush=True)
+    rng = np.random.default_rng(17)
+    A = rng.standard_normal((m, 24)) @ rng.standard_normal((24, n)) + 0.1 * rng.standard_normal((m, n))
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=10, window=window, sketch_nu=1.0, sketch_eta=1.1)
+    osk = O.SketchySVD(p)
+    osk.linear_update(A)
+    g = _run(S, A, p, osk, "fp8")
+    X, Y, Z = g.state()
+    tol = TOLS["fp8"]
+    ex, ey, ez = O.rel_err(X, osk.X), O.rel_err(Y, osk.Y), O.rel_err(Z, osk.Z)
+
+    def worst(G, R, axis):
+        return float((np.linalg.norm(G - R, axis=axis) / np.linalg.norm(R, axis=axis)).max())
+
+    wx = max(worst(X, osk.X, 0), worst(X, osk.X, 1))
+    wy = max(worst(Y, osk.Y, 0), worst(Y, osk.Y, 1))
+    print(f"fp8 one pass rel err X {ex:.3e} Y {ey:.3e} Z {ez:.3e}; worst row/column X {wx:.3e} Y {wy:.3e}",
+          flush=True)
+    assert ex <= tol and ey <= tol and ez <= 2 * tol, (ex, ey, ez)
+    assert wx <= 2 * tol and wy <= 2 * tol, (wx, wy)
+
+
 @pytest.mark.parametrize("form", ["bf16", "bf16x3", "fp32", "fp8"])
 def test_lowp_rank_r_error_close_to_fp64(S, form):
     """The finalize is FP64 on every form; the rank-r error of a mixed form
@@ -163,9 +194,10 @@ def test_lowp_direct_lds_products(S, form):
     assert wx <= 2 * tol and wy <= 2 * tol, (wx, wy)
 
 
+@pytest.mark.parametrize("nu", [0.9, 1.0])
 @pytest.mark.parametrize("form", ["fp8", "bf16", "bf16x3", "fp32"])
 @pytest.mark.parametrize("m,n,window", [(5096, 8320, 4096), (1000, 2100, 256)])
-def test_lowp_exact_on_representable_data(S, form, m, n, window):
+def test_lowp_exact_on_representable_data(S, form, m, n, window, nu):
     """Inputs every reduced form encodes exactly: window entries in {-2, ..., 2}
     times 2^(a + b), a constant over each 32-row block and b over each
     32-column block, and maps with entries in {-2, ..., 2}.  For fp8 each
@@ -176,16 +208,17 @@ def test_lowp_exact_on_representable_data(S, form, m, n, window):
     epilogue; an fp8 scale taken from the wrong block, a dropped scale or a
     misplaced tile is off by a power of two or worse.  The first shape takes
     the direct-to-LDS products, the second (256-row windows) the other
-    kernels.  The core product of fp8 and bf16 rounds its right operand G to
-    BF16, so their Z keeps the BF16 bound; split-BF16 and FP32 carry G
-    exactly, so their Z is exact too."""
-    print(f"case {form} exact {m}x{n} w={window}", flush=True)
+    kernels; with nu = 1 the fp8 form forms x in one pass over all windows.
+    The core product of fp8 and bf16 rounds its right operand G to BF16, so
+    their Z keeps the BF16 bound; split-BF16 and FP32 carry G exactly, so
+    their Z is exact too."""
+    print(f"case {form} exact {m}x{n} w={window} nu={nu}", flush=True)
     rng = np.random.default_rng(5)
     a = rng.integers(0, 3, size=-(-m // 32))
     b = rng.integers(0, 3, size=-(-n // 32))
     scale = np.exp2(np.repeat(a, 32)[:m, None] + np.repeat(b, 32)[None, :n])
     A = rng.integers(-2, 3, size=(m, n)).astype(np.float64) * scale
-    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=100, window=window, sketch_nu=0.9, sketch_eta=1.1)
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=100, window=window, sketch_nu=nu, sketch_eta=1.1)
     maps = O.make_maps(p)
     for mp in maps.values():
         mp.data = rng.integers(-2, 3, size=mp.data.shape).astype(np.float64)
