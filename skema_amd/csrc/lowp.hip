@@ -1249,9 +1249,9 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
 //  - waves 4-7 (MFMA waves) hold the accumulators: wave 4 + m owns Upsilon
 //    rows [16 TM m, 16 TM (m + 1)) against the tile's four 16-column groups
 //    (tiles past the map's rows are computed on zero rows, never stored),
-//    its Upsilon fragments copied to LDS by DMA one step ahead (the window's
-//    slice of Upsilon, at most 448 x w bytes, stays in L2), and the L tile
-//    read after the second barrier.  The L tile is double
+//    its Upsilon fragments copied to LDS by DMA one step ahead, between the
+//    barriers (the window's slice of Upsilon, at most 448 x w bytes, stays
+//    in L2), and the L tile read after the second barrier.  The L tile is double
 //    buffered, so the tile waves quantise the next step while the MFMAs run.
 // Blocks start at different steps (block b at step b mod the steps of a
 // window) and wrap around, so the blocks in flight spread over the rows.
@@ -1467,8 +1467,12 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
     for (int i = 0; i < nsteps; ++i) {
       const int buf = i & 1;
       lds_barrier();  // 1
-      lds_barrier();  // 2: the step's L tile is complete
-      wait_vm<0>();   // this wave's fragment copies (issued a step ago)
+      // While the tile waves quantise: this step's fragments into registers
+      // (their copy was issued a step ago), then the next step's copy, whose
+      // issue may wait behind the tile waves' loads; after barrier 2 only the
+      // L tile reads and the MFMAs remain, short enough not to hold the tile
+      // waves at the next barrier 1.
+      wait_vm<0>();
       i8v u[TM];
 #pragma unroll
       for (int x = 0; x < TM; ++x) {
@@ -1476,24 +1480,21 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
         const i4v hi = *reinterpret_cast<const i4v*>(ureg + (16 * x + l16) * 128 + so1);
         u[x] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
       }
-      i8v bf[4];
-      int sb[4];
+      // the fragments are in registers before the next copy lands in ureg
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      issue_u(step_of(i + 1 < nsteps ? i + 1 : 0));
+      lds_barrier();  // 2: the step's L tile is complete
 #pragma unroll
       for (int y = 0; y < 4; ++y) {
         const int col = 16 * y + l16;
         const i4v lo = *reinterpret_cast<const i4v*>(&Lt[buf][col][so0]);
         const i4v hi = *reinterpret_cast<const i4v*>(&Lt[buf][col][so1]);
-        bf[y] = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-        sb[y] = sc[buf][col][lq];
-      }
-      // the fragments are in registers before the next copy lands in ureg
-      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-      issue_u(step_of(i + 1 < nsteps ? i + 1 : 0));
-#pragma unroll
-      for (int y = 0; y < 4; ++y)
+        const i8v bf = i8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        const int sb = sc[buf][col][lq];
 #pragma unroll
         for (int x = 0; x < TM; ++x)
-          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(u[x], bf[y], acc[x][y], 0, 0, 0, 127, 0, sb[y]);
+          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(u[x], bf, acc[x][y], 0, 0, 0, 127, 0, sb);
+      }
     }
     wait_vm<0>();  // the last (unused) copy lands before su is reused
   }
