@@ -143,17 +143,19 @@ def host_cpu():
     return {"cpu_model": model, "host_cores_available": avail, "host_cpus_online": os.cpu_count()}
 
 
-def right_product(cfg):
-    """(name, algorithmic flops per launch) of [y | G] = A_w [Omega; Psi]^T,
-    2 w n (k + s) flops per window."""
-    return "sketch_right", 2.0 * cfg["window"] * cfg["n"] * (cfg["k"] + cfg["s"])
-
-
 def one_pass_fp8(cfg, form):
-    """Whether the fp8 update runs its quantisation and left product as one
-    pass over all windows (nu = 1, windows of whole 128-row steps, k <= 448)."""
+    """Whether the fp8 update runs as one pass over the whole matrix (nu = 1,
+    windows of whole 128-row steps, k <= 448): quantisation and left product
+    in one launch, then one right and one core product over all m rows."""
     w, m = cfg["window"], cfg["m"]
     return form == "fp8" and w % 128 == 0 and w < m and cfg["k"] <= 448 and m % 2 == 0
+
+
+def right_product(cfg, form):
+    """(name, algorithmic flops per launch) of [y | G] = A_w [Omega; Psi]^T,
+    2 w n (k + s) flops per window (2 m n (k + s) in the one-pass form)."""
+    rows = cfg["m"] if one_pass_fp8(cfg, form) else cfg["window"]
+    return "sketch_right", 2.0 * rows * cfg["n"] * (cfg["k"] + cfg["s"])
 
 
 def dominant_kernel(cfg, form):
@@ -165,7 +167,7 @@ def dominant_kernel(cfg, form):
     if one_pass_fp8(cfg, form):
         m, n, k = cfg["m"], cfg["n"], cfg["k"]
         return "sketch_left", m * n * (8.0 + 1.0 + 1.0 / 32.0) + 8.0 * k * n + 1.0 * k * m, "hbm"
-    name, flops = right_product(cfg)
+    name, flops = right_product(cfg, form)
     return name, flops, "mfma"
 
 
@@ -747,7 +749,7 @@ def bench_dense(args, S, torch):
                 "frac": (achieved / PEAKS_TFLOPS[form]) if achieved else None, "traffic": committed_traffic(form),
                 "flops_per_launch": mfma_flops, "avg_launch_ms": best["kernel_avg_ms"]}
     # the right product's own figure (from the profiled breakdown step)
-    rname, rflops = right_product(cfg)
+    rname, rflops = right_product(cfg, form)
     rms = best["kernel_ms"].get(rname)
     right_roof = None
     if rms:

@@ -602,11 +602,15 @@ int skema_linear_update_dense(skema_sketch* h, const double* A, int64_t lda) {
     set_device(sk.ctx);
     SKEMA_REQUIRE(sk.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "linear_update_dense on an RBF sketch");
     check_mat(A, sk.p.m, sk.p.n, lda, "linear_update_dense A");
-    zero_state(sk);
-    // fp8: one quantise + left-product pass over every window, then the
-    // windows' right and core products
-    if (!update_dense_all_lowp(sk, A, lda))
+    // fp8 with nu = 1: one quantise + left-product pass over the whole
+    // matrix, then one right and one core product; they write X, Y and Z
+    // whole without reading them, so the state is not cleared first
+    if (update_dense_all_lowp(sk, A, lda)) {
+      sk.finalized = false;
+    } else {
+      zero_state(sk);
       for_each_window(sk, [&](int64_t r0, int64_t w) { update_dense_window(sk, A + r0, lda, r0, w); });
+    }
     sk.ctx->sync_if_needed();
   });
 }

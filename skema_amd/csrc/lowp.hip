@@ -1317,9 +1317,10 @@ __global__ __launch_bounds__(kQlThreads, 1) void k_quant_left_fp8(NtArgs g, QlAr
   // rows, zeroes them, writes no R row and adds zero products
   const int nsteps = (int)((((rows + kQlRows - 1) / kQlRows) + 1) & ~1);
   const int spw = q.rwin ? (int)(q.wrows / kQlRows) : nsteps;  // steps per window
-  // staggered over a window's steps at most, so the blocks in flight read
-  // one or two windows' slices of Upsilon (which stay in L2)
-  const int s0 = (int)(blockIdx.x % (unsigned)(spw < nsteps ? spw : nsteps));
+  // start staggered over 32 steps (4096 rows) at most, so the blocks in
+  // flight read one or two 4096-column slices of Upsilon (which stay in L2)
+  const int sspan = spw < 32 ? spw : 32;
+  const int s0 = (int)(blockIdx.x % (unsigned)(sspan < nsteps ? sspan : nsteps));
   auto step_of = [&](int i) { return (i + s0 < nsteps) ? i + s0 : i + s0 - nsteps; };
 
   f4v acc[TM][4];
@@ -1779,8 +1780,6 @@ void launch_nt(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
   else launch_nt_tile<64, 64>(st, prec, a, grid);
 }
 
-// t (optional): a timing scope closed right after the product kernel, so
-// it covers that kernel and not the split-K reduction that follows.
 // Whether gemm_nt runs this product on the direct-to-LDS kernel k_mx (the
 // one consumer of A in 64-byte column strips).
 bool nt_picks_mx(int prec, const NtArgs& a) {
@@ -1790,7 +1789,11 @@ bool nt_picks_mx(int prec, const NtArgs& a) {
   return pl.ns > 0 && ntx_eligible(prec, a) && a.K % kstep_mx == 0 && launch_mx(0, prec, a, pl, dim3(), true);
 }
 
-void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr) {
+// t (optional): a timing scope closed right after the product kernel, so
+// it covers that kernel and not the split-K reduction that follows.
+// ws = nullptr: no K split.  fill: as many K splits as fill the 256 CUs
+// once with the tiles (at least the plan's; for a long K over few tiles).
+void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr, bool fill = false) {
   SKEMA_REQUIRE(!a.a_strip || nt_picks_mx(prec, a), kErrUnsupported, "A in column strips needs the k_mx kernel");
   NtPlan pl = plan_nt(prec, a.M, a.N, a.K);
   SKEMA_REQUIRE(!(a.asc && a.bsc), kErrUnsupported, "block scales on both NT operands");
@@ -1808,6 +1811,10 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr)
   if (pl.ns > 0) { pl.bm = 16 * pl.tm * pl.wm; pl.bn = 16 * pl.tn * pl.wn; }
   else if (pl.tm > 0) { pl.bm = 64 * pl.tm; pl.bn = 16 * pl.tn; }
   int splits = ws ? pl.splits : 1;
+  if (ws && fill) {
+    const int64_t tiles = cdiv(a.M, pl.bm) * cdiv(a.N, pl.bn);
+    splits = (int)std::max<int64_t>(splits, std::min<int64_t>(256 / std::max<int64_t>(tiles, 1), a.K / (8 * kstep)));
+  }
   // block-scaled operands: splits start on a 4-step boundary (one scale dword)
   const int64_t kgrain = (a.asc || a.bsc) ? 4 * kstep : kstep;
   a.k_per_split = cdiv(cdiv(a.K, splits), kgrain) * kgrain;
@@ -2142,34 +2149,47 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
 }
 
 bool update_dense_all_lowp(Sketch& sk, const double* A, int64_t lda) {
-  const int64_t m = sk.p.m, n = sk.p.n, k = sk.k, wr = sk.p.window;
+  const int64_t m = sk.p.m, n = sk.p.n, k = sk.k, s = sk.s, wr = sk.p.window;
   const int prec = sk.p.precision;
-  // nu = 1: the windows' left products are one plain sum (other nu would
-  // weigh them in the FP32 accumulators, not exactly as the FP64 state does)
+  // nu = 1: the windows' products are plain sums over the windows (other nu
+  // would weigh them in the FP32 accumulators, not exactly as the FP64 state
+  // does)
   if (prec != SKEMA_PREC_FP8 || sk.sparse_maps || sk.p.nu != 1.0) return false;
-  // several windows of whole 128-row steps (one window takes the per-window path)
+  // several windows of whole 128-row steps (one window takes the per-window
+  // path): the block scales of L (32-row blocks) and R (per row) are then
+  // the same whether the blocks are counted from each window's first row or
+  // from the matrix's
   if (wr <= 0 || wr % kQlRows != 0 || wr >= m) return false;
   if (!ql_tm(k) || (m & 1) || (lda & 1) || ((uintptr_t)A & 15) || n < 1) return false;
   // 32-bit offsets: the lane's rows of a column, Upsilon rows
   if (m * 8 >= ((int64_t)1 << 31) || k * sk.lp_ldups >= ((int64_t)1 << 31)) return false;
   Ctx* c = sk.ctx;
   hipStream_t st = c->stream;
-  const int64_t W = cdiv(m, wr), last = m - (W - 1) * wr;
-  // every window in the full window's layout, one after another
-  const WinLayout w = win_layout(sk, 0, wr);
-  const size_t stride = align_up(w.total, 256);
-  sk.win.reserve((size_t)W * stride);
+  const double eta = sk.p.eta;
+  // The whole matrix as one window: R (m rows + the pass's sink row), its
+  // block scales and G (m x s); no L (the pass builds L in LDS).
+  WinLayout w = win_layout(sk, 0, m);
+  w.o_l = 0;
+  w.o_r = 0;
+  w.o_rs = 0;
+  // R in 64-column strips (256-byte aligned) when the right product runs on
+  // k_mx
+  const int64_t rstrip = nt_picks_mx(prec, right_args(sk, m, w, nullptr)) ? (int64_t)align_up((size_t)(m + 1) * 64, 256) : 0;
+  if (rstrip) w.re = (size_t)(w.ldr / 64) * rstrip;
+  w.o_rmax = w.o_cmax = w.o_g = align_up(w.re * w.esz, 256);
+  w.total = w.o_g + align_up(w.ge * w.gsz, 256);
+  sk.win.reserve(w.total);
   char* base = sk.win.as<char>();
-  const size_t sb = align_up(w.o_rs + align_up((size_t)(wr + 1) * w.ldrs, 256), 256);
-  if (sk.lp_scales.bytes < (size_t)W * sb || sk.lp_scales_rows != wr || sk.lp_scales_windows != W) {
-    sk.lp_scales.reserve((size_t)W * sb);
-    fill_i32(st, sk.lp_scales.as<int>(), (int64_t)((size_t)W * sb / 4), 0x7f7f7f7f);
-    sk.lp_scales_rows = wr;
-    sk.lp_scales_windows = W;
+  // unit scales (0x7f) in the K padding of R's rows; refilled when the
+  // layout changes (lp_scales_windows 0 marks this one)
+  const size_t sbytes = align_up((size_t)(m + 1) * w.ldrs, 256);
+  if (sk.lp_scales.bytes < sbytes || sk.lp_scales_rows != m || sk.lp_scales_windows != 0) {
+    sk.lp_scales.reserve(sbytes);
+    fill_i32(st, sk.lp_scales.as<int>(), (int64_t)(sbytes / 4), 0x7f7f7f7f);
+    sk.lp_scales_rows = m;
+    sk.lp_scales_windows = 0;
   }
-  // R in 64-column strips when every window's right product runs on k_mx
-  const bool strips = nt_picks_mx(prec, right_args(sk, wr, w, base)) && nt_picks_mx(prec, right_args(sk, last, w, base));
-  const int64_t rstrip = strips ? (wr + 1) * 64 : 0;
+  NtArgs r = right_args(sk, m, w, base);
   {
     Timed t(c, "sketch_left");
     NtArgs a{};
@@ -2177,17 +2197,36 @@ bool update_dense_all_lowp(Sketch& sk, const double* A, int64_t lda) {
     a.ah = sk.lp_ups.p;
     a.amax_a = (const float*)(sk.lp_ups.as<uint8_t>() + fp8_ups_amax_offset(sk));
     a.lda = sk.lp_ldups; a.koff_a = 0;
-    // x (zero before the update, k x n with ld k) = eta sum_w Upsilon_w
-    // Q(A_w) (nu = 1): written once, no read
-    a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = sk.p.eta; a.out.beta1 = 0.0; a.out.nsplit = n;
+    // x (zero before the update, k x n with ld k) = eta Upsilon Q(A)
+    // (nu = 1): written once, no read
+    a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = 0.0; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
     QlArgs q{A, lda, m, n, (uint8_t*)(base + w.o_r), w.ldr, sk.lp_scales.as<uint8_t>() + w.o_rs, w.ldrs,
-             sk.lp_ldups, rstrip, wr, (int64_t)stride, (int64_t)sb};
+             sk.lp_ldups, rstrip, m, 0, 0};
     launch_quant_left(st, k, n, a, q);
   }
-  for (int64_t i = 0; i < W; ++i) {
-    const int64_t row0 = i * wr, rows = (i + 1 < W) ? wr : last;
-    window_products(sk, row0, rows, w, base + i * stride, false, rstrip, (size_t)i * sb);
+  {
+    // [y | G] = Q(A) [Omega; Psi]^T over all m rows: enough tiles to fill the
+    // chip without a K split; y (zero before the update) written once
+    Timed t(c, "sketch_right");
+    r.a_strip = rstrip;
+    r.out.C1 = sk.Y.as<double>(); r.out.ldc1 = m; r.out.alpha1 = eta; r.out.beta1 = 0.0; r.out.nsplit = k;
+    r.gtype = w.gtype; r.gh = base + w.o_g; r.gl = nullptr; r.ldg = w.ldl;
+    gemm_nt(st, prec, r, nullptr, &t);
+  }
+  {
+    // z = eta Phi G (K = m), K split so the tiles times the splits fill
+    // the CUs once
+    Timed t(c, "sketch_core");
+    NtArgs a{};
+    a.M = s; a.N = s; a.K = m;
+    a.ah = sk.lp_phi.p;
+    a.lda = sk.lp_ldups; a.koff_a = 0;
+    a.bh = base + w.o_g;
+    a.ldb = w.ldl;
+    a.out.C1 = sk.Z.as<double>(); a.out.ldc1 = s; a.out.alpha1 = eta; a.out.beta1 = 0.0; a.out.nsplit = s;
+    a.out.C2 = a.out.C1; a.out.ldc2 = s;
+    gemm_nt(st, w.core_prec, a, &c->gws.buf, &t, true);
   }
   return true;
 }
