@@ -61,8 +61,9 @@ def test_lowp_state_against_oracle(S, form, m, n, window):
     assert ex <= tol and ey <= tol and ez <= 2 * tol, (ex, ey, ez)
 
 
-@pytest.mark.parametrize("m,n,window", [(384, 200, 128), (1152, 700, 256), (6144, 8192, 2048)])
-def test_fp8_one_pass_over_all_windows(S, m, n, window):
+@pytest.mark.parametrize("m,n,window,rank", [(384, 200, 128, 10), (1152, 700, 256, 10), (6144, 8192, 2048, 10),
+                                             (16514, 8192, 4096, 100)])
+def test_fp8_one_pass_over_all_windows(S, m, n, window, rank):
     """nu = 1 and windows of whole 128-row steps: fp8 quantises the whole
     matrix and forms x in one pass (one launch), then runs one right product
     over all m rows and one core product (K = m); the windows only split the
@@ -70,13 +71,17 @@ def test_fp8_one_pass_over_all_windows(S, m, n, window):
     (384, 200, 128): an odd step count, so the pass's padding step lies past
     the last row; (1152, 700, 256): a short last window and a ragged last
     64-column strip; (6144, 8192, 2048): R in 64-column strips (the right
-    product on the direct-to-LDS kernel).  The state is not cleared before
+    product on the direct-to-LDS kernel); (16514, 8192, 4096) at rank 100
+    (k = 401, s = 803, the largest k the pass takes): one right product of
+    65 x 5 tiles of 256 x 256 with no K split, a ragged last row tile and a
+    ragged last column tile of [Omega; Psi], and the core product's K split
+    chosen to fill the CUs.  The state is not cleared before
     this path (it writes X, Y and Z whole): a first update with another
     matrix must leave no trace.  Every row and column of X and Y is checked."""
     print(f"case fp8 one pass {m}x{n} w={window}", flush=True)
     rng = np.random.default_rng(17)
     A = rng.standard_normal((m, 24)) @ rng.standard_normal((24, n)) + 0.1 * rng.standard_normal((m, n))
-    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=10, window=window, sketch_nu=1.0, sketch_eta=1.1)
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=rank, window=window, sketch_nu=1.0, sketch_eta=1.1)
     osk = O.SketchySVD(p)
     osk.linear_update(A)
     g = _run(S, 3.0 * rng.standard_normal((m, n)), p, osk, "fp8")
