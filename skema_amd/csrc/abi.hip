@@ -602,9 +602,11 @@ int skema_linear_update_dense(skema_sketch* h, const double* A, int64_t lda) {
     set_device(sk.ctx);
     SKEMA_REQUIRE(sk.p.kernel == SKEMA_KERNEL_NONE, kErrArg, "linear_update_dense on an RBF sketch");
     check_mat(A, sk.p.m, sk.p.n, lda, "linear_update_dense A");
-    // fp8 with nu = 1: one quantise + left-product pass over the whole
-    // matrix, then one right and one core product; they write X, Y and Z
-    // whole without reading them, so the state is not cleared first
+    // nu = 1 in the fp8, bf16 and split-BF16 forms: the whole matrix as one
+    // window (fp8: one quantise + left-product pass, then one right and one
+    // core product; bf16 forms: one quantise pass and the three products);
+    // they write X, Y and Z whole without reading them, so the state is not
+    // cleared first
     if (update_dense_all_lowp(sk, A, lda)) {
       sk.finalized = false;
     } else {

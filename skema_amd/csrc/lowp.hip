@@ -1811,6 +1811,10 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr,
   if (pl.ns > 0) { pl.bm = 16 * pl.tm * pl.wm; pl.bn = 16 * pl.tn * pl.wn; }
   else if (pl.tm > 0) { pl.bm = 64 * pl.tm; pl.bn = 16 * pl.tn; }
   int splits = ws ? pl.splits : 1;
+  // a large output (the right product over a whole matrix) with enough
+  // tiles to fill the CUs four times over: no K split (its FP32 partials
+  // would be the output's size times the splits)
+  if (a.M * a.N >= ((int64_t)1 << 24) && cdiv(a.M, pl.bm) * cdiv(a.N, pl.bn) >= 1024) splits = 1;
   if (ws && fill) {
     const int64_t tiles = cdiv(a.M, pl.bm) * cdiv(a.N, pl.bn);
     splits = (int)std::max<int64_t>(splits, std::min<int64_t>(256 / std::max<int64_t>(tiles, 1), a.K / (8 * kstep)));
@@ -2036,14 +2040,16 @@ NtArgs right_args(const Sketch& sk, int64_t rows, const WinLayout& w, char* base
 
 // left = false: the left product was done by k_quant_left_fp8; rstrip > 0:
 // R in 64-column strips of rstrip bytes; scale_off: byte offset of this
-// window's block scales in lp_scales.
+// window's block scales in lp_scales.  whole: the window is the whole matrix
+// of a nu = 1 update: X, Y and Z are written (not read), the core product's
+// K split fills the CUs.
 void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w, char* base, bool left = true,
-                     int64_t rstrip = 0, size_t scale_off = 0) {
+                     int64_t rstrip = 0, size_t scale_off = 0, bool whole = false) {
   Ctx* c = sk.ctx;
   hipStream_t st = c->stream;
   const int prec = sk.p.precision;
   const int64_t k = sk.k, s = sk.s, m = sk.p.m, n = sk.p.n;
-  const double eta = sk.p.eta, nu = sk.p.nu;
+  const double eta = sk.p.eta, nu = whole ? 0.0 : sk.p.nu;
   const bool fp8 = w.fp8, split = w.split, mx = w.mx;
   float* cmax = (float*)(base + w.o_cmax);
   const size_t ue = (size_t)k * sk.lp_ldups;
@@ -2085,7 +2091,7 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.ldb = w.ldl;
     a.out.C1 = sk.Z.as<double>(); a.out.ldc1 = s; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = s;
     a.out.C2 = a.out.C1; a.out.ldc2 = s;
-    gemm_nt(st, w.core_prec, a, &c->gws.buf, &t);
+    gemm_nt(st, w.core_prec, a, &c->gws.buf, &t, whole);
   }
 }
 
@@ -2148,9 +2154,31 @@ void update_dense_window_lowp(Sketch& sk, const double* H, int64_t ldh, int64_t 
   window_products(sk, row0, rows, w, base);
 }
 
+// bf16 and split-BF16 at nu = 1: the quantisation is elementwise, so one
+// quantise pass over the whole matrix and the three products over all rows
+// give the windows' sums up to FP32 summation order.  Taken when the whole
+// matrix's L, R and G copies fit in kWholeCopyBytes.  Not fp32: its left
+// product would accumulate K = m terms in FP32 instead of K = w per window
+// (summed in FP64), 4x its per-window error at 65536 x 16384.
+constexpr size_t kWholeCopyBytes = (size_t)32 << 30;
+bool update_dense_whole_lowp(Sketch& sk, const double* A, int64_t lda) {
+  const int64_t m = sk.p.m, wr = sk.p.window;
+  const int prec = sk.p.precision;
+  if (prec != SKEMA_PREC_BF16 && prec != SKEMA_PREC_BF16X3) return false;
+  if (sk.sparse_maps || sk.p.nu != 1.0 || wr <= 0 || wr >= m) return false;
+  const WinLayout w = win_layout(sk, 0, m);
+  if (w.total > kWholeCopyBytes) return false;
+  sk.win.reserve(w.total);
+  char* base = sk.win.as<char>();
+  quantize_window(sk, A, lda, m, w, base, sk.ctx->stream);
+  window_products(sk, 0, m, w, base, true, 0, 0, true);
+  return true;
+}
+
 bool update_dense_all_lowp(Sketch& sk, const double* A, int64_t lda) {
   const int64_t m = sk.p.m, n = sk.p.n, k = sk.k, s = sk.s, wr = sk.p.window;
   const int prec = sk.p.precision;
+  if (update_dense_whole_lowp(sk, A, lda)) return true;
   // nu = 1: the windows' products are plain sums over the windows (other nu
   // would weigh them in the FP32 accumulators, not exactly as the FP64 state
   // does)

@@ -101,6 +101,41 @@ def test_fp8_one_pass_over_all_windows(S, m, n, window, rank):
     assert wx <= 2 * tol and wy <= 2 * tol, (wx, wy)
 
 
+@pytest.mark.parametrize("form", ["bf16", "bf16x3", "fp32"])
+@pytest.mark.parametrize("m,n,window,rank", [(1154, 700, 256, 10), (16514, 8192, 4096, 100)])
+def test_whole_matrix_update_nu1(S, form, m, n, window, rank):
+    """nu = 1 with several windows: bf16 and split-BF16 quantise the whole
+    matrix once (their rounding is elementwise, so windows change nothing
+    but the FP32 summation order) and run the left, right and core products
+    over all rows (fp32 keeps the window loop, checked here the same way):
+    (1154, 700, 256) ragged in every dimension;
+    (16514, 8192, 4096) at rank 100 (k = 401, s = 803) with the right product
+    unsplit over 16514 rows and the core's K split filling the CUs.  A first
+    update with another matrix must leave no trace (X, Y and Z are written,
+    not accumulated).  Every row and column of X and Y is checked."""
+    print(f"case whole-matrix {form} {m}x{n} w={window} rank={rank}", flush=True)
+    rng = np.random.default_rng(19)
+    A = rng.standard_normal((m, 24)) @ rng.standard_normal((24, n)) + 0.1 * rng.standard_normal((m, n))
+    p = O.AlgParams(matrix_m=m, matrix_n=n, rank=rank, window=window, sketch_nu=1.0, sketch_eta=1.1)
+    osk = O.SketchySVD(p)
+    osk.linear_update(A)
+    g = _run(S, 3.0 * rng.standard_normal((m, n)), p, osk, form)
+    g.linear_update(dev(A))
+    X, Y, Z = g.state()
+    tol = TOLS[form]
+    ex, ey, ez = O.rel_err(X, osk.X), O.rel_err(Y, osk.Y), O.rel_err(Z, osk.Z)
+
+    def worst(G, R, axis):
+        return float((np.linalg.norm(G - R, axis=axis) / np.linalg.norm(R, axis=axis)).max())
+
+    wx = max(worst(X, osk.X, 0), worst(X, osk.X, 1))
+    wy = max(worst(Y, osk.Y, 0), worst(Y, osk.Y, 1))
+    print(f"{form} whole rel err X {ex:.3e} Y {ey:.3e} Z {ez:.3e}; worst row/column X {wx:.3e} Y {wy:.3e}",
+          flush=True)
+    assert ex <= tol and ey <= tol and ez <= 2 * tol, (ex, ey, ez)
+    assert wx <= 4 * tol and wy <= 4 * tol, (wx, wy)
+
+
 @pytest.mark.parametrize("form", ["bf16", "bf16x3", "fp32", "fp8"])
 def test_lowp_rank_r_error_close_to_fp64(S, form):
     """The finalize is FP64 on every form; the rank-r error of a mixed form
