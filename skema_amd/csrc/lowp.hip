@@ -381,6 +381,12 @@ struct NtArgs {
   // when > 0 -- row r, K byte k at (k / 64) a_strip + 64 r + k % 64; a_strip
   // = bytes per strip
   int64_t a_strip;
+  // k_mx<..., SEG> (bf16): K in three segments of seg_steps 128-byte steps
+  // each, segment z read at byte offset seg_da[z] (A) / seg_db[z] (B) from
+  // ah / bh -- split-BF16 as one bf16 product over [Ah | Ah | Al] and
+  // [Bh | Bl | Bh] with nothing copied
+  int seg_steps;
+  int64_t seg_da0, seg_da1, seg_da2, seg_db0, seg_db1, seg_db2;
 };
 
 // Rows i0 .. i0+3 of column j (the four accumulator values of one lane in
@@ -1017,7 +1023,7 @@ __global__ __launch_bounds__(kThreads) void k_ntx(NtArgs g) {
 // would drain every load in flight.
 // Fragment K assignment and block scales as in k_ntx.  Requires 16-byte
 // aligned rows and every split's K range a multiple of 128 bytes.
-template <int FORM, int WM, int WN, int TM, int TN, int NS, bool ASC, bool BSC>
+template <int FORM, int WM, int WN, int TM, int TN, int NS, bool ASC, bool BSC, bool SEG = false>
 __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   constexpr int NW = WM * WN;  // waves: WM along M by WN along N
   constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN, SB = 128;
@@ -1095,7 +1101,15 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   }
   auto issue = [&](int st) {
     uint8_t* slot = smem + (st % NS) * STAGE;
-    const int64_t off = (int64_t)st * SB, offa = (int64_t)st * astep;
+    int64_t off = (int64_t)st * SB, offa = (int64_t)st * astep;
+    if constexpr (SEG) {
+      // K segments (no K split): the step within its segment, from that
+      // segment's plane
+      const int z = st / g.seg_steps;
+      const int64_t r = (int64_t)(st - z * g.seg_steps) * SB;
+      offa = r + (z == 0 ? g.seg_da0 : (z == 1 ? g.seg_da1 : g.seg_da2));
+      off = r + (z == 0 ? g.seg_db0 : (z == 1 ? g.seg_db1 : g.seg_db2));
+    }
 #pragma unroll
     for (int i = 0; i < IA; ++i)
       __builtin_amdgcn_global_load_lds((const void*)(ga[i] + offa),
@@ -1711,9 +1725,9 @@ void launch_ntx_tile(hipStream_t st, int prec, const NtArgs& a, dim3 grid) {
     hipLaunchKernelGGL((k_ntx<kNxBf16, TM, TN, false, false>), grid, dim3(kThreads), 0, st, a);
   }
 }
-template <int FORM, int TM, int TN, int NS, bool ASC, bool BSC, int WM = 2, int WN = 2>
+template <int FORM, int TM, int TN, int NS, bool ASC, bool BSC, int WM = 2, int WN = 2, bool SEG = false>
 void launch_mx_t(hipStream_t st, const NtArgs& a, dim3 grid) {
-  hipLaunchKernelGGL((k_mx<FORM, WM, WN, TM, TN, NS, ASC, BSC>), grid, dim3(64 * WM * WN), 0, st, a);
+  hipLaunchKernelGGL((k_mx<FORM, WM, WN, TM, TN, NS, ASC, BSC, SEG>), grid, dim3(64 * WM * WN), 0, st, a);
 }
 // k_mx variants built (the ones the plans select; launch_mx returns false
 // for any other combination):
@@ -1739,7 +1753,10 @@ bool launch_mx(hipStream_t st, int prec, const NtArgs& a, const NtPlan& pl, dim3
   }
   if (prec == SKEMA_PREC_BF16 && !a.asc && !a.bsc) {
     if (w24 && key == 842) {
-      if (!dry) launch_mx_t<kNxBf16, 8, 4, 2, false, false, 2, 4>(st, a, grid);
+      if (!dry) {
+        if (a.seg_steps) launch_mx_t<kNxBf16, 8, 4, 2, false, false, 2, 4, true>(st, a, grid);
+        else launch_mx_t<kNxBf16, 8, 4, 2, false, false, 2, 4>(st, a, grid);
+      }
       return true;
     }
     if (w42 && key == 443) {
@@ -1831,6 +1848,9 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr,
   a.tiles_m = (int)cdiv(a.M, pl.bm);
   // wide kernels: keep the larger operand's rows within an XCD
   a.n_fast = (a.M >= a.N) ? 1 : 0;
+  SKEMA_REQUIRE(!a.seg_steps || (prec == SKEMA_PREC_BF16 && pl.ns == 2 && pl.tm == 8 && pl.tn == 4 && pl.wm == 2 &&
+                                 pl.wn == 4 && splits == 1),
+                kErrUnsupported, "K segments need the 256 x 256 bf16 k_mx without a K split");
   const dim3 grid((unsigned)(a.tiles_m * cdiv(a.N, pl.bn)), 1, (unsigned)splits);
   launch_nt(st, prec, a, pl, grid);
   check_launch("k_nt");
@@ -2038,6 +2058,14 @@ NtArgs right_args(const Sketch& sk, int64_t rows, const WinLayout& w, char* base
   return a;
 }
 
+// Whether a bf16 product (a: K already the three segments' length) runs on
+// the 256 x 256 k_mx, the one kernel that reads K segments.
+bool seg_kernel_ok(const NtArgs& a) {
+  const NtPlan pl = plan_nt(SKEMA_PREC_BF16, a.M, a.N, a.K);
+  return pl.ns == 2 && pl.tm == 8 && pl.tn == 4 && pl.wm == 2 && pl.wn == 4 && a.K % 64 == 0 &&
+         ntx_eligible(SKEMA_PREC_BF16, a);
+}
+
 // left = false: the left product was done by k_quant_left_fp8; rstrip > 0:
 // R in 64-column strips of rstrip bytes; scale_off: byte offset of this
 // window's block scales in lp_scales.  whole: the window is the whole matrix
@@ -2076,7 +2104,25 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.a_strip = rstrip;
     a.out.C1 = sk.Y.as<double>() + row0; a.out.ldc1 = m; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = k;
     a.gtype = w.gtype; a.gh = base + w.o_g; a.gl = base + w.o_g + w.ge * w.gsz; a.ldg = w.ldl;
-    gemm_nt(st, prec, a, &c->gws.buf, &t);
+    bool done = false;
+    if (whole && split && n % 64 == 0) {
+      // split-BF16 over the whole matrix: hi.hi + hi.lo + lo.hi as one bf16
+      // product over K segments [Ah | Ah | Al] x [Bh | Bl | Bh] (whole
+      // 64-element steps, no padding inside a segment), on the 256 x 256
+      // kernel (no K split) instead of the two-plane 128 x 128 one
+      NtArgs sgm = a;
+      const int64_t segs = n / 64;
+      sgm.K = 3 * segs * 64;
+      sgm.seg_steps = (int)segs;
+      sgm.seg_da0 = 0; sgm.seg_da1 = 0; sgm.seg_da2 = static_cast<const char*>(a.al) - static_cast<const char*>(a.ah);
+      sgm.seg_db0 = 0; sgm.seg_db1 = static_cast<const char*>(a.bl) - static_cast<const char*>(a.bh); sgm.seg_db2 = 0;
+      sgm.al = nullptr; sgm.bl = nullptr;
+      if (seg_kernel_ok(sgm)) {
+        gemm_nt(st, SKEMA_PREC_BF16, sgm, nullptr, &t);
+        done = true;
+      }
+    }
+    if (!done) gemm_nt(st, prec, a, &c->gws.buf, &t);
   }
   {
     Timed t(c, "sketch_core");
