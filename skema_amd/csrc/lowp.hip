@@ -1103,10 +1103,11 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
     uint8_t* slot = smem + (st % NS) * STAGE;
     int64_t off = (int64_t)st * SB, offa = (int64_t)st * astep;
     if constexpr (SEG) {
-      // K segments (no K split): the step within its segment, from that
-      // segment's plane
-      const int z = st / g.seg_steps;
-      const int64_t r = (int64_t)(st - z * g.seg_steps) * SB;
+      // K segments: the absolute step's place in its segment, from that
+      // segment's plane (ga and gb already hold this split's start kb0)
+      const int sa = (int)(kb0 / SB) + st;
+      const int z = sa / g.seg_steps;
+      const int64_t r = (int64_t)(sa - z * g.seg_steps) * SB - kb0;
       offa = r + (z == 0 ? g.seg_da0 : (z == 1 ? g.seg_da1 : g.seg_da2));
       off = r + (z == 0 ? g.seg_db0 : (z == 1 ? g.seg_db1 : g.seg_db2));
     }
@@ -1810,9 +1811,11 @@ bool nt_picks_mx(int prec, const NtArgs& a) {
 // it covers that kernel and not the split-K reduction that follows.
 // ws = nullptr: no K split.  fill: as many K splits as fill the 256 CUs
 // once with the tiles (at least the plan's; for a long K over few tiles).
-void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr, bool fill = false) {
+// force: this plan instead of plan_nt's.
+void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr, bool fill = false,
+             const NtPlan* force = nullptr) {
   SKEMA_REQUIRE(!a.a_strip || nt_picks_mx(prec, a), kErrUnsupported, "A in column strips needs the k_mx kernel");
-  NtPlan pl = plan_nt(prec, a.M, a.N, a.K);
+  NtPlan pl = force ? *force : plan_nt(prec, a.M, a.N, a.K);
   SKEMA_REQUIRE(!(a.asc && a.bsc), kErrUnsupported, "block scales on both NT operands");
   if ((a.asc || a.bsc) && pl.tm == 0) pl = plan_ntx(prec, a.M, a.N, a.K);
   SKEMA_REQUIRE(!(a.asc || a.bsc) || (ntx_eligible(prec, a) && (!a.asc || a.koff_a == 0)), kErrUnsupported,
@@ -1849,8 +1852,8 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr,
   // wide kernels: keep the larger operand's rows within an XCD
   a.n_fast = (a.M >= a.N) ? 1 : 0;
   SKEMA_REQUIRE(!a.seg_steps || (prec == SKEMA_PREC_BF16 && pl.ns == 2 && pl.tm == 8 && pl.tn == 4 && pl.wm == 2 &&
-                                 pl.wn == 4 && splits == 1),
-                kErrUnsupported, "K segments need the 256 x 256 bf16 k_mx without a K split");
+                                 pl.wn == 4),
+                kErrUnsupported, "K segments need the 256 x 256 bf16 k_mx");
   const dim3 grid((unsigned)(a.tiles_m * cdiv(a.N, pl.bn)), 1, (unsigned)splits);
   launch_nt(st, prec, a, pl, grid);
   check_launch("k_nt");
@@ -2060,11 +2063,15 @@ NtArgs right_args(const Sketch& sk, int64_t rows, const WinLayout& w, char* base
 
 // Whether a bf16 product (a: K already the three segments' length) runs on
 // the 256 x 256 k_mx, the one kernel that reads K segments.
-bool seg_kernel_ok(const NtArgs& a) {
+bool seg_kernel_ok(const NtArgs& a, bool forced = false) {
   const NtPlan pl = plan_nt(SKEMA_PREC_BF16, a.M, a.N, a.K);
-  return pl.ns == 2 && pl.tm == 8 && pl.tn == 4 && pl.wm == 2 && pl.wn == 4 && a.K % 64 == 0 &&
+  return (forced || (pl.ns == 2 && pl.tm == 8 && pl.tn == 4 && pl.wm == 2 && pl.wn == 4)) && a.K % 64 == 0 &&
          ntx_eligible(SKEMA_PREC_BF16, a);
 }
+// The 256 x 256 bf16 k_mx with a K split of 2: split-BF16's whole-matrix
+// left product as a segmented bf16 product (k x n x 3m: 2 row tiles of the
+// 401 rows of x, 64 column tiles, 256 blocks)
+constexpr NtPlan kSegLeftPlan{0, 0, 2, 8, 4, 2, 2, 4};
 
 // left = false: the left product was done by k_quant_left_fp8; rstrip > 0:
 // R in 64-column strips of rstrip bytes; scale_off: byte offset of this
@@ -2096,7 +2103,23 @@ void window_products(Sketch& sk, int64_t row0, int64_t rows, const WinLayout& w,
     a.ldb = w.ldl;
     a.out.C1 = sk.X.as<double>(); a.out.ldc1 = k; a.out.alpha1 = eta; a.out.beta1 = nu; a.out.nsplit = n;
     a.out.C2 = a.out.C1; a.out.ldc2 = k;
-    gemm_nt(st, prec, a, &c->gws.buf, &t);
+    bool done = false;
+    if (whole && split && rows % 64 == 0) {
+      // split-BF16 over the whole matrix: [Uh | Uh | Ul] x [Lh | Ll | Lh] as
+      // one bf16 product over K segments of the window's rows (see the right
+      // product below), on the 256 x 256 kernel with a K split of 2
+      NtArgs sgm = a;
+      sgm.K = 3 * rows;
+      sgm.seg_steps = (int)(rows / 64);
+      sgm.seg_da0 = 0; sgm.seg_da1 = 0; sgm.seg_da2 = static_cast<const char*>(a.al) - static_cast<const char*>(a.ah);
+      sgm.seg_db0 = 0; sgm.seg_db1 = static_cast<const char*>(a.bl) - static_cast<const char*>(a.bh); sgm.seg_db2 = 0;
+      sgm.al = nullptr; sgm.bl = nullptr;
+      if (seg_kernel_ok(sgm, true)) {
+        gemm_nt(st, SKEMA_PREC_BF16, sgm, &c->gws.buf, &t, false, &kSegLeftPlan);
+        done = true;
+      }
+    }
+    if (!done) gemm_nt(st, prec, a, &c->gws.buf, &t);
   }
   {
     Timed t(c, "sketch_right");

@@ -102,7 +102,7 @@ def test_fp8_one_pass_over_all_windows(S, m, n, window, rank):
 
 
 @pytest.mark.parametrize("form", ["bf16", "bf16x3", "fp32"])
-@pytest.mark.parametrize("m,n,window,rank", [(1154, 700, 256, 10), (16514, 8192, 4096, 100)])
+@pytest.mark.parametrize("m,n,window,rank", [(1154, 700, 256, 10), (16514, 8192, 4096, 100), (8192, 4096, 2048, 100)])
 def test_whole_matrix_update_nu1(S, form, m, n, window, rank):
     """nu = 1 with several windows: bf16 and split-BF16 quantise the whole
     matrix once (their rounding is elementwise, so windows change nothing
@@ -110,7 +110,10 @@ def test_whole_matrix_update_nu1(S, form, m, n, window, rank):
     over all rows (fp32 keeps the window loop, checked here the same way):
     (1154, 700, 256) ragged in every dimension;
     (16514, 8192, 4096) at rank 100 (k = 401, s = 803) with the right product
-    unsplit over 16514 rows and the core's K split filling the CUs.  A first
+    unsplit over 16514 rows and the core's K split filling the CUs (for
+    split-BF16 the right product as one bf16 product over three K segments);
+    (8192, 4096, 2048) at rank 100: m a multiple of 64, so split-BF16's left
+    product also runs as a segmented bf16 product (K split 2).  A first
     update with another matrix must leave no trace (X, Y and Z are written,
     not accumulated).  Every row and column of X and Y is checked."""
     print(f"case whole-matrix {form} {m}x{n} w={window} rank={rank}", flush=True)
