@@ -2275,6 +2275,8 @@ bool update_dense_all_lowp(Sketch& sk, const double* A, int64_t lda) {
   if (rstrip) w.re = (size_t)(w.ldr / 64) * rstrip;
   w.o_rmax = w.o_cmax = w.o_g = align_up(w.re * w.esz, 256);
   w.total = w.o_g + align_up(w.ge * w.gsz, 256);
+  // R and G of the whole matrix beyond kWholeCopyBytes: the window loop
+  if (w.total > kWholeCopyBytes) return false;
   sk.win.reserve(w.total);
   char* base = sk.win.as<char>();
   // unit scales (0x7f) in the K padding of R's rows; refilled when the
