@@ -1099,6 +1099,9 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
     const int64_t ld = ASC ? g.ldas : g.ldbs;
     gs[i] = (ASC ? g.asc : g.bsc) + row * ld + (lane & 3) * (ld / 4) + kb0 / SB;
   }
+  // fp8 with A scales only, B fragments held: the A scale dwords (four
+  // steps each) are copied and read from LDS once per four steps (sw)
+  constexpr bool kScaleSel = FORM == kNxFp8 && ASC && !BSC && TM > TN && NS == 2;
   auto issue = [&](int st) {
     uint8_t* slot = smem + (st % NS) * STAGE;
     int64_t off = (int64_t)st * SB, offa = (int64_t)st * astep;
@@ -1130,10 +1133,12 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
                                          (SKEMA_LDS void*)(slot + LOOFF + BM * SB + 1024 * (wave * IB + j)), 16,
                                          0, 0);
     }
+    if (!kScaleSel || (st & 3) == 0) {
 #pragma unroll
-    for (int i = 0; i < SC; ++i)
-      __builtin_amdgcn_global_load_lds((const void*)(gs[i] + 4 * (st >> 2)),
-                                       (SKEMA_LDS void*)(slot + SCOFF + 256 * (wave * SC + i)), 4, 0, 0);
+      for (int i = 0; i < SC; ++i)
+        __builtin_amdgcn_global_load_lds((const void*)(gs[i] + 4 * (st >> 2)),
+                                         (SKEMA_LDS void*)(slot + SCOFF + 256 * (wave * SC + i)), 4, 0, 0);
+    }
   };
 
   f4v acc[TM][TN];
@@ -1154,6 +1159,9 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   for (int p = 0; p < NS - 1; ++p)
     if (p < nsteps) issue(p);
 
+  int sw[TM];
+#pragma unroll
+  for (int x = 0; x < TM; ++x) sw[x] = 0x7f7f7f7f;
   for (int st = 0; st < nsteps; ++st) {
     // slot st is complete once at most the DMA groups issued after it are
     // outstanding (fewer of those near the end)
@@ -1210,6 +1218,25 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
           bf2(acc[x][y], ah[x], bl);
           bf2(acc[x][y], al[x], bh);
         }
+      }
+    } else if constexpr (kScaleSel) {
+      // A's scale dword of a row tile holds the bytes of four steps: read
+      // from LDS once every four steps (splits start on a 4-step boundary),
+      // the step's byte shifted out of the register
+      if ((st & 3) == 0) {
+#pragma unroll
+        for (int x = 0; x < TM; ++x) sw[x] = (int)ts[(wm * 16 * TM + 16 * x + l16) * 4 + q];
+      }
+      i8v bfr[TN];
+#pragma unroll
+      for (int y = 0; y < TN; ++y) bfr[y] = frag(tB, 16 * y + l16);
+#pragma unroll
+      for (int x = 0; x < TM; ++x) {
+        const i8v af = frag(tA, 16 * x + l16);
+        const int sx = (int)(((uint32_t)sw[x] >> sh) & 0xffu);
+#pragma unroll
+        for (int y = 0; y < TN; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af, bfr[y], acc[x][y], 0, 0, 0, sx, 0, 127);
       }
     } else if constexpr (TM <= TN) {
       i8v af[TM];
