@@ -1039,7 +1039,12 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
   // split-BF16 carries both planes of both operands: A hi | B hi | A lo | B lo
   constexpr int PL = FORM == kNxBf16x3 ? 2 : 1;
   constexpr int D = PL * (IA + IB) + SC;  // DMA instructions per wave and step
-  constexpr int LOOFF = (BM + BN) * SB, SCOFF = PL * LOOFF, STAGE = SCOFF + SR * 16;
+  // fp8 with A scales only, B fragments held (the right product): a row's
+  // scale bytes of sixteen steps (16 bytes per lane group, four dwords of
+  // four steps) are copied once per sixteen steps into the scale area of
+  // slot 0, read from there once per four steps (sw) and shifted per step
+  constexpr bool kScaleSel = FORM == kNxFp8 && ASC && !BSC && TM > TN && NS == 2;
+  constexpr int LOOFF = (BM + BN) * SB, SCOFF = PL * LOOFF, STAGE = SCOFF + SR * (kScaleSel ? 64 : 16);
   static_assert(NS >= 2 && NS <= 4 && (NS - 2) * D < 64, "ring depth");
   __shared__ __attribute__((aligned(1024))) uint8_t smem[NS * STAGE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, q = lane >> 4;
@@ -1099,9 +1104,6 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
     const int64_t ld = ASC ? g.ldas : g.ldbs;
     gs[i] = (ASC ? g.asc : g.bsc) + row * ld + (lane & 3) * (ld / 4) + kb0 / SB;
   }
-  // fp8 with A scales only, B fragments held: the A scale dwords (four
-  // steps each) are copied and read from LDS once per four steps (sw)
-  constexpr bool kScaleSel = FORM == kNxFp8 && ASC && !BSC && TM > TN && NS == 2;
   auto issue = [&](int st) {
     uint8_t* slot = smem + (st % NS) * STAGE;
     int64_t off = (int64_t)st * SB, offa = (int64_t)st * astep;
@@ -1133,7 +1135,16 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
                                          (SKEMA_LDS void*)(slot + LOOFF + BM * SB + 1024 * (wave * IB + j)), 16,
                                          0, 0);
     }
-    if (!kScaleSel || (st & 3) == 0) {
+    if constexpr (kScaleSel) {
+      // sixteen steps of (row, lane group) scale bytes: 16-byte aligned
+      // (splits start on a 16-step boundary, lane-group rows padded to 16)
+      if ((st & 15) == 0) {
+#pragma unroll
+        for (int i = 0; i < SC; ++i)
+          __builtin_amdgcn_global_load_lds((const void*)(gs[i] + st), (SKEMA_LDS void*)(slot + SCOFF + 1024 * (wave * SC + i)),
+                                           16, 0, 0);
+      }
+    } else {
 #pragma unroll
       for (int i = 0; i < SC; ++i)
         __builtin_amdgcn_global_load_lds((const void*)(gs[i] + 4 * (st >> 2)),
@@ -1221,11 +1232,17 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_mx(NtArgs g) {
       }
     } else if constexpr (kScaleSel) {
       // A's scale dword of a row tile holds the bytes of four steps: read
-      // from LDS once every four steps (splits start on a 4-step boundary),
-      // the step's byte shifted out of the register
+      // from LDS once every four steps, the step's byte shifted out of the
+      // register
       if ((st & 3) == 0) {
+        // the sixteen-step record (row, q) in slot 0's scale area, this
+        // four-step dword of it
+        const uint32_t* sr = reinterpret_cast<const uint32_t*>(smem + SCOFF) + ((st & 15) >> 2);
 #pragma unroll
-        for (int x = 0; x < TM; ++x) sw[x] = (int)ts[(wm * 16 * TM + 16 * x + l16) * 4 + q];
+        for (int x = 0; x < TM; ++x) sw[x] = (int)sr[((wm * 16 * TM + 16 * x + l16) * 4 + q) * 4];
+        // read before the copy of the next sixteen steps may land (three
+        // barriers later)
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
       }
       i8v bfr[TN];
 #pragma unroll
@@ -1866,8 +1883,9 @@ void gemm_nt(hipStream_t st, int prec, NtArgs a, DevBuf* ws, Timed* t = nullptr,
     const int64_t tiles = cdiv(a.M, pl.bm) * cdiv(a.N, pl.bn);
     splits = (int)std::max<int64_t>(splits, std::min<int64_t>(256 / std::max<int64_t>(tiles, 1), a.K / (8 * kstep)));
   }
-  // block-scaled operands: splits start on a 4-step boundary (one scale dword)
-  const int64_t kgrain = (a.asc || a.bsc) ? 4 * kstep : kstep;
+  // block-scaled operands: splits start on a 4-step boundary (one scale
+  // dword), 16 for A scales (k_mx copies sixteen steps' bytes at once)
+  const int64_t kgrain = a.asc ? 16 * kstep : (a.bsc ? 4 * kstep : kstep);
   a.k_per_split = cdiv(cdiv(a.K, splits), kgrain) * kgrain;
   splits = (int)cdiv(a.K, a.k_per_split);
   a.part = nullptr;
@@ -2000,9 +2018,10 @@ WinLayout win_layout(const Sketch& sk, int64_t row0, int64_t rows) {
   w.ldr = cdiv(n, 128) * 128;
   w.esz = w.fp8 ? 1 : (w.f32 ? 4 : 2);
   w.planes = w.split ? 2 : 1;
-  // scale rows: 4 groups of ceil(steps / 4) * 4 bytes (steps of 128 elements)
+  // scale rows: 4 groups of one byte per step of 128 elements, padded to
+  // 4 bytes (L) or 16 (R: k_mx copies sixteen steps of R's scales at once)
   w.ldls = 4 * align_up((size_t)cdiv(rows, 128), 4);
-  w.ldrs = 4 * align_up((size_t)cdiv(n, 128), 4);
+  w.ldrs = 4 * align_up((size_t)cdiv(n, 128), 16);
   w.le = (size_t)n * w.ldl;
   w.re = (size_t)(rows + 1) * w.ldr;  // one spare row: the sink of k_quant_left_fp8's out-of-window stores
   w.ge = (size_t)s * w.ldl;
